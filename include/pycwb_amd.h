@@ -1,0 +1,137 @@
+/*
+ * pycwb_amd -- C ABI of the MI355X-native coherent sky-scan likelihood and WDM
+ * forward transform for PycWB.
+ *
+ * Every pointer argument is a device pointer (HBM) unless stated otherwise;
+ * `stream` is a hipStream_t passed as void*.  Calls enqueue work on `stream` and
+ * return without synchronising.  No entry point keeps global mutable state, so
+ * concurrent calls on different streams with different buffers are safe.
+ * Return value: PYCWB_OK, or PYCWB_ERR_ARG (shapes / pointers rejected before
+ * any launch), or PYCWB_ERR_HIP (a HIP runtime call failed).
+ *
+ * Array layouts follow the reference's Python call sites:
+ *   FP, FX  (n_sky, n_ifo) float32  -- setup["FP_t"], setup["FX_t"]
+ *   ml      (n_ifo, n_sky) int32    -- setup["ml"]
+ *   rms     (n_pix, n_ifo) float32  -- noise_weights.T
+ *   td00/90 (n_delay, n_ifo, n_pix) float32, n_delay = 2K+1 odd
+ *   maps    (11, n_sky) float32, in the order of the reference's returned tuple:
+ *           nAntennaPrior, nAlignment, nLikelihood, nNullEnergy, nCorrEnergy,
+ *           nCorrelation, nSkyStat, nDisbalance, nNetIndex, nEllipticity,
+ *           nPolarisation
+ * Preconditions the host checks (the kernels only clamp for memory safety):
+ *   0 <= sky_valid_indices[k] < n_sky, |ml| <= K, 1 <= n_ifo <= 8.
+ */
+#ifndef PYCWB_AMD_H
+#define PYCWB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYCWB_OK 0
+#define PYCWB_ERR_ARG 1
+#define PYCWB_ERR_HIP 2
+
+const char* pycwb_version(void);
+
+/* Bytes of device workspace needed by pycwb_td_relayout / pycwb_sky_scan_plain
+ * for one cluster (256 bytes of counters + the packed amplitude table). */
+size_t pycwb_sky_scan_workspace_bytes(int n_ifo, int n_pix, int n_delay);
+
+/* Packs td00/td90 (reference layout) into the workspace's (n_pix, n_ifo,
+ * n_delay) {00,90} pair table read by the scan.  Replaces the per-cluster
+ * transposes at pycwb/modules/likelihoodWP/likelihood.py:299-305. */
+int pycwb_td_relayout(int n_ifo, int n_pix, int n_delay, const float* td00, const float* td90,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* DPF regulator pass.  Replaces calculate_dpf
+ * (pycwb/modules/likelihoodWP/dpf.py:7-34; called at likelihood.py:310).
+ * Writes REG[0] = reg0 (delta*sqrt(2) as float32), REG[1] = (float)regulator,
+ * REG[2] = 0, and the unrounded float64 regulator to *reg_f64 when non-NULL.
+ * workspace needs >= 256 bytes. */
+int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, const float* FX,
+                        const float* rms, double gamma_regulator,
+                        double network_energy_threshold, float reg0,
+                        const int64_t* sky_valid_indices, int64_t n_valid, float* REG,
+                        double* reg_f64, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
+/* Sky scan of one cluster.  Replaces scan_sky_for_best_fit
+ * (pycwb/modules/likelihoodWP/sky_scan.py:17-192; called at likelihood.py:319).
+ * Requires pycwb_td_relayout on the same workspace first.  maps may be NULL
+ * (argmax only).  Writes *l_max (int64) and *sky_stat_max (float64) on the
+ * device.  A direction with no pixel above the energy threshold is treated as
+ * gated (the reference divides 0/0 there). */
+int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const float* FP,
+                         const float* FX, const int32_t* ml, const float* rms, const float* REG,
+                         double netCC, double network_energy_threshold,
+                         const int64_t* sky_valid_indices, int64_t n_valid, float* maps,
+                         int64_t* l_max, double* sky_stat_max, void* workspace,
+                         size_t workspace_bytes, void* stream);
+
+/* Batched regulator + scan + argmax over many clusters, as one hipGraph of
+ * persistent work-queue kernels.  Replaces the per-cluster loop of
+ * evaluate_fragment_clusters / likelihood_wrapper
+ * (pycwb/modules/likelihoodWP/likelihood.py:58-135) up to the scan.
+ * Clusters are listed in `order`, grouped by detector count: group g covers
+ * order[group_first[g] .. group_first[g]+group_count[g]) whose clusters all
+ * have group_nifo[g] detectors.  Per cluster c:
+ *   rms at rms + rms_off[c]  (n_pix[c], n) float32
+ *   td  at td + 2*td_off[c]  (n_pix[c], n, 2K[c]+1) {00,90} float pairs
+ *   network net[c]: FP/FX at FP/FX + net_off[net] (n_sky, n), ml at
+ *   ml + net_off[net] (n, n_sky), energy threshold net_En[net].
+ * Scratch: tickets (2*n_groups uint32), counts, best (per cluster).
+ * Outputs per cluster: REG1 (float32), l_max (int64), sky_stat_max (float64),
+ * reg_f64 (float64, may be NULL). */
+typedef struct pycwb_batch_desc {
+  int n_clusters;
+  int64_t n_sky;
+  const int64_t* sky_valid_indices;
+  int64_t n_valid;
+  const int32_t* order;
+  int n_groups;
+  int group_first[8];
+  int group_count[8];
+  int group_nifo[8];
+  const int32_t* n_pix;
+  const int32_t* net;
+  const int32_t* K;
+  const int64_t* rms_off;
+  const int64_t* td_off;
+  const float* rms;
+  const float* td;
+  const int32_t* net_nifo;
+  const int64_t* net_off;
+  const double* net_En;
+  const float* FP;
+  const float* FX;
+  const int32_t* ml;
+  double gamma_regulator;
+  float reg0;
+  double netCC;
+  uint32_t* tickets;
+  uint32_t* counts;
+  unsigned long long* best;
+  float* REG1;
+  int64_t* l_max;
+  double* sky_stat_max;
+  double* reg_f64;
+  int block;      /* sky directions per workgroup: 64, 128 or 256 */
+  int grid;       /* persistent workgroups */
+  int use_graph;  /* capture the launches in a hipGraph */
+} pycwb_batch_desc;
+
+typedef struct pycwb_batch pycwb_batch;
+
+int pycwb_batch_create(const pycwb_batch_desc* desc, pycwb_batch** out);
+int pycwb_batch_launch(pycwb_batch* batch, void* stream);
+int pycwb_batch_destroy(pycwb_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYCWB_AMD_H */

@@ -1,0 +1,115 @@
+"""ctypes binding of libpycwb_amd.so (the C ABI in include/pycwb_amd.h).
+
+The library is built in-tree (``pycwb_amd/libpycwb_amd.so``).  There is no CPU
+fallback: importing a GPU entry point without the library, or calling one
+without a ROCm device, raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpycwb_amd.so")
+
+PYCWB_OK = 0
+PYCWB_ERR_ARG = 1
+PYCWB_ERR_HIP = 2
+
+_lock = threading.Lock()
+_lib = None
+
+vp = C.c_void_p
+i32 = C.c_int
+i64 = C.c_int64
+f32 = C.c_float
+f64 = C.c_double
+sz = C.c_size_t
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [
+        ("n_clusters", i32),
+        ("n_sky", i64),
+        ("sky_valid_indices", vp),
+        ("n_valid", i64),
+        ("order", vp),
+        ("n_groups", i32),
+        ("group_first", i32 * 8),
+        ("group_count", i32 * 8),
+        ("group_nifo", i32 * 8),
+        ("n_pix", vp),
+        ("net", vp),
+        ("K", vp),
+        ("rms_off", vp),
+        ("td_off", vp),
+        ("rms", vp),
+        ("td", vp),
+        ("net_nifo", vp),
+        ("net_off", vp),
+        ("net_En", vp),
+        ("FP", vp),
+        ("FX", vp),
+        ("ml", vp),
+        ("gamma_regulator", f64),
+        ("reg0", f32),
+        ("netCC", f64),
+        ("tickets", vp),
+        ("counts", vp),
+        ("best", vp),
+        ("REG1", vp),
+        ("l_max", vp),
+        ("sky_stat_max", vp),
+        ("reg_f64", vp),
+        ("block", i32),
+        ("grid", i32),
+        ("use_graph", i32),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "pycwb_version": (C.c_char_p, []),
+    "pycwb_sky_scan_workspace_bytes": (sz, [i32, i32, i32]),
+    "pycwb_td_relayout": (i32, [i32, i32, i32, vp, vp, vp, sz, vp]),
+    "pycwb_dpf_regulator": (i32, [i32, i32, i64, vp, vp, vp, f64, f64, f32, vp, i64, vp, vp,
+                                  vp, sz, vp]),
+    "pycwb_sky_scan_plain": (i32, [i32, i32, i64, i32, vp, vp, vp, vp, vp, f64, f64, vp, i64,
+                                   vp, vp, vp, vp, sz, vp]),
+    "pycwb_batch_create": (i32, [C.POINTER(BatchDesc), C.POINTER(vp)]),
+    "pycwb_batch_launch": (i32, [vp, vp]),
+    "pycwb_batch_destroy": (i32, [vp]),
+}
+
+
+def load():
+    """Load the library once; raise if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                "g.build()'` (hipcc --offload-arch=gfx950); there is no CPU fallback")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+class PycwbError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    if rc == PYCWB_OK:
+        return
+    if rc == PYCWB_ERR_ARG:
+        raise ValueError(f"{what}: arguments rejected by the C ABI")
+    raise PycwbError(f"{what}: HIP runtime error (code {rc})")

@@ -1,0 +1,674 @@
+// Coherent sky-scan likelihood on MI355X (gfx950): DPF regulator pass, sky scan
+// and argmax, as a plain (one cluster per launch) family and as a batched
+// persistent work-queue family captured in a hipGraph.
+//
+// Data layout in HBM
+//   sky tables   FP, FX : (S, n) float32 row-major (setup FP_t / FX_t)
+//                ml     : (n, S) int32 delay taps
+//   cluster      rms    : (P, n) float32
+//                td     : (P, n, 2K+1) float2 {phase 00, phase 90}; one lane reads
+//                         the 8-byte pair at its own delay tap, and the taps that
+//                         neighbouring sky directions need sit in the same lines.
+//   batch        all clusters' rms / td concatenated, addressed by 64-bit offsets.
+//
+// One lane = one sky direction; the pixel sweep is sequential per lane, so all
+// per-direction sums have the reference's forward pixel order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "skyscan_lane.hh"
+#include "pycwb_amd.h"
+
+using namespace pycwb;
+
+namespace {
+
+constexpr int kMaxIfo = 8;
+
+#define HIP_TRY(expr)                                  \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return PYCWB_ERR_HIP;        \
+  } while (0)
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide max of a key, one atomicMax per block.
+template <int BLOCK>
+__device__ __forceinline__ void block_argmax(unsigned long long key, unsigned long long* dst) {
+  __shared__ unsigned long long s_key[BLOCK / 64];
+  key = wave_max_u64(key);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_key[wave] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long k = s_key[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) k = s_key[w] > k ? s_key[w] : k;
+    if (k) atomicMax(dst, k);
+  }
+  __syncthreads();
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void block_count(unsigned v, unsigned* dst) {
+  __shared__ unsigned s_cnt[BLOCK / 64];
+  v = wave_sum_u32(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_cnt[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) t += s_cnt[w];
+    if (t) atomicAdd(dst, t);
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Per-tile bodies shared by the plain and batched kernels.
+// ---------------------------------------------------------------------------
+
+// DPF quality of the direction valid[k]; returns 1 if quality > gamma.
+template <int N>
+__device__ __forceinline__ unsigned regulator_lane(const float* __restrict__ FP,
+                                                   const float* __restrict__ FX,
+                                                   const float* __restrict__ rms, int P,
+                                                   int64_t s, double gamma) {
+  float fp0[N], fx0[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    fp0[i] = FP[s * N + i];
+    fx0[i] = FX[s * N + i];
+  }
+  float NI = 0.f;
+  unsigned NN = 0;
+  for (int j = 0; j < P; ++j) {
+    float r[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] = rms[(int64_t)j * N + i];
+    regulator_pixel<N>(r, fp0, fx0, NI, NN);
+  }
+  const double q = __builtin_sqrt((double)NI / ((double)NN + 0.01));
+  return q > gamma ? 1u : 0u;
+}
+
+// Sky scan of direction s; returns the argmax key for position k.
+template <int N>
+__device__ __forceinline__ unsigned long long scan_lane(
+    const float* __restrict__ FP, const float* __restrict__ FX, const int32_t* __restrict__ ml,
+    int64_t S, const float* __restrict__ rms, const float2* __restrict__ td, int P, int K,
+    float rr, float RR, double En, double netCC, int64_t s, int64_t k, bool active,
+    float* __restrict__ maps) {
+  const int nd = 2 * K + 1;
+  float fp0[N], fx0[N];
+  int tap[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    fp0[i] = FP[s * N + i];
+    fx0[i] = FX[s * N + i];
+    int t = ml[(int64_t)i * S + s] + K;
+    t = t < 0 ? 0 : (t >= nd ? nd - 1 : t);  // host validates |ml| <= K
+    tap[i] = i * nd + t;
+  }
+  SkySums acc;
+  sums_init(acc);
+  const int64_t pstride = (int64_t)N * nd;
+  // software pipeline: the next pixel's amplitudes load while this one computes
+  float2 cur[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) cur[i] = td[tap[i]];
+  for (int j = 0; j < P; ++j) {
+    float2 nxt[N];
+    const int jn = j + 1 < P ? j + 1 : j;
+#pragma unroll
+    for (int i = 0; i < N; ++i) nxt[i] = td[(int64_t)jn * pstride + tap[i]];
+    float v00[N], v90[N], r[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      v00[i] = cur[i].x;
+      v90[i] = cur[i].y;
+      r[i] = rms[(int64_t)j * N + i];
+    }
+    scan_pixel<N>(v00, v90, r, fp0, fx0, rr, RR, En, acc);
+#pragma unroll
+    for (int i = 0; i < N; ++i) cur[i] = nxt[i];
+  }
+  SkyStat st;
+  sky_epilogue(acc, N, netCC, st);
+  if (maps && active) {
+#pragma unroll
+    for (int m = 0; m < kNumMaps; ++m) maps[(int64_t)m * S + s] = st.gated ? 0.f : st.maps[m];
+  }
+  if (!active || st.gated) return 0ull;
+  return argmax_key(st.AA, (uint32_t)k);
+}
+
+// Regulator formula (dpf.py:31-34), float64.
+__device__ __forceinline__ double regulator_value(int64_t n_valid, unsigned count, double En) {
+  const double FF = (double)n_valid, ff = (double)count;
+  return (FF * FF / (ff * ff + 1.e-9) - 1) * En;
+}
+
+// ---------------------------------------------------------------------------
+// Plain family: one cluster per launch.
+// ---------------------------------------------------------------------------
+
+// Re-lay td00/td90 from the reference's (2K+1, n, P) planes into (P, n, 2K+1)
+// float2 pairs, and clear the per-call counters.
+__global__ void k_relayout_td(const float* __restrict__ td00, const float* __restrict__ td90,
+                              int n, int P, int nd, float2* __restrict__ out,
+                              unsigned* __restrict__ counters, int n_counters) {
+  const int64_t total = (int64_t)n * P * nd;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < n_counters) counters[gid] = 0u;
+  // gid enumerates the source (tap, ifo, pixel) so the loads are coalesced
+  for (int64_t e = gid; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e % P;
+    const int64_t rest = e / P;
+    const int64_t i = rest % n;
+    const int64_t t = rest / n;
+    out[(j * n + i) * nd + t] = make_float2(td00[e], td90[e]);
+  }
+}
+
+__global__ void k_fill(float* __restrict__ p, int64_t count, float v) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t e = gid; e < count; e += (int64_t)gridDim.x * blockDim.x) p[e] = v;
+}
+
+template <int N, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_regulator_plain(
+    const float* __restrict__ FP, const float* __restrict__ FX, const float* __restrict__ rms,
+    int P, const int64_t* __restrict__ valid, int64_t n_valid, double gamma,
+    unsigned* __restrict__ count) {
+  const int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  unsigned v = 0;
+  if (k < n_valid) v = regulator_lane<N>(FP, FX, rms, P, valid[k], gamma);
+  block_count<BLOCK>(v, count);
+}
+
+__global__ void k_regulator_finish(const unsigned* __restrict__ count, int64_t n_valid,
+                                   double En, float rr, float* __restrict__ REG,
+                                   double* __restrict__ reg_f64) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double v = regulator_value(n_valid, *count, En);
+    if (reg_f64) *reg_f64 = v;
+    REG[0] = rr;
+    REG[1] = (float)v;
+    REG[2] = 0.f;
+  }
+}
+
+template <int N, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_scan_plain(
+    const float* __restrict__ FP, const float* __restrict__ FX, const int32_t* __restrict__ ml,
+    int64_t S, const float* __restrict__ rms, const float2* __restrict__ td, int P, int K,
+    const float* __restrict__ REG, double En, double netCC, const int64_t* __restrict__ valid,
+    int64_t n_valid, float* __restrict__ maps, unsigned long long* __restrict__ best) {
+  const int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = k < n_valid;
+  const int64_t s = valid[active ? k : 0];
+  const unsigned long long key =
+      scan_lane<N>(FP, FX, ml, S, rms, td, P, K, REG[0], REG[1], En, netCC, s, k, active, maps);
+  block_argmax<BLOCK>(key, best);
+}
+
+// l_max / sky_stat_max from the packed key (sky_scan.py:183-189).
+__global__ void k_argmax_finish(const unsigned long long* __restrict__ best,
+                                const int64_t* __restrict__ valid, int64_t* __restrict__ l_max,
+                                double* __restrict__ stat_max) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const unsigned long long k = *best;
+    const float v = k ? key_value(k) : -1.f;
+    if (k && v >= 0.f) {
+      *l_max = valid[(uint32_t)(k & 0xffffffffu)];
+      *stat_max = (double)v;
+    } else {
+      *l_max = valid[0];
+      *stat_max = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched family.  Clusters are grouped by detector count n (one persistent
+// kernel instantiation per n, so each keeps the register budget of its own n).
+// Inside a group, persistent blocks pull (cluster, sky tile) items from a
+// ticket counter; tickets are cluster-major in the group's order (largest
+// work first), so concurrent blocks share one cluster's amplitudes in L2.
+// ---------------------------------------------------------------------------
+struct BatchDev {
+  int n_clusters;
+  int tiles_per_cluster;
+  int64_t S;
+  const int64_t* valid;
+  int64_t n_valid;
+  const int32_t* order;     // processing order, grouped by n
+  const int32_t* npix;      // pixels per cluster
+  const int32_t* net;       // network id per cluster
+  const int32_t* K;         // delay half-width per cluster
+  const int64_t* rms_off;   // float offset into rms
+  const int64_t* td_off;    // float2 offset into td
+  const float* rms;
+  const float2* td;
+  const int32_t* net_nifo;  // ifos per network
+  const int64_t* net_off;   // element offset of the network's FP/FX (S, n) and ml (n, S)
+  const double* net_En;     // network energy threshold per network
+  const float* FP;
+  const float* FX;
+  const int32_t* ml;
+  double gamma;
+  float rr;
+  double netCC;
+  unsigned* tickets;        // 2 per group: regulator, scan
+  unsigned* counts;         // per cluster regulator count
+  unsigned long long* best; // per cluster argmax key
+  float* REG1;              // per cluster REG[1] (float32)
+  int64_t* l_max;
+  double* stat_max;
+  double* reg_f64;
+};
+
+__global__ void k_batch_init(BatchDev b, int n_tickets) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < n_tickets) b.tickets[gid] = 0u;
+  for (int64_t c = gid; c < b.n_clusters; c += (int64_t)gridDim.x * blockDim.x) {
+    b.counts[c] = 0u;
+    b.best[c] = 0ull;
+  }
+}
+
+template <int N, int BLOCK, int PHASE>
+__global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev b, int first, int count,
+                                                            unsigned* ticket) {
+  __shared__ unsigned s_item;
+  const unsigned n_items = (unsigned)count * (unsigned)b.tiles_per_cluster;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const unsigned item = s_item;
+    __syncthreads();
+    if (item >= n_items) break;  // every block reaches this once the tickets run out
+    const int c = b.order[first + (int)(item / (unsigned)b.tiles_per_cluster)];
+    const int tile = (int)(item % (unsigned)b.tiles_per_cluster);
+    const int64_t k = (int64_t)tile * BLOCK + threadIdx.x;
+    const bool active = k < b.n_valid;
+    const int64_t s = b.valid[active ? k : 0];
+    const int net = b.net[c];
+    const float* FP = b.FP + b.net_off[net];
+    const float* FX = b.FX + b.net_off[net];
+    const float* rms = b.rms + b.rms_off[c];
+    const int P = b.npix[c];
+    if (PHASE == 0) {
+      const unsigned v = active ? regulator_lane<N>(FP, FX, rms, P, s, b.gamma) : 0u;
+      block_count<BLOCK>(v, &b.counts[c]);
+    } else {
+      const double En = b.net_En[net];
+      const float RR = (float)regulator_value(b.n_valid, b.counts[c], En);
+      const int32_t* ml = b.ml + b.net_off[net];
+      const float2* td = b.td + b.td_off[c];
+      const unsigned long long key = scan_lane<N>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR,
+                                                  En, b.netCC, s, k, active, nullptr);
+      block_argmax<BLOCK>(key, &b.best[c]);
+    }
+  }
+}
+
+__global__ void k_batch_finish(BatchDev b) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.n_clusters) return;
+  const int net = b.net[c];
+  const double reg = regulator_value(b.n_valid, b.counts[c], b.net_En[net]);
+  b.REG1[c] = (float)reg;
+  if (b.reg_f64) b.reg_f64[c] = reg;
+  const unsigned long long k = b.best[c];
+  const float v = k ? key_value(k) : -1.f;
+  if (k && v >= 0.f) {
+    b.l_max[c] = b.valid[(uint32_t)(k & 0xffffffffu)];
+    b.stat_max[c] = (double)v;
+  } else {
+    b.l_max[c] = b.valid[0];
+    b.stat_max[c] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+int launch_regulator_plain(int n, const float* FP, const float* FX, const float* rms, int P,
+                           const int64_t* valid, int64_t n_valid, double gamma,
+                           unsigned* count, hipStream_t st) {
+  const unsigned grid = (unsigned)((n_valid + BLOCK - 1) / BLOCK);
+  switch (n) {
+#define CASE(NN)                                                                          \
+  case NN:                                                                                \
+    hipLaunchKernelGGL((k_regulator_plain<NN, BLOCK>), dim3(grid), dim3(BLOCK), 0, st, FP, \
+                       FX, rms, P, valid, n_valid, gamma, count);                         \
+    break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: return PYCWB_ERR_ARG;
+  }
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
+template <int BLOCK>
+int launch_scan_plain(int n, const float* FP, const float* FX, const int32_t* ml, int64_t S,
+                      const float* rms, const float2* td, int P, int K, const float* REG,
+                      double En, double netCC, const int64_t* valid, int64_t n_valid,
+                      float* maps, unsigned long long* best, hipStream_t st) {
+  const unsigned grid = (unsigned)((n_valid + BLOCK - 1) / BLOCK);
+  switch (n) {
+#define CASE(NN)                                                                          \
+  case NN:                                                                                \
+    hipLaunchKernelGGL((k_scan_plain<NN, BLOCK>), dim3(grid), dim3(BLOCK), 0, st, FP, FX, \
+                       ml, S, rms, td, P, K, REG, En, netCC, valid, n_valid, maps, best); \
+    break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: return PYCWB_ERR_ARG;
+  }
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
+template <int BLOCK, int PHASE>
+int launch_group(int n, const BatchDev& d, int grid, int first, int count, unsigned* ticket,
+                 hipStream_t st) {
+  switch (n) {
+#define CASE(NN)                                                                           \
+  case NN:                                                                                 \
+    hipLaunchKernelGGL((k_batch_persistent<NN, BLOCK, PHASE>), dim3(grid), dim3(BLOCK), 0, \
+                       st, d, first, count, ticket);                                       \
+    break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: return PYCWB_ERR_ARG;
+  }
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
+// Workspace of the plain entry points: 256 bytes of counters, then packed td.
+constexpr size_t kCounterBytes = 256;
+
+size_t plain_ws_bytes(int n, int P, int nd) {
+  return kCounterBytes + (size_t)n * P * nd * sizeof(float2);
+}
+
+struct PlainWs {
+  unsigned long long* best;  // argmax key
+  unsigned* count;           // regulator count
+  float2* td;
+};
+
+PlainWs plain_ws(void* ws) {
+  PlainWs w;
+  w.best = (unsigned long long*)ws;
+  w.count = (unsigned*)((char*)ws + 8);
+  w.td = (float2*)((char*)ws + kCounterBytes);
+  return w;
+}
+
+bool valid_args(int n, int P, int64_t S, int64_t n_valid) {
+  return n >= 1 && n <= kMaxIfo && P >= 1 && S >= 1 && n_valid >= 1 && n_valid <= S &&
+         n_valid < (int64_t)0xffffffff;
+}
+
+unsigned grid_for(int64_t count, int64_t cap) {
+  int64_t g = (count + 255) / 256;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI (declared in include/pycwb_amd.h)
+// ===========================================================================
+struct pycwb_batch {
+  BatchDev dev;
+  int block;
+  int grid;
+  int n_groups;
+  int group_first[kMaxIfo];
+  int group_count[kMaxIfo];
+  int group_nifo[kMaxIfo];
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" {
+
+size_t pycwb_sky_scan_workspace_bytes(int n_ifo, int n_pix, int n_delay) {
+  return plain_ws_bytes(n_ifo, n_pix, n_delay);
+}
+
+int pycwb_td_relayout(int n_ifo, int n_pix, int n_delay, const float* td00, const float* td90,
+                      void* workspace, size_t workspace_bytes, void* stream) {
+  if (n_ifo < 1 || n_ifo > kMaxIfo || n_pix < 1 || n_delay < 1 || !(n_delay & 1))
+    return PYCWB_ERR_ARG;
+  if (!td00 || !td90 || !workspace) return PYCWB_ERR_ARG;
+  if (workspace_bytes < plain_ws_bytes(n_ifo, n_pix, n_delay)) return PYCWB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  PlainWs w = plain_ws(workspace);
+  const int64_t total = (int64_t)n_ifo * n_pix * n_delay;
+  hipLaunchKernelGGL(k_relayout_td, dim3(grid_for(total, 65536)), dim3(256), 0, st, td00, td90,
+                     n_ifo, n_pix, n_delay, w.td, (unsigned*)w.best, 4);
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
+int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, const float* FX,
+                        const float* rms, double gamma_regulator,
+                        double network_energy_threshold, float reg0,
+                        const int64_t* sky_valid_indices, int64_t n_valid, float* REG,
+                        double* reg_f64, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+  if (!valid_args(n_ifo, n_pix, n_sky, n_valid)) return PYCWB_ERR_ARG;
+  if (!FP || !FX || !rms || !sky_valid_indices || !REG || !workspace) return PYCWB_ERR_ARG;
+  if (workspace_bytes < kCounterBytes) return PYCWB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  PlainWs w = plain_ws(workspace);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (float*)w.count, (int64_t)1, 0.f);
+  HIP_TRY(hipGetLastError());
+  int rc = launch_regulator_plain<256>(n_ifo, FP, FX, rms, n_pix, sky_valid_indices, n_valid,
+                                       gamma_regulator, w.count, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_regulator_finish, dim3(1), dim3(64), 0, st, w.count, n_valid,
+                     network_energy_threshold, reg0, REG, reg_f64);
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
+int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const float* FP,
+                         const float* FX, const int32_t* ml, const float* rms, const float* REG,
+                         double netCC, double network_energy_threshold,
+                         const int64_t* sky_valid_indices, int64_t n_valid, float* maps,
+                         int64_t* l_max, double* sky_stat_max, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+  if (!valid_args(n_ifo, n_pix, n_sky, n_valid) || n_delay < 1 || !(n_delay & 1))
+    return PYCWB_ERR_ARG;
+  if (!FP || !FX || !ml || !rms || !REG || !sky_valid_indices || !l_max || !sky_stat_max ||
+      !workspace)
+    return PYCWB_ERR_ARG;
+  if (workspace_bytes < plain_ws_bytes(n_ifo, n_pix, n_delay)) return PYCWB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  PlainWs w = plain_ws(workspace);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (float*)w.best, (int64_t)2, 0.f);
+  HIP_TRY(hipGetLastError());
+  if (maps && n_valid < n_sky) {
+    const int64_t cnt = (int64_t)kNumMaps * n_sky;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(cnt, 65536)), dim3(256), 0, st, maps, cnt, 0.f);
+    HIP_TRY(hipGetLastError());
+  }
+  const int K = n_delay / 2;
+  int rc = launch_scan_plain<256>(n_ifo, FP, FX, ml, n_sky, rms, w.td, n_pix, K, REG,
+                                  network_energy_threshold, netCC, sky_valid_indices, n_valid,
+                                  maps, w.best, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_argmax_finish, dim3(1), dim3(64), 0, st, w.best, sky_valid_indices,
+                     l_max, sky_stat_max);
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
+static int batch_enqueue(const pycwb_batch* b, hipStream_t st) {
+  const BatchDev& d = b->dev;
+  hipLaunchKernelGGL(k_batch_init, dim3(grid_for(d.n_clusters, 4096)), dim3(256), 0, st, d,
+                     2 * b->n_groups);
+  HIP_TRY(hipGetLastError());
+  for (int phase = 0; phase < 2; ++phase) {
+    for (int g = 0; g < b->n_groups; ++g) {
+      unsigned* ticket = d.tickets + 2 * g + phase;
+      int rc = PYCWB_ERR_ARG;
+      const int n = b->group_nifo[g], first = b->group_first[g], cnt = b->group_count[g];
+      if (b->block == 64)
+        rc = phase ? launch_group<64, 1>(n, d, b->grid, first, cnt, ticket, st)
+                   : launch_group<64, 0>(n, d, b->grid, first, cnt, ticket, st);
+      else if (b->block == 128)
+        rc = phase ? launch_group<128, 1>(n, d, b->grid, first, cnt, ticket, st)
+                   : launch_group<128, 0>(n, d, b->grid, first, cnt, ticket, st);
+      else if (b->block == 256)
+        rc = phase ? launch_group<256, 1>(n, d, b->grid, first, cnt, ticket, st)
+                   : launch_group<256, 0>(n, d, b->grid, first, cnt, ticket, st);
+      if (rc) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_batch_finish, dim3(grid_for(d.n_clusters, 1 << 20)), dim3(256), 0, st, d);
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
+int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
+  if (!d || !out) return PYCWB_ERR_ARG;
+  *out = nullptr;
+  if (d->n_clusters < 1 || d->n_sky < 1 || d->n_valid < 1 || d->n_valid > d->n_sky ||
+      d->n_valid >= (int64_t)0xffffffff)
+    return PYCWB_ERR_ARG;
+  if (d->block != 64 && d->block != 128 && d->block != 256) return PYCWB_ERR_ARG;
+  if (d->grid < 1) return PYCWB_ERR_ARG;
+  if (d->n_groups < 1 || d->n_groups > kMaxIfo) return PYCWB_ERR_ARG;
+  const int64_t tiles = (d->n_valid + d->block - 1) / d->block;
+  if ((int64_t)d->n_clusters * tiles >= (int64_t)0xffffffff) return PYCWB_ERR_ARG;
+  int covered = 0;
+  for (int g = 0; g < d->n_groups; ++g) {
+    if (d->group_nifo[g] < 1 || d->group_nifo[g] > kMaxIfo) return PYCWB_ERR_ARG;
+    if (d->group_first[g] != covered || d->group_count[g] < 0) return PYCWB_ERR_ARG;
+    covered += d->group_count[g];
+  }
+  if (covered != d->n_clusters) return PYCWB_ERR_ARG;
+  if (!d->sky_valid_indices || !d->order || !d->n_pix || !d->net || !d->K || !d->rms_off ||
+      !d->td_off || !d->rms || !d->td || !d->net_nifo || !d->net_off || !d->net_En || !d->FP ||
+      !d->FX || !d->ml || !d->tickets || !d->counts || !d->best || !d->REG1 || !d->l_max ||
+      !d->sky_stat_max)
+    return PYCWB_ERR_ARG;
+  pycwb_batch* b = new pycwb_batch();
+  BatchDev& v = b->dev;
+  v.n_clusters = d->n_clusters;
+  v.tiles_per_cluster = (int)tiles;
+  v.S = d->n_sky;
+  v.valid = d->sky_valid_indices;
+  v.n_valid = d->n_valid;
+  v.order = d->order;
+  v.npix = d->n_pix;
+  v.net = d->net;
+  v.K = d->K;
+  v.rms_off = d->rms_off;
+  v.td_off = d->td_off;
+  v.rms = d->rms;
+  v.td = (const float2*)d->td;
+  v.net_nifo = d->net_nifo;
+  v.net_off = d->net_off;
+  v.net_En = d->net_En;
+  v.FP = d->FP;
+  v.FX = d->FX;
+  v.ml = d->ml;
+  v.gamma = d->gamma_regulator;
+  v.rr = d->reg0;
+  v.netCC = d->netCC;
+  v.tickets = d->tickets;
+  v.counts = d->counts;
+  v.best = d->best;
+  v.REG1 = d->REG1;
+  v.l_max = d->l_max;
+  v.stat_max = d->sky_stat_max;
+  v.reg_f64 = d->reg_f64;
+  b->block = d->block;
+  b->grid = d->grid;
+  b->n_groups = d->n_groups;
+  for (int g = 0; g < d->n_groups; ++g) {
+    b->group_first[g] = d->group_first[g];
+    b->group_count[g] = d->group_count[g];
+    b->group_nifo[g] = d->group_nifo[g];
+  }
+  b->graph = nullptr;
+  b->exec = nullptr;
+  if (d->use_graph) {
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+      delete b;
+      return PYCWB_ERR_HIP;
+    }
+    int rc = PYCWB_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      rc = PYCWB_ERR_HIP;
+    if (!rc) {
+      int erc = batch_enqueue(b, cs);
+      hipError_t e = hipStreamEndCapture(cs, &b->graph);
+      if (erc) rc = erc;
+      else if (e != hipSuccess) rc = PYCWB_ERR_HIP;
+    }
+    if (!rc && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+      rc = PYCWB_ERR_HIP;
+    hipStreamDestroy(cs);
+    if (rc) {
+      if (b->graph) hipGraphDestroy(b->graph);
+      delete b;
+      return rc;
+    }
+  }
+  *out = b;
+  return PYCWB_OK;
+}
+
+int pycwb_batch_launch(pycwb_batch* b, void* stream) {
+  if (!b) return PYCWB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (b->exec) {
+    HIP_TRY(hipGraphLaunch(b->exec, st));
+    return PYCWB_OK;
+  }
+  return batch_enqueue(b, st);
+}
+
+int pycwb_batch_destroy(pycwb_batch* b) {
+  if (!b) return PYCWB_OK;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  delete b;
+  return PYCWB_OK;
+}
+
+const char* pycwb_version(void) { return "pycwb_amd 0.1 gfx950"; }
+
+}  // extern "C"
