@@ -37,6 +37,10 @@ extern "C" {
 
 const char* pycwb_version(void);
 
+/* Text of the last PYCWB_ERR_HIP raised on the calling thread (HIP error name
+ * and the failing call). */
+const char* pycwb_last_error(void);
+
 /* Bytes of device workspace needed by pycwb_td_relayout / pycwb_sky_scan_plain
  * for one cluster (256 bytes of counters + the packed amplitude table). */
 size_t pycwb_sky_scan_workspace_bytes(int n_ifo, int n_pix, int n_delay);

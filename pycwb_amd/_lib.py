@@ -72,6 +72,7 @@ class BatchDesc(C.Structure):
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
+    "pycwb_last_error": (C.c_char_p, []),
     "pycwb_sky_scan_workspace_bytes": (sz, [i32, i32, i32]),
     "pycwb_td_relayout": (i32, [i32, i32, i32, vp, vp, vp, sz, vp]),
     "pycwb_dpf_regulator": (i32, [i32, i32, i64, vp, vp, vp, f64, f64, f32, vp, i64, vp, vp,
@@ -112,4 +113,7 @@ def check(rc: int, what: str) -> None:
         return
     if rc == PYCWB_ERR_ARG:
         raise ValueError(f"{what}: arguments rejected by the C ABI")
-    raise PycwbError(f"{what}: HIP runtime error (code {rc})")
+    detail = ""
+    if _lib is not None:
+        detail = _lib.pycwb_last_error().decode(errors="replace")
+    raise PycwbError(f"{what}: HIP runtime error (code {rc}) {detail}")

@@ -15,6 +15,7 @@
 // per-direction sums have the reference's forward pixel order.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <string.h>
 
 #include "skyscan_lane.hh"
@@ -26,11 +27,22 @@ namespace {
 
 constexpr int kMaxIfo = 8;
 
+thread_local char g_last_error[256] = "";
+
+int fail_hip(hipError_t e, const char* what) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+  return PYCWB_ERR_HIP;
+}
+
 #define HIP_TRY(expr)                                  \
   do {                                                 \
     hipError_t _e = (expr);                            \
-    if (_e != hipSuccess) return PYCWB_ERR_HIP;        \
+    if (_e != hipSuccess) return fail_hip(_e, #expr);  \
   } while (0)
+
+// Clears an error left by earlier, unrelated runtime calls of the process so
+// that the launch checks below only report this call's own failures.
+inline void clear_stale_error() { (void)hipGetLastError(); }
 
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
 #pragma unroll
@@ -465,6 +477,7 @@ size_t pycwb_sky_scan_workspace_bytes(int n_ifo, int n_pix, int n_delay) {
 
 int pycwb_td_relayout(int n_ifo, int n_pix, int n_delay, const float* td00, const float* td90,
                       void* workspace, size_t workspace_bytes, void* stream) {
+  clear_stale_error();
   if (n_ifo < 1 || n_ifo > kMaxIfo || n_pix < 1 || n_delay < 1 || !(n_delay & 1))
     return PYCWB_ERR_ARG;
   if (!td00 || !td90 || !workspace) return PYCWB_ERR_ARG;
@@ -484,6 +497,7 @@ int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, co
                         const int64_t* sky_valid_indices, int64_t n_valid, float* REG,
                         double* reg_f64, void* workspace, size_t workspace_bytes,
                         void* stream) {
+  clear_stale_error();
   if (!valid_args(n_ifo, n_pix, n_sky, n_valid)) return PYCWB_ERR_ARG;
   if (!FP || !FX || !rms || !sky_valid_indices || !REG || !workspace) return PYCWB_ERR_ARG;
   if (workspace_bytes < kCounterBytes) return PYCWB_ERR_ARG;
@@ -506,6 +520,7 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
                          const int64_t* sky_valid_indices, int64_t n_valid, float* maps,
                          int64_t* l_max, double* sky_stat_max, void* workspace,
                          size_t workspace_bytes, void* stream) {
+  clear_stale_error();
   if (!valid_args(n_ifo, n_pix, n_sky, n_valid) || n_delay < 1 || !(n_delay & 1))
     return PYCWB_ERR_ARG;
   if (!FP || !FX || !ml || !rms || !REG || !sky_valid_indices || !l_max || !sky_stat_max ||
@@ -533,6 +548,7 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
 }
 
 static int batch_enqueue(const pycwb_batch* b, hipStream_t st) {
+  clear_stale_error();
   const BatchDev& d = b->dev;
   hipLaunchKernelGGL(k_batch_init, dim3(grid_for(d.n_clusters, 4096)), dim3(256), 0, st, d,
                      2 * b->n_groups);
@@ -670,5 +686,7 @@ int pycwb_batch_destroy(pycwb_batch* b) {
 }
 
 const char* pycwb_version(void) { return "pycwb_amd 0.1 gfx950"; }
+
+const char* pycwb_last_error(void) { return g_last_error; }
 
 }  // extern "C"
