@@ -65,16 +65,18 @@ int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, co
 
 /* Sky scan of one cluster.  Replaces scan_sky_for_best_fit
  * (pycwb/modules/likelihoodWP/sky_scan.py:17-192; called at likelihood.py:319).
- * Requires pycwb_td_relayout on the same workspace first.  maps may be NULL
+ * Reads the amplitude pairs from td_packed ((n_pix, n_ifo, n_delay) float
+ * pairs) or, when td_packed is NULL, from the workspace filled by
+ * pycwb_td_relayout (workspace >= 256 bytes suffices with td_packed).  maps may be NULL
  * (argmax only).  Writes *l_max (int64) and *sky_stat_max (float64) on the
  * device.  A direction with no pixel above the energy threshold is treated as
  * gated (the reference divides 0/0 there). */
 int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const float* FP,
-                         const float* FX, const int32_t* ml, const float* rms, const float* REG,
-                         double netCC, double network_energy_threshold,
-                         const int64_t* sky_valid_indices, int64_t n_valid, float* maps,
-                         int64_t* l_max, double* sky_stat_max, void* workspace,
-                         size_t workspace_bytes, void* stream);
+                         const float* FX, const int32_t* ml, const float* rms,
+                         const float* td_packed, const float* REG, double netCC,
+                         double network_energy_threshold, const int64_t* sky_valid_indices,
+                         int64_t n_valid, float* maps, int64_t* l_max, double* sky_stat_max,
+                         void* workspace, size_t workspace_bytes, void* stream);
 
 /* Batched regulator + scan + argmax over many clusters, as one hipGraph of
  * persistent work-queue kernels.  Replaces the per-cluster loop of
@@ -132,6 +134,9 @@ typedef struct pycwb_batch pycwb_batch;
 
 int pycwb_batch_create(const pycwb_batch_desc* desc, pycwb_batch** out);
 int pycwb_batch_launch(pycwb_batch* batch, void* stream);
+/* One stage of the batch outside the graph (0 init, 1 regulator, 2 scan,
+ * 3 finish), for per-stage timing; stages must run in order. */
+int pycwb_batch_launch_stage(pycwb_batch* batch, int stage, void* stream);
 int pycwb_batch_destroy(pycwb_batch* batch);
 
 #ifdef __cplusplus

@@ -77,10 +77,11 @@ SIGNATURES = {
     "pycwb_td_relayout": (i32, [i32, i32, i32, vp, vp, vp, sz, vp]),
     "pycwb_dpf_regulator": (i32, [i32, i32, i64, vp, vp, vp, f64, f64, f32, vp, i64, vp, vp,
                                   vp, sz, vp]),
-    "pycwb_sky_scan_plain": (i32, [i32, i32, i64, i32, vp, vp, vp, vp, vp, f64, f64, vp, i64,
-                                   vp, vp, vp, vp, sz, vp]),
+    "pycwb_sky_scan_plain": (i32, [i32, i32, i64, i32, vp, vp, vp, vp, vp, vp, f64, f64, vp,
+                                   i64, vp, vp, vp, vp, sz, vp]),
     "pycwb_batch_create": (i32, [C.POINTER(BatchDesc), C.POINTER(vp)]),
     "pycwb_batch_launch": (i32, [vp, vp]),
+    "pycwb_batch_launch_stage": (i32, [vp, i32, vp]),
     "pycwb_batch_destroy": (i32, [vp]),
 }
 

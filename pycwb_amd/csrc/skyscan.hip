@@ -515,20 +515,22 @@ int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, co
 }
 
 int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const float* FP,
-                         const float* FX, const int32_t* ml, const float* rms, const float* REG,
-                         double netCC, double network_energy_threshold,
-                         const int64_t* sky_valid_indices, int64_t n_valid, float* maps,
-                         int64_t* l_max, double* sky_stat_max, void* workspace,
-                         size_t workspace_bytes, void* stream) {
+                         const float* FX, const int32_t* ml, const float* rms,
+                         const float* td_packed, const float* REG, double netCC,
+                         double network_energy_threshold, const int64_t* sky_valid_indices,
+                         int64_t n_valid, float* maps, int64_t* l_max, double* sky_stat_max,
+                         void* workspace, size_t workspace_bytes, void* stream) {
   clear_stale_error();
   if (!valid_args(n_ifo, n_pix, n_sky, n_valid) || n_delay < 1 || !(n_delay & 1))
     return PYCWB_ERR_ARG;
   if (!FP || !FX || !ml || !rms || !REG || !sky_valid_indices || !l_max || !sky_stat_max ||
       !workspace)
     return PYCWB_ERR_ARG;
-  if (workspace_bytes < plain_ws_bytes(n_ifo, n_pix, n_delay)) return PYCWB_ERR_ARG;
+  if (workspace_bytes < (td_packed ? kCounterBytes : plain_ws_bytes(n_ifo, n_pix, n_delay)))
+    return PYCWB_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   PlainWs w = plain_ws(workspace);
+  const float2* td = td_packed ? (const float2*)td_packed : w.td;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (float*)w.best, (int64_t)2, 0.f);
   HIP_TRY(hipGetLastError());
   if (maps && n_valid < n_sky) {
@@ -537,7 +539,7 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
     HIP_TRY(hipGetLastError());
   }
   const int K = n_delay / 2;
-  int rc = launch_scan_plain<256>(n_ifo, FP, FX, ml, n_sky, rms, w.td, n_pix, K, REG,
+  int rc = launch_scan_plain<256>(n_ifo, FP, FX, ml, n_sky, rms, td, n_pix, K, REG,
                                   network_energy_threshold, netCC, sky_valid_indices, n_valid,
                                   maps, w.best, st);
   if (rc) return rc;
@@ -547,31 +549,13 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
   return PYCWB_OK;
 }
 
+int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream);
+
 static int batch_enqueue(const pycwb_batch* b, hipStream_t st) {
-  clear_stale_error();
-  const BatchDev& d = b->dev;
-  hipLaunchKernelGGL(k_batch_init, dim3(grid_for(d.n_clusters, 4096)), dim3(256), 0, st, d,
-                     2 * b->n_groups);
-  HIP_TRY(hipGetLastError());
-  for (int phase = 0; phase < 2; ++phase) {
-    for (int g = 0; g < b->n_groups; ++g) {
-      unsigned* ticket = d.tickets + 2 * g + phase;
-      int rc = PYCWB_ERR_ARG;
-      const int n = b->group_nifo[g], first = b->group_first[g], cnt = b->group_count[g];
-      if (b->block == 64)
-        rc = phase ? launch_group<64, 1>(n, d, b->grid, first, cnt, ticket, st)
-                   : launch_group<64, 0>(n, d, b->grid, first, cnt, ticket, st);
-      else if (b->block == 128)
-        rc = phase ? launch_group<128, 1>(n, d, b->grid, first, cnt, ticket, st)
-                   : launch_group<128, 0>(n, d, b->grid, first, cnt, ticket, st);
-      else if (b->block == 256)
-        rc = phase ? launch_group<256, 1>(n, d, b->grid, first, cnt, ticket, st)
-                   : launch_group<256, 0>(n, d, b->grid, first, cnt, ticket, st);
-      if (rc) return rc;
-    }
+  for (int stage = 0; stage < 4; ++stage) {
+    int rc = pycwb_batch_launch_stage(const_cast<pycwb_batch*>(b), stage, (void*)st);
+    if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_batch_finish, dim3(grid_for(d.n_clusters, 1 << 20)), dim3(256), 0, st, d);
-  HIP_TRY(hipGetLastError());
   return PYCWB_OK;
 }
 
@@ -664,6 +648,42 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
     }
   }
   *out = b;
+  return PYCWB_OK;
+}
+
+int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
+  if (!b || stage < 0 || stage > 3) return PYCWB_ERR_ARG;
+  clear_stale_error();
+  hipStream_t st = (hipStream_t)stream;
+  const BatchDev& d = b->dev;
+  if (stage == 0) {
+    hipLaunchKernelGGL(k_batch_init, dim3(grid_for(d.n_clusters, 4096)), dim3(256), 0, st, d,
+                       2 * b->n_groups);
+    HIP_TRY(hipGetLastError());
+    return PYCWB_OK;
+  }
+  if (stage == 3) {
+    hipLaunchKernelGGL(k_batch_finish, dim3(grid_for(d.n_clusters, 1 << 20)), dim3(256), 0, st,
+                       d);
+    HIP_TRY(hipGetLastError());
+    return PYCWB_OK;
+  }
+  const int phase = stage - 1;
+  for (int g = 0; g < b->n_groups; ++g) {
+    unsigned* ticket = d.tickets + 2 * g + phase;
+    int rc = PYCWB_ERR_ARG;
+    const int n = b->group_nifo[g], first = b->group_first[g], cnt = b->group_count[g];
+    if (b->block == 64)
+      rc = phase ? launch_group<64, 1>(n, d, b->grid, first, cnt, ticket, st)
+                 : launch_group<64, 0>(n, d, b->grid, first, cnt, ticket, st);
+    else if (b->block == 128)
+      rc = phase ? launch_group<128, 1>(n, d, b->grid, first, cnt, ticket, st)
+                 : launch_group<128, 0>(n, d, b->grid, first, cnt, ticket, st);
+    else if (b->block == 256)
+      rc = phase ? launch_group<256, 1>(n, d, b->grid, first, cnt, ticket, st)
+                 : launch_group<256, 0>(n, d, b->grid, first, cnt, ticket, st);
+    if (rc) return rc;
+  }
   return PYCWB_OK;
 }
 

@@ -76,12 +76,14 @@ class PlainClusterScan:
                                        ptr(self.ws), self.ws.numel(), stream_ptr(stream))
         _lib.check(rc, "pycwb_dpf_regulator")
 
-    def scan(self, FP, FX, ml, rms, n_sky, netCC, En, valid, maps=None, stream=None) -> None:
+    def scan(self, FP, FX, ml, rms, n_sky, netCC, En, valid, maps=None, stream=None,
+             td_packed=None) -> None:
         rc = lib().pycwb_sky_scan_plain(self.n_ifo, self.n_pix, int(n_sky), self.n_delay,
-                                        ptr(FP), ptr(FX), ptr(ml), ptr(rms), ptr(self.REG),
-                                        float(netCC), float(En), ptr(valid), int(valid.numel()),
-                                        ptr(maps), ptr(self.l_max), ptr(self.stat_max),
-                                        ptr(self.ws), self.ws.numel(), stream_ptr(stream))
+                                        ptr(FP), ptr(FX), ptr(ml), ptr(rms), ptr(td_packed),
+                                        ptr(self.REG), float(netCC), float(En), ptr(valid),
+                                        int(valid.numel()), ptr(maps), ptr(self.l_max),
+                                        ptr(self.stat_max), ptr(self.ws), self.ws.numel(),
+                                        stream_ptr(stream))
         _lib.check(rc, "pycwb_sky_scan_plain")
 
 
