@@ -50,7 +50,7 @@ def test_batched_matches_plain_and_oracle(oracle, block, use_graph):
     for c in range(sizes.size):
         net = nets[net_ids[c]]
         td00, td90, rms = _host_cluster(b, c)
-        En = 2 * 2.0 * net.n_ifo
+        En = nt.En[net_ids[c]]  # 2 * Acore**2 * n with Acore = sqrt(2), as the reference
         reg = oracle.calculate_dpf(net.FP_t, net.FX_t, rms, net.n_sky, net.n_ifo, gamma, En, vh)
         assert reg == rb["regulator"][c]
         REG = np.array([delta * np.sqrt(2), reg, 0], np.float32)
@@ -76,7 +76,7 @@ def test_batched_with_sky_mask(oracle):
     for c, k in enumerate([0, 1, 0]):
         net = nets[k]
         td00, td90, rms = _host_cluster(b, c)
-        En = 4.0 * net.n_ifo
+        En = nt.En[k]
         reg = oracle.calculate_dpf(net.FP_t, net.FX_t, rms, net.n_sky, net.n_ifo, 0.2, En, vh)
         REG = np.array([0.5 * np.sqrt(2), reg, 0], np.float32)
         l_o, _, s_o = oracle.scan_sky(net.n_ifo, int(sizes[c]), net.n_sky, net.FP_t, net.FX_t,
