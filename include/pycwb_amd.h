@@ -139,6 +139,59 @@ int pycwb_batch_launch(pycwb_batch* batch, void* stream);
 int pycwb_batch_launch_stage(pycwb_batch* batch, int stage, void* stream);
 int pycwb_batch_destroy(pycwb_batch* batch);
 
+/* Best-sky statistics of a batch of clusters at their selected directions,
+ * one workgroup per cluster.  Replaces compute_statistics_at_sky_position
+ * (pycwb/modules/likelihoodWP/sky_statistics.py:26-290; called at
+ * likelihood.py:359) with its packet operations (packet_ops.py).
+ * Cluster c uses the batch layout of pycwb_batch_desc (rms, td, sky tables),
+ * its direction l_max[c] and regulator REG1[c] (REG[0] = reg0), and its
+ * cross-talk CSR: lookup + 2*lk_off[c] ((n_pix, 2) int32 [start, end) rows into
+ * xt + 8*xt_off[c], whose (m, 8) float32 rows hold the neighbour's local pixel
+ * index in column 0 and the coefficients in columns 4..7).
+ * Scratch per cluster: s32 + s32_off[c] with pycwb_sky_stats_scratch32(n, P)
+ * floats and s64 + s64_off[c] with pycwb_sky_stats_scratch64(n, P) doubles;
+ * the reconstructed packets stay there (layout in pycwb_amd/skystats.py).
+ * Outputs: scal (16 doubles per cluster: Gn, Ec, Dc, Rc, Eh, Es, Np, Em, Lm,
+ * norm, cc, rho, xrho, Lo, Eo, N_pix_effective) and S_snr (8 per cluster).
+ * Launches n_clusters workgroups; block b handles cluster which[b] (or b when
+ * which is NULL). */
+typedef struct pycwb_stats_desc {
+  int n_clusters;
+  const int32_t* which;
+  int64_t n_sky;
+  const int32_t* n_pix;
+  const int32_t* nifo;
+  const int32_t* net;
+  const int32_t* K;
+  const int64_t* rms_off;
+  const int64_t* td_off;
+  const float* rms;
+  const float* td;
+  const int64_t* net_off;
+  const double* net_En;
+  const float* FP;
+  const float* FX;
+  const int32_t* ml;
+  const int64_t* l_max;
+  const float* REG1;
+  float reg0;
+  const int64_t* lk_off;
+  const int32_t* lookup;
+  const int64_t* xt_off;
+  const float* xt;
+  int xgb_rho_mode;
+  float* s32;
+  const int64_t* s32_off;
+  double* s64;
+  const int64_t* s64_off;
+  double* scal;
+  double* S_snr;
+} pycwb_stats_desc;
+
+size_t pycwb_sky_stats_scratch32(int n_ifo, int n_pix);
+size_t pycwb_sky_stats_scratch64(int n_ifo, int n_pix);
+int pycwb_sky_statistics(const pycwb_stats_desc* desc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -675,3 +675,283 @@ void oracle_scan_sky_f64(int n_ifo, int n_pix, int n_sky, int n_delay, const flo
     free(bs);
     free(rmsd);
 }
+
+/* ========================================================================= */
+/* Best-sky statistics: compute_statistics_at_sky_position                   */
+/* (likelihoodWP/sky_statistics.py:26-290) with the packet operations of     */
+/* likelihoodWP/packet_ops.py.  Numba kernels keep their float32/float64     */
+/* typing; the NumPy helpers (avx_noise_ps, avx_setAMP_ps, xtalk sums) run   */
+/* in float64 with sequential sums.                                          */
+/*                                                                           */
+/* xtalk CSR: lookup (P, 2) int32 [start, end), xt (m, 8) float32 with col 0 */
+/* = neighbour pixel index and cols 4..7 = coefficients.                     */
+/* scal[16] = Gn, Ec, Dc, Rc, Eh, Es, Np, Em, Lm, norm, cc, rho, xrho, Lo,   */
+/*            Eo, N_pix_effective (all float64; the dataclass stores f32).   */
+/* arrays (may be NULL): pd, pD, ps, pS (n, P) f64; nul00, nul90 (n, P) f32; */
+/* pmask, gnc, ecv, ee, EE (P) f32; S_snr (n) f64.                           */
+/* ========================================================================= */
+
+/* avx_packet_ps (packet_ops.py:658-753), numba: f32 arrays, f64 islands. */
+static double packet_mixed(const float *v00, const float *v90, const float *mask, int n, int P,
+                           float *out00, float *out90, float *E, float *si, float *co,
+                           float *a_save, float *A_save)
+{
+    const double o = 1.e-9;   /* python float */
+    for (int j = 0; j < n; ++j) {
+        float aa = 0.f, AA = 0.f, aA = 0.f;
+        for (int i = 0; i < P; ++i) {
+            float mk = mask[i] > 0.f ? 1.f : 0.f;
+            aa += mk * (v00[j * P + i] * v00[j * P + i]);
+            AA += mk * (v90[j * P + i] * v90[j * P + i]);
+            aA += mk * (v00[j * P + i] * v90[j * P + i]);
+        }
+        float _si = 2.f * aA;
+        float _co = aa - AA;
+        double _x = (double)(aa + AA) + o;
+        float _cc = _co * _co;
+        float _ss = _si * _si;
+        float _nn = sqrtf(_cc + _ss);
+        float a = (float)sqrt((_x + (double)_nn) / 2.0);
+        float A = (float)sqrt(fabs((_x - (double)_nn) / 2.0));
+        double cc = (double)_co / ((double)_nn + o);
+        float sgn = _si > 0.f ? 1.f : -1.f;
+        si[j] = (float)sqrt(((double)1.f - cc) / 2.0);
+        co[j] = (float)(sqrt(((double)1.f + cc) / 2.0) * (double)sgn);
+        float s = a + A;
+        E[j] = (s * s) / 2.f;
+        a_save[j] = a;
+        A_save[j] = A;
+        float ai = (float)(1.0 / ((double)a + o));
+        float Ai = (float)(1.0 / ((double)A + o));
+        for (int i = 0; i < P; ++i) {
+            float mk = mask[i] > 0.f ? 1.f : 0.f;
+            float _a = v00[j * P + i] * co[j] + v90[j * P + i] * si[j];
+            float _A = v90[j * P + i] * co[j] - v00[j * P + i] * si[j];
+            out00[j * P + i] = mk * _a * ai;
+            out90[j * P + i] = mk * _A * Ai;
+        }
+    }
+    double Ep = 0.0;
+    for (int j = 0; j < n; ++j) Ep += (double)E[j];
+    return Ep / 2.0;
+}
+
+/* packet_norm_numpy (packet_ops.py:813-901): returns per-ifo norm (clamped
+ * >= 2) in norm[n], halo noise rn[P], q_norm (n, P); p, q given in f64. */
+static void packet_norm_f64(const double *p, const double *q, const int32_t *lookup,
+                            const float *xt, const float *mk, int n, int P, double *norm,
+                            double *rn, double *q_norm)
+{
+    const double o = 1.e-12;
+    double *x = malloc(sizeof(double) * 4 * n);
+    for (int j = 0; j < n; ++j) norm[j] = 0.0;
+    for (int i = 0; i < n * P; ++i) q_norm[i] = 0.0;
+    for (int i = 0; i < P; ++i) {
+        rn[i] = 0.0;
+        if (mk[i] <= 0.f) continue;
+        for (int k = 0; k < 4 * n; ++k) x[k] = 0.0;
+        for (int e = lookup[2 * i]; e < lookup[2 * i + 1]; ++e) {
+            int nb = (int)xt[8 * e];
+            for (int j = 0; j < n; ++j) {
+                x[0 * n + j] += p[j * P + nb] * (double)xt[8 * e + 4];
+                x[1 * n + j] += p[j * P + nb] * (double)xt[8 * e + 5];
+                x[2 * n + j] += q[j * P + nb] * (double)xt[8 * e + 6];
+                x[3 * n + j] += q[j * P + nb] * (double)xt[8 * e + 7];
+            }
+        }
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double pi = p[j * P + i], qi = q[j * P + i];
+            double t = x[0 * n + j] * pi + x[1 * n + j] * qi + x[2 * n + j] * pi +
+                       x[3 * n + j] * qi;
+            double tc = t < 0 ? 0 : t;
+            norm[j] += tc;
+            double e = (pi * pi + qi * qi) / (tc + o);
+            q_norm[j * P + i] = e >= 1 ? 0 : e;
+            double u = x[0 * n + j] + x[2 * n + j], v = x[1 * n + j] + x[3 * n + j];
+            s += u * u + v * v;
+        }
+        rn[i] = s;
+    }
+    for (int j = 0; j < n; ++j) norm[j] = norm[j] < 2.0 ? 2.0 : norm[j];
+    free(x);
+}
+
+/* xtalk_energy_sum_numpy (packet_ops.py:756-810) */
+static double xtalk_sum_f64(const double *p, const double *q, const int32_t *lookup,
+                            const float *xt, const float *mk, int n, int P)
+{
+    double *g = calloc(n, sizeof(double));
+    double x[4];
+    for (int i = 0; i < P; ++i) {
+        if (mk[i] <= 0.f) continue;
+        for (int j = 0; j < n; ++j) {
+            x[0] = x[1] = x[2] = x[3] = 0.0;
+            for (int e = lookup[2 * i]; e < lookup[2 * i + 1]; ++e) {
+                int nb = (int)xt[8 * e];
+                x[0] += p[j * P + nb] * (double)xt[8 * e + 4];
+                x[1] += p[j * P + nb] * (double)xt[8 * e + 5];
+                x[2] += q[j * P + nb] * (double)xt[8 * e + 6];
+                x[3] += q[j * P + nb] * (double)xt[8 * e + 7];
+            }
+            double pi = p[j * P + i], qi = q[j * P + i];
+            double t = x[0] * pi + x[1] * qi + x[2] * pi + x[3] * qi;
+            if (t > 0.0) g[j] += t;
+        }
+    }
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s += g[j];
+    free(g);
+    return s;
+}
+
+void oracle_sky_statistics(int n_ifo, int n_pix, int n_sky, int n_delay, int64_t l,
+                           const float *FP, const float *FX, const float *rms,
+                           const float *td00, const float *td90, const int32_t *ml,
+                           const float *REG, double En, const int32_t *lookup, const float *xt,
+                           int xgb_rho_mode, double *scal, double *pd, double *pD, double *ps,
+                           double *pS, float *nul00, float *nul90, float *pmask, float *gnc,
+                           float *ecv, float *eev, float *EEv, double *S_snr)
+{
+    const int n = n_ifo, P = n_pix, S = n_sky, offset = n_delay / 2;
+    scan_tmp t = tmp_alloc(n, P);
+    for (int i = 0; i < n; ++i) {
+        int64_t row = (int64_t)ml[(int64_t)i * S + l] + offset;
+        memcpy(t.v00 + i * P, td00 + (row * n + i) * P, sizeof(float) * P);
+        memcpy(t.v90 + i * P, td90 + (row * n + i) * P, sizeof(float) * P);
+    }
+    int64_t M0;
+    load_data_mixed(t.v00, t.v90, n, P, En, t.et, t.mask, &M0);
+    dpf_mixed(FP + l * n, FX + l * n, rms, P, n, t.f, t.F, t.fp, t.fx, t.si, t.co, t.ni);
+    gw_mixed(t.v00, t.v90, t.f, t.F, t.fp, t.fx, t.ni, t.et, t.mask, REG, n, P, t.p, t.q, t.mk,
+             t.au, t.AU, t.av, t.AV);
+    ort_mixed(t.p, t.q, t.mk, n, P, t.si2, t.co2, t.ee, t.EE);
+    stat_mixed(t.v00, t.v90, t.p, t.q, t.si2, t.co2, t.mk, n, P, t.ec, t.gn, t.rn);
+
+    float *d00 = malloc(sizeof(float) * n * P * 4), *d90 = d00 + n * P, *s00 = d90 + n * P,
+          *s90 = s00 + n * P;
+    float dE[8], dsi[8], dco[8], da[8], dA[8], sE[8], ssi[8], sco[8], sa[8], sA[8];
+    double Eo = packet_mixed(t.v00, t.v90, t.mk, n, P, d00, d90, dE, dsi, dco, da, dA);
+    double Lo = packet_mixed(t.p, t.q, t.mk, n, P, s00, s90, sE, ssi, sco, sa, sA);
+
+    double *dd00 = malloc(sizeof(double) * n * P * 8);
+    double *dd90 = dd00 + n * P, *qn = dd90 + n * P, *pn = qn + n * P, *sd00 = pn + n * P,
+           *sd90 = sd00 + n * P, *ap00 = sd90 + n * P, *ap90 = ap00 + n * P;
+    double *rnx = malloc(sizeof(double) * P);
+    double norm[8], snorm[8];
+    for (int i = 0; i < n * P; ++i) { dd00[i] = d00[i]; dd90[i] = d90[i]; }
+    packet_norm_f64(dd00, dd90, lookup, xt, t.mk, n, P, norm, rnx, qn);
+    /* gw_norm_numpy (packet_ops.py:904-948): S_snr = 2 * E_signal / norm,
+     * p_norm = q_norm where ec > 0. */
+    for (int j = 0; j < n; ++j) {
+        snorm[j] = (double)sE[j] * 2 / norm[j];
+        if (S_snr) S_snr[j] = snorm[j];
+        for (int i = 0; i < P; ++i) pn[j * P + i] = t.ec[i] > 0.f ? qn[j * P + i] : 0.0;
+    }
+    /* avx_noise_ps (packet_ops.py:93-153), float64 */
+    double EC = 0, NC = 0, ES = 0, SC = 0, RC = 0, GN = 0, NS = 0, EH = 0;
+    for (int i = 0; i < P; ++i) {
+        double ns = 0, nx = 0;
+        for (int j = 0; j < n; ++j) { ns += pn[j * P + i]; nx += qn[j * P + i]; }
+        ns /= n;
+        nx /= n;
+        double mk = t.mk[i] > 0.f ? 1.0 : 0.0;
+        double core = mk * (nx > 0 ? 1.0 : 0.0);
+        EC += core * (double)t.ec[i];
+        NC += core;
+        ES += (mk - core) * rnx[i];
+        double gn = t.gn[i];
+        double rc = gn < 2.0 ? 1.0 : 0.0;
+        double nn = gn * (1.0 - rc);
+        rc = rc + nn * 0.5;
+        rc = (double)t.ec[i] / (rc + 1e-9);
+        double sig = mk * (ns > 0 ? 1.0 : 0.0);
+        t.gn[i] = (float)(mk * gn * nx);
+        SC += sig * (double)t.ec[i];
+        RC += sig * rc;
+        GN += sig * (double)t.gn[i];
+        NS += sig;
+        EH += (mk - sig) * (double)t.et[i];
+    }
+    double Gn = GN, Ec = EC, Dc = SC - EC, Rc = RC / (SC + 0.01), Eh = EH / 2, Es = ES / 2;
+    /* avx_setAMP_ps (packet_ops.py:234-285) for data and signal packets */
+    double Np_eff = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const float *p = pass ? s00 : d00, *q = pass ? s90 : d90;
+        const double *nrm = pass ? pn : qn;
+        const float *a = pass ? sa : da, *A = pass ? sA : dA, *si = pass ? ssi : dsi,
+                    *co = pass ? sco : dco;
+        double *o00 = pass ? ap00 : sd00, *o90 = pass ? ap90 : sd90;
+        double Npp = 0;
+        for (int i = 0; i < P; ++i) {
+            float mk = 0.5f * (t.mk[i] > 0.f ? 1.f : 0.f);
+            double col = 0;
+            for (int j = 0; j < n; ++j) {
+                double na = (double)((a[j] + A[j]) * mk) * nrm[j * P + i];
+                float rp = p[j * P + i] * co[j] - q[j * P + i] * si[j];
+                float rq = q[j * P + i] * co[j] + p[j * P + i] * si[j];
+                o00[j * P + i] = na * (double)rp;
+                o90[j * P + i] = na * (double)rq;
+                col += nrm[j * P + i];
+            }
+            Npp += col * (double)mk;
+        }
+        if (!pass) Np_eff = Npp * 4 / n - 1;
+    }
+    /* avx_loadNULL_ps: f32(data) - f32(signal) */
+    float *n00 = malloc(sizeof(float) * n * P * 2), *n90 = n00 + n * P;
+    double *dn00 = malloc(sizeof(double) * n * P * 2), *dn90 = dn00 + n * P;
+    for (int i = 0; i < n * P; ++i) {
+        n00[i] = (float)sd00[i] - (float)ap00[i];
+        n90[i] = (float)sd90[i] - (float)ap90[i];
+        dn00[i] = n00[i];
+        dn90[i] = n90[i];
+    }
+    double Em = xtalk_sum_f64(sd00, sd90, lookup, xt, t.mk, n, P);
+    double Np = xtalk_sum_f64(dn00, dn90, lookup, xt, t.mk, n, P);
+    double Lm = Em - Np - Gn;
+    double nrm = Em > 0 ? (Eo - Eh) / Em : 1.e9;
+    if (nrm < 1) nrm = 1;
+    Ec /= nrm;
+    Dc /= nrm;
+    double chi = (Np + Gn) / (Np_eff * n);
+    double ccf = chi > 1 ? chi : 1, rho, xrho = 0;
+    if (!xgb_rho_mode) {
+        rho = Ec > 0 ? sqrt(Ec * Rc / 2.) : 0;
+    } else {
+        double pen = chi;
+        rho = Ec > 0 ? sqrt(Ec / (1 + pen * ((pen > 1 ? pen : 1) - 1))) : 0;
+        xrho = Ec > 0 ? sqrt(Ec * Rc / 2.) : 0;
+    }
+    double v[16] = {Gn, Ec, Dc, Rc, Eh, Es, Np, Em, Lm, nrm, ccf, rho, xrho, Lo, Eo, Np_eff};
+    memcpy(scal, v, sizeof(v));
+    if (pd) memcpy(pd, sd00, sizeof(double) * n * P);
+    if (pD) memcpy(pD, sd90, sizeof(double) * n * P);
+    if (ps) memcpy(ps, ap00, sizeof(double) * n * P);
+    if (pS) memcpy(pS, ap90, sizeof(double) * n * P);
+    if (nul00) memcpy(nul00, n00, sizeof(float) * n * P);
+    if (nul90) memcpy(nul90, n90, sizeof(float) * n * P);
+    if (pmask) memcpy(pmask, t.mk, sizeof(float) * P);
+    if (gnc) memcpy(gnc, t.gn, sizeof(float) * P);
+    if (ecv) memcpy(ecv, t.ec, sizeof(float) * P);
+    if (eev) memcpy(eev, t.ee, sizeof(float) * P);
+    if (EEv) memcpy(EEv, t.EE, sizeof(float) * P);
+    free(n00); free(dn00); free(dd00); free(rnx); free(d00);
+    tmp_free(&t);
+}
+
+/* Exported single-purpose wrappers for the reference's packet-op contract
+ * tests (packet_ops.py:756-901). */
+void oracle_packet_norm(const double *p, const double *q, const int32_t *lookup, const float *xt,
+                        const float *mk, int n, int P, const float *q_E, double *det_snr,
+                        double *norm, double *rn, double *q_norm)
+{
+    packet_norm_f64(p, q, lookup, xt, mk, n, P, norm, rn, q_norm);
+    for (int j = 0; j < n; ++j) det_snr[j] = (double)q_E[j] * 2.0 / norm[j];
+}
+
+double oracle_xtalk_sum(const double *p, const double *q, const int32_t *lookup, const float *xt,
+                        const float *mk, int n, int P)
+{
+    return xtalk_sum_f64(p, q, lookup, xt, mk, n, P);
+}

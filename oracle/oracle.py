@@ -143,3 +143,75 @@ def wdm_t2w(x, M, filt, MM=-1):
     n = load().wdm_oracle_t2w(_p(x), L, M, _p(filt), filt.size, MM, _p(m00), _p(m90))
     assert n == N
     return m00, m90
+
+
+SKYSTAT_SCALARS = ("Gn", "Ec", "Dc", "Rc", "Eh", "Es", "Np", "Em", "Lm", "norm", "cc", "rho",
+                   "xrho", "Lo", "Eo", "N_pix_effective")
+
+
+def self_xtalk(n_pix):
+    """Self-only cross-talk CSR: each pixel its own neighbour, coefficients (1,0,0,1)."""
+    lookup = np.stack([np.arange(n_pix), np.arange(1, n_pix + 1)], axis=1).astype(np.int32)
+    xt = np.zeros((n_pix, 8), dtype=np.float32)
+    xt[:, 0] = np.arange(n_pix)
+    xt[:, 4] = 1.0
+    xt[:, 7] = 1.0
+    return lookup, xt
+
+
+def sky_statistics(l, n_ifo, n_pix, n_sky, FP, FX, rms, td00, td90, ml, REG, En, lookup=None,
+                   xt=None, xgb_rho_mode=False):
+    """compute_statistics_at_sky_position (sky_statistics.py:26-290)."""
+    lib = load()
+    fn = lib.oracle_sky_statistics
+    fn.restype = None
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64] + [vp] * 7 + [C.c_double] + \
+        [vp, vp, C.c_int, vp] + [vp] * 12
+    if lookup is None:
+        lookup, xt = self_xtalk(n_pix)
+    FP, FX, rms = _c(FP, np.float32), _c(FX, np.float32), _c(rms, np.float32)
+    td00, td90 = _c(td00, np.float32), _c(td90, np.float32)
+    ml, REG = _c(ml, np.int32), _c(REG, np.float32)
+    lookup, xt = _c(lookup, np.int32), _c(xt, np.float32)
+    scal = np.zeros(16)
+    arr = {k: np.zeros((n_ifo, n_pix)) for k in ("pd", "pD", "ps", "pS")}
+    arr.update({k: np.zeros((n_ifo, n_pix), np.float32) for k in ("nul00", "nul90")})
+    arr.update({k: np.zeros(n_pix, np.float32) for k in ("pixel_mask", "gn", "ec", "ee", "EE")})
+    arr["S_snr"] = np.zeros(n_ifo)
+    fn(n_ifo, n_pix, n_sky, td00.shape[0], int(l), _p(FP), _p(FX), _p(rms), _p(td00), _p(td90),
+       _p(ml), _p(REG), En, _p(lookup), _p(xt), int(bool(xgb_rho_mode)), _p(scal),
+       _p(arr["pd"]), _p(arr["pD"]), _p(arr["ps"]), _p(arr["pS"]), _p(arr["nul00"]),
+       _p(arr["nul90"]), _p(arr["pixel_mask"]), _p(arr["gn"]), _p(arr["ec"]), _p(arr["ee"]),
+       _p(arr["EE"]), _p(arr["S_snr"]))
+    out = dict(zip(SKYSTAT_SCALARS, scal.tolist()))
+    out.update(arr)
+    return out
+
+
+def packet_norm(p, q, xtalks, lookup, mk, q_E):
+    lib = load()
+    fn = lib.oracle_packet_norm
+    fn.restype = None
+    fn.argtypes = [vp] * 5 + [C.c_int, C.c_int] + [vp] * 5
+    p, q = _c(p, np.float64), _c(q, np.float64)
+    n, P = p.shape
+    xt = _c(xtalks, np.float32).reshape(-1, 8)
+    lk = _c(lookup, np.int32)
+    mk, q_E = _c(mk, np.float32), _c(q_E, np.float32)
+    det, norm, rn, qn = np.zeros(n), np.zeros(n), np.zeros(P), np.zeros((n, P))
+    fn(_p(p), _p(q), _p(lk), _p(xt) if xt.size else vp(), _p(mk), n, P, _p(q_E), _p(det),
+       _p(norm), _p(rn), _p(qn))
+    return det, norm, rn, qn
+
+
+def xtalk_sum(p, q, xtalks, lookup, mk):
+    lib = load()
+    fn = lib.oracle_xtalk_sum
+    fn.restype = C.c_double
+    fn.argtypes = [vp] * 5 + [C.c_int, C.c_int]
+    p, q = _c(p, np.float64), _c(q, np.float64)
+    n, P = p.shape
+    xt = _c(xtalks, np.float32).reshape(-1, 8)
+    lk = _c(lookup, np.int32)
+    mk = _c(mk, np.float32)
+    return fn(_p(p), _p(q), _p(lk), _p(xt) if xt.size else vp(), _p(mk), n, P)

@@ -69,6 +69,16 @@ class BatchDesc(C.Structure):
     ]
 
 
+class StatsDesc(C.Structure):
+    _fields_ = [("n_clusters", C.c_int), ("which", vp), ("n_sky", C.c_int64), ("n_pix", vp),
+                ("nifo", vp), ("net", vp), ("K", vp), ("rms_off", vp), ("td_off", vp),
+                ("rms", vp), ("td", vp), ("net_off", vp), ("net_En", vp), ("FP", vp),
+                ("FX", vp), ("ml", vp), ("l_max", vp), ("REG1", vp), ("reg0", C.c_float),
+                ("lk_off", vp), ("lookup", vp), ("xt_off", vp), ("xt", vp),
+                ("xgb_rho_mode", C.c_int), ("s32", vp), ("s32_off", vp), ("s64", vp),
+                ("s64_off", vp), ("scal", vp), ("S_snr", vp)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -83,6 +93,9 @@ SIGNATURES = {
     "pycwb_batch_launch": (i32, [vp, vp]),
     "pycwb_batch_launch_stage": (i32, [vp, i32, vp]),
     "pycwb_batch_destroy": (i32, [vp]),
+    "pycwb_sky_stats_scratch32": (sz, [i32, i32]),
+    "pycwb_sky_stats_scratch64": (sz, [i32, i32]),
+    "pycwb_sky_statistics": (i32, [C.POINTER(StatsDesc), vp]),
 }
 
 

@@ -23,14 +23,20 @@
 
 using namespace pycwb;
 
+namespace pycwb {
+static thread_local char g_last_error[256] = "";
+void record_error(const char* what, const char* why) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, why);
+}
+const char* last_error() { return g_last_error; }
+}  // namespace pycwb
+
 namespace {
 
 constexpr int kMaxIfo = 8;
 
-thread_local char g_last_error[256] = "";
-
 int fail_hip(hipError_t e, const char* what) {
-  snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+  pycwb::record_error(what, hipGetErrorString(e));
   return PYCWB_ERR_HIP;
 }
 
@@ -707,6 +713,6 @@ int pycwb_batch_destroy(pycwb_batch* b) {
 
 const char* pycwb_version(void) { return "pycwb_amd 0.1 gfx950"; }
 
-const char* pycwb_last_error(void) { return g_last_error; }
+const char* pycwb_last_error(void) { return pycwb::last_error(); }
 
 }  // extern "C"

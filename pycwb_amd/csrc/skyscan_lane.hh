@@ -14,6 +14,10 @@
 
 namespace pycwb {
 
+// Thread-local text of the last HIP failure (read by pycwb_last_error).
+void record_error(const char* what, const char* why);
+const char* last_error();
+
 constexpr int kNumMaps = 11;
 
 // Per-direction sums of one sky sweep.
@@ -102,13 +106,28 @@ __device__ __forceinline__ void regulator_pixel(const float (&r)[N], const float
   NN += d.fp > 0.f ? 1u : 0u;
 }
 
-// One pixel of the scan: load_data_from_td, DPF, avx_GW_ps, avx_ort_ps,
-// avx_stat_ps and the antenna-prior sums, fused.
+// Every intermediate of one (direction, pixel) evaluation that a later
+// consumer needs: the sky scan keeps only the sums, the best-sky statistics
+// keep the per-pixel arrays.
 template <int N>
-__device__ __forceinline__ void scan_pixel(const float (&v00)[N], const float (&v90)[N],
+struct PixelEval {
+  float e;         // pixel energy + 1e-12
+  int m;           // e > En
+  float et;        // masked energy
+  Dpf<N> d;        // dominant polarisation frame
+  float p[N], q[N];// signal packet (avx_GW_ps)
+  float mk;        // updated mask (float32 array of avx_GW_ps)
+  float ee, EE;    // orthogonalised packet energies (avx_ort_ps)
+  float ec, gn, rn;// coherent energy, G-noise, residual noise (avx_stat_ps)
+  double ll, ccd;  // signal energy and correlation factor (float64)
+};
+
+// load_data_from_td, DPF, avx_GW_ps, avx_ort_ps and avx_stat_ps of one pixel.
+template <int N>
+__device__ __forceinline__ void pixel_eval(const float (&v00)[N], const float (&v90)[N],
                                            const float (&r)[N], const float (&fp0)[N],
                                            const float (&fx0)[N], float rr, float RR,
-                                           double En, SkySums& acc) {
+                                           double En, PixelEval<N>& o) {
   // --- load_data_from_td (sky_stat.py:74-87) ---
   float aa = 0.f, AA = 0.f;
 #pragma unroll
@@ -119,15 +138,16 @@ __device__ __forceinline__ void scan_pixel(const float (&v00)[N], const float (&
   const float e = aa + AA + 1e-12f;
   const int m = (double)e > En ? 1 : 0;
   const float et = e * (float)m;
-  acc.M += m;
-  acc.EE += et;
+  o.e = e;
+  o.m = m;
+  o.et = et;
 
   // --- dpf_np_loops_vec ---
-  Dpf<N> d;
+  Dpf<N>& d = o.d;
   dpf_pixel<N>(r, fp0, fx0, d);
 
   // --- avx_GW_ps (sky_stat.py:156-189) ---
-  const float o = 1e-9f;
+  const float eps = 1e-9f;
   float xp = 0.f, XP = 0.f, xx = 0.f, XX = 0.f;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -136,55 +156,58 @@ __device__ __forceinline__ void scan_pixel(const float (&v00)[N], const float (&
     xx += v00[i] * d.F[i];
     XX += v90[i] * d.F[i];
   }
-  float tf = __builtin_sqrtf(d.ni * (xp * xp + XP * XP) / (et + o)) * rr - d.fp;
+  float tf = __builtin_sqrtf(d.ni * (xp * xp + XP * XP) / (et + eps)) * rr - d.fp;
   tf = tf > 0.f ? tf : 0.f;
   const double md = (double)m;
-  const double fd = md / (double)(d.fp + tf + o);
+  const double fd = md / (double)(d.fp + tf + eps);
   double h = (double)xp * fd;
   const double H = (double)XP * fd;
   h = h * h + H * H;
   const float Hx = xx * xx + XX * XX;
-  double Fd = __builtin_sqrt((double)Hx / (h + (double)o));
-  const float R = 0.1f + RR / (et + o);
+  double Fd = __builtin_sqrt((double)Hx / (h + (double)eps));
+  const float R = 0.1f + RR / (et + eps);
   Fd = Fd * (double)R - (double)d.fx;
   Fd = Fd > 0.0 ? Fd : 0.0;
-  Fd = md / (((double)d.fx + Fd) + (double)o);
+  Fd = md / (((double)d.fx + Fd) + (double)eps);
   const float au = (float)((double)xp * fd);
   const float AU = (float)((double)XP * fd);
   const float av = (float)((double)xx * Fd);
   const float AV = (float)((double)XX * Fd);
   const double ga = fd * (double)d.fp + Fd * (double)d.fx;
   const float mk = (float)((ga + md) - 1.0);
-  float p[N], q[N];
+  o.mk = mk;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    p[i] = d.f[i] * au + d.F[i] * av;
-    q[i] = d.f[i] * AU + d.F[i] * AV;
+    o.p[i] = d.f[i] * au + d.F[i] * av;
+    o.q[i] = d.f[i] * AU + d.F[i] * AV;
   }
 
   // --- avx_ort_ps (sky_stat.py:264-288) ---
   float pa = 0.f, PA = 0.f, pA = 0.f;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    pa += p[i] * p[i];
-    PA += q[i] * q[i];
-    pA += p[i] * q[i];
+    pa += o.p[i] * o.p[i];
+    PA += o.q[i] * o.q[i];
+    pA += o.p[i] * o.q[i];
   }
   const float s2 = pA * 2.f;
   const float c2 = pa - PA;
+  const float et2 = pa + PA + 1e-21f;
   const float nn2 = __builtin_sqrtf(c2 * c2 + s2 * s2);
+  o.ee = (et2 + nn2) / 2.f;
+  o.EE = (et2 - nn2) / 2.f;
   const float cc2 = c2 / (nn2 + 1e-21f);
   const float si = __builtin_sqrtf((1.f - cc2) / 2.f);
   const float cq = __builtin_sqrtf((1.f + cc2) / 2.f);
   const float co = s2 > 0.f ? cq : -cq;
 
-  // --- avx_stat_ps (sky_stat.py:357-405) ---
+  // --- avx_stat_ps (sky_stat.py:357-398) ---
   float c = 0.f, C = 0.f, ss = 0.f, SS = 0.f, rq = 0.f, RQ = 0.f, xs = 0.f, XS = 0.f;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    const float s_ = p[i] * co + q[i] * si;
+    const float s_ = o.p[i] * co + o.q[i] * si;
     const float x_ = v00[i] * co + v90[i] * si;
-    const float S_ = q[i] * co - p[i] * si;
+    const float S_ = o.q[i] * co - o.p[i] * si;
     const float X_ = v90[i] * co - v00[i] * si;
     const float a = s_ * x_;
     const float A = S_ * X_;
@@ -194,35 +217,46 @@ __device__ __forceinline__ void scan_pixel(const float (&v00)[N], const float (&
     C += A * A;
     ss += s_ * s_;
     SS += S_ * S_;
-    const float dd = p[i] - v00[i];
-    const float DD = q[i] - v90[i];
+    const float dd = o.p[i] - v00[i];
+    const float DD = o.q[i] - v90[i];
     rq += dd * dd;
     RQ += DD * DD;
   }
   const double mkd = mk >= 0.f ? 1.0 : 0.0;
-  c = c / (xs * xs + o);
-  C = C / (XS * XS + o);
-  const double ll = mkd * (double)(ss + SS);
+  c = c / (xs * xs + eps);
+  C = C / (XS * XS + eps);
+  o.ll = mkd * (double)(ss + SS);
   const double ssd = (double)ss * (1.0 - (double)c);
   const double SSd = (double)SS * (1.0 - (double)C);
-  const float ec = (float)(mkd * (ssd + SSd));
-  const float gn = (float)((mkd * 2.0) * (double)mk);
-  const float rn = (float)(mkd * (double)(rq + RQ));
-  const double a2 = 2.0 * (double)fabsf(ec);
-  const float A2 = rn + gn + o;
-  const double ccd = (double)ec / (a2 + (double)A2);
-  acc.Lr += ll * ccd;
-  acc.LL += ll;
-  acc.EC += ec;
-  acc.GN += gn;
-  acc.RN += rn;
-  acc.Mp += ec > o ? 1.0 : 0.0;
+  o.ec = (float)(mkd * (ssd + SSd));
+  o.gn = (float)((mkd * 2.0) * (double)mk);
+  o.rn = (float)(mkd * (double)(rq + RQ));
+  const double a2 = 2.0 * (double)fabsf(o.ec);
+  const float A2 = o.rn + o.gn + eps;
+  o.ccd = (double)o.ec / (a2 + (double)A2);
+}
 
-  // --- antenna-prior sums over accepted pixels (sky_scan.py:148-153) ---
-  if (mk > 0.f) {
-    acc.sde += et;
-    acc.pwe += d.fp * et;
-    acc.cwe += d.fx * et;
+// One pixel of the scan: pixel_eval plus the per-direction sums
+// (sky_stat.py:398-405, sky_scan.py:148-153).
+template <int N>
+__device__ __forceinline__ void scan_pixel(const float (&v00)[N], const float (&v90)[N],
+                                           const float (&r)[N], const float (&fp0)[N],
+                                           const float (&fx0)[N], float rr, float RR,
+                                           double En, SkySums& acc) {
+  PixelEval<N> o;
+  pixel_eval<N>(v00, v90, r, fp0, fx0, rr, RR, En, o);
+  acc.M += o.m;
+  acc.EE += o.et;
+  acc.Lr += o.ll * o.ccd;
+  acc.LL += o.ll;
+  acc.EC += o.ec;
+  acc.GN += o.gn;
+  acc.RN += o.rn;
+  acc.Mp += o.ec > 1e-9f ? 1.0 : 0.0;
+  if (o.mk > 0.f) {
+    acc.sde += o.et;
+    acc.pwe += o.d.fp * o.et;
+    acc.cwe += o.d.fx * o.et;
   }
 }
 

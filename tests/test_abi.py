@@ -12,7 +12,9 @@ def _declared():
     for fn in os.listdir(os.path.join(ROOT, "include")):
         if fn.endswith(".h"):
             txt = open(os.path.join(ROOT, "include", fn)).read()
-            names |= set(re.findall(r"\b(pycwb_[a-z0-9_]+)\s*\(", txt))
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)  # declarations only
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+[\s*]+(pycwb_\w+)\s*\(", txt,
+                                    flags=re.M))
     return names
 
 
