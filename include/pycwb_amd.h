@@ -192,6 +192,21 @@ size_t pycwb_sky_stats_scratch32(int n_ifo, int n_pix);
 size_t pycwb_sky_stats_scratch64(int n_ifo, int n_pix);
 int pycwb_sky_statistics(const pycwb_stats_desc* desc, void* stream);
 
+/* WDM forward transform t2w of n_series float64 series (row stride ldx), all
+ * with the same filter.  Replaces WDM<T>::t2w (cwb-core/WDM.cc:1385-1534) as
+ * called through wdm_wavelet's t2w at
+ * pycwb/modules/coherence_native/tf_batch_generation.py:111-191 and
+ * pycwb/utils/td_vector_batch.py:72-80.
+ * M = layers (power of two, 2M <= 2048), stride = time step MM (M for the
+ * quadrature transform), filter = m_H taps (WDM<T>::getFilter),
+ * twiddle = 2M complex float64 values exp(-2 pi i j / 2M), j = 0..2M-1.
+ * quadrature != 0: out holds (n_series, N, M+1) complex pairs (map00, map90);
+ * quadrature == 0: out holds the power map in the real parts.
+ * N = ceil(n_samples / stride).  Requires n_samples > m_H. */
+int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series, int M,
+                  int stride, const double* filter, int m_H, const double* twiddle,
+                  int quadrature, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

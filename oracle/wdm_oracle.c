@@ -129,8 +129,8 @@ int wdm_oracle_filter(int M, int K, int beta, int precision, double *filter, int
 }
 
 /* Forward transform of one series x[0..L).  stride MM (<=0 means M);
- * quadrature (MM<=0 in the reference's convention KK<0) writes map00/map90
- * with layout (N, M+1); returns N, the number of time bins. */
+ * MM < 0 (the reference's KK < 0) writes the quadrature maps map00/map90,
+ * otherwise the power map into map00; layout (N, M+1); returns N. */
 int64_t wdm_oracle_t2w(const double *x, int64_t L, int M, const double *filt, int m_H,
                        int MM, double *map00, double *map90)
 {
@@ -140,7 +140,7 @@ int64_t wdm_oracle_t2w(const double *x, int64_t L, int M, const double *filt, in
     double *ts = calloc(m, sizeof(double));
     for (int64_t n = 0; n <= m_H && n < L; ++n) ts[m_H - n] = x[n];
     for (int64_t n = 0; n < L; ++n) ts[m_H + n] = x[n];
-    for (int64_t n = 0; n < m - m_H - L; ++n) ts[n + m_H + L] = x[L - n - 1];
+    for (int64_t n = 0; n < m - m_H - L; ++n) ts[n + m_H + L] = L - n - 1 >= 0 ? x[L - n - 1] : 0.0;
     int64_t N = nW / stride;
     int M2 = 2 * M;
     double *re = malloc(sizeof(double) * M2);
@@ -170,7 +170,7 @@ int64_t wdm_oracle_t2w(const double *x, int64_t L, int M, const double *filt, in
             double v00, v90;
             if ((f + odd) & 1) { v00 = sqrt2 * I; v90 = sqrt2 * R; }
             else { v00 = sqrt2 * R; v90 = -sqrt2 * I; }
-            if (MM > 0) { v00 = R * R + I * I; v90 = 0; }
+            if (MM >= 0) { v00 = R * R + I * I; v90 = 0; }
             map00[n * (M + 1) + f] = v00;
             if (map90) map90[n * (M + 1) + f] = v90;
         }

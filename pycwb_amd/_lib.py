@@ -96,6 +96,7 @@ SIGNATURES = {
     "pycwb_sky_stats_scratch32": (sz, [i32, i32]),
     "pycwb_sky_stats_scratch64": (sz, [i32, i32]),
     "pycwb_sky_statistics": (i32, [C.POINTER(StatsDesc), vp]),
+    "pycwb_wdm_t2w": (i32, [vp, i64, i64, i32, i32, i32, vp, i32, vp, i32, vp, vp]),
 }
 
 

@@ -1,0 +1,137 @@
+"""WDM filter design and forward transform.
+
+CPU part pins the oracle and the host-side filter design: known answers from
+the reference's coefficient table (tests/golden/wdm_fourier_coefficients.json,
+from cwb-core/FourierCoefficients.icc), the filter length rule of
+cwb-core/WDM.cc:149-171, and transform invariants (orthonormality of each
+phase, band localisation of a tone).  The GPU part checks the HIP transform
+against the oracle within 1e-6 relative to the map's largest coefficient.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _fixture():
+    with open(os.path.join(HERE, "golden", "wdm_fourier_coefficients.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("beta", [2, 3, 4, 5, 6, 7])
+def test_fourier_coefficients_known_answers(oracle, beta):
+    from pycwb_amd.wdm import COS_SIZE, edge_fourier
+
+    fx = _fixture()
+    ref = np.array(fx["cos"][str(beta)])
+    assert COS_SIZE[beta] == fx["size"][str(beta)]
+    np.testing.assert_allclose(oracle.wdm_fourier(beta)[:ref.size], ref, rtol=0, atol=5e-14)
+    np.testing.assert_allclose(edge_fourier(beta)[:ref.size], ref, rtol=0, atol=5e-14)
+
+
+@pytest.mark.parametrize("M,beta,prec", [(4, 6, 10), (8, 6, 10), (32, 4, 8), (64, 6, 10),
+                                         (256, 2, 10)])
+def test_filter_rule_and_agreement(oracle, M, beta, prec):
+    from pycwb_amd.wdm import wdm_filter
+
+    h = wdm_filter(M, M, beta, prec)
+    ho = oracle.wdm_filter(M, M, beta, prec)
+    assert h.size == ho.size
+    np.testing.assert_allclose(h, ho, rtol=0, atol=1e-14)
+    m_H = h.size
+    assert (m_H - 1) % (2 * M) == 0 and m_H // (2 * M) >= 3
+    norm = h[0] ** 2 + 2 * np.sum(h[1:] ** 2)
+    assert 1 - norm <= 10.0 ** -prec
+    # the previous admissible length (m_H - 2M) would not have passed the rule
+    prev = m_H - 2 * M
+    norm_prev = h[0] ** 2 + 2 * np.sum(h[1:prev] ** 2)
+    assert 1 - norm_prev > 10.0 ** -prec or prev // (2 * M) < 3
+
+
+@pytest.mark.parametrize("M", [8, 32])
+@pytest.mark.parametrize("freq", [0.11, 0.25, 0.3])
+def test_oracle_t2w_phase_orthonormality(oracle, M, freq):
+    """Away from the half-band edge layers (0 and M, which cWB writes into both
+    phases), each phase of the transform is an orthonormal basis: for a
+    band-limited pulse in the middle of the series sum |c_00|^2 = sum |c_90|^2
+    = sum x^2."""
+    from pycwb_amd.wdm import wdm_filter
+
+    h = wdm_filter(M, M, 6, 10)
+    L = 256 * M + 8 * h.size
+    t = np.arange(L)
+    x = np.exp(-0.5 * ((t - L // 2) / (6 * M)) ** 2) * np.cos(2 * np.pi * freq * t)
+    m00, m90 = oracle.wdm_t2w(x, M, h, -1)
+    e = np.sum(x ** 2)
+    assert np.sum(m00 ** 2) == pytest.approx(e, rel=1e-5)
+    assert np.sum(m90 ** 2) == pytest.approx(e, rel=1e-5)
+
+
+def test_oracle_t2w_tone_localised(oracle):
+    """A tone at the centre frequency of layer m puts most energy in layer m."""
+    from pycwb_amd.wdm import wdm_filter
+
+    M, m, fs = 16, 5, 1.0
+    h = wdm_filter(M, M, 6, 10)
+    df = fs / (2 * M)
+    t = np.arange(4096)
+    x = np.cos(2 * np.pi * m * df * t + 0.3)
+    m00, m90 = oracle.wdm_t2w(x, M, h, -1)
+    core = slice(h.size // M + 2, m00.shape[0] - h.size // M - 2)
+    E = (m00[core] ** 2 + m90[core] ** 2).sum(axis=0)
+    assert E[m] / E.sum() > 0.9
+
+
+def test_oracle_power_map_matches_quadrature(oracle):
+    """MM = 0 power map = |X|^2 of the same folds; here via the quadrature
+    maps: 00^2 + 90^2 = 2 |X|^2 away from the edge layers."""
+    from pycwb_amd.wdm import wdm_filter
+
+    M = 8
+    h = wdm_filter(M, M, 6, 10)
+    x = np.random.default_rng(1).normal(size=1000)
+    m00, m90 = oracle.wdm_t2w(x, M, h, -1)
+    pw, _ = oracle.wdm_t2w(x, M, h, 0)
+    np.testing.assert_allclose((m00 ** 2 + m90 ** 2)[:, 1:M], 2 * pw[:, 1:M], rtol=1e-9,
+                               atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("M,L,S", [(2, 3001, 2), (8, 8192, 3), (16, 5000, 1), (64, 20000, 2),
+                                   (256, 65536, 3), (1024, 200000, 1)])
+def test_gpu_t2w_matches_oracle(oracle, M, L, S):
+    import torch
+
+    from pycwb_amd.wdm import WDM
+
+    print(f"case t2w M={M} L={L} S={S}", flush=True)
+    w = WDM(M, M, 6, 10)
+    rng = np.random.default_rng(M + L)
+    x = rng.normal(size=(S, L))
+    g = w.t2w_device(torch.from_numpy(x).to("cuda:0"), MM=-1).cpu().numpy()
+    for s in range(S):
+        m00, m90 = oracle.wdm_t2w(x[s], M, w.filter, -1)
+        scale = max(np.abs(m00).max(), np.abs(m90).max())
+        np.testing.assert_allclose(g[s].real, m00.T, rtol=0, atol=1e-6 * scale)
+        np.testing.assert_allclose(g[s].imag, m90.T, rtol=0, atol=1e-6 * scale)
+        err = max(np.abs(g[s].real - m00.T).max(), np.abs(g[s].imag - m90.T).max()) / scale
+        assert err < 1e-12  # float64 throughout; observed error is rounding-level
+    pw = w.t2w_device(torch.from_numpy(x).to("cuda:0"), MM=0).cpu().numpy()
+    p_o, _ = oracle.wdm_t2w(x[0], M, w.filter, 0)
+    np.testing.assert_allclose(pw[0].real, p_o.T, rtol=1e-9, atol=1e-12 * p_o.max())
+
+
+@pytest.mark.gpu
+def test_gpu_wdm_object_surface():
+    from pycwb_amd.wdm import WDM
+
+    w = WDM(32, 32, 6, 10)
+    x = np.random.default_rng(3).normal(size=4096)
+    tf = w.t2w(x, sample_rate=1024.0, t0=10.0, MM=-1)
+    assert tf.data.shape == (33, 128) and tf.data.dtype == np.complex128
+    assert tf.dt == 32 / 1024.0 and tf.df == 1024.0 / 64
+    assert tf.start_time == 10.0 and tf.len_timeseries == 4096
