@@ -43,6 +43,15 @@ def scan_flops(batch, n_valid):
     return float(np.sum(batch.n_pix.astype(np.float64) * (78.0 * batch.nifo + 129.0))) * n_valid
 
 
+def required_scan_flops(batch, n_valid, active):
+    """Flops the scan needs on this data: the energy test (4n + 4) on every
+    evaluation and the rest of the chain (74n + 125) on the evaluations above
+    the energy threshold; the others contribute exactly nothing."""
+    n = batch.nifo.astype(np.float64)
+    every = float(np.sum(batch.n_pix.astype(np.float64) * (4.0 * n + 4.0))) * n_valid
+    return every + float(np.sum(active.astype(np.float64) * (74.0 * n + 125.0)))
+
+
 def regulator_flops(batch, n_valid):
     return float(np.sum(batch.n_pix.astype(np.float64) * (24.0 * batch.nifo + 25.0))) * n_valid
 
@@ -158,9 +167,14 @@ def main():
     reg_ms = e[0].elapsed_time(e[1])
     scan_ms = e[1].elapsed_time(e[2])
     rs = bs.results()
+    active = bs.active_pairs()
     sf = scan_flops(batch, n_valid)
-    achieved = sf / (scan_ms * 1e-3) / 1e12
-    log(f"scan stage {scan_ms:.1f} ms ({achieved:.2f} TFLOP/s), regulator {reg_ms:.1f} ms")
+    rf = required_scan_flops(batch, n_valid, active)
+    achieved = rf / (scan_ms * 1e-3) / 1e12
+    act_frac = float(active.sum()) / float(batch.evals(n_valid))
+    log(f"scan stage {scan_ms:.1f} ms ({achieved:.2f} TFLOP/s required work, "
+        f"{sf / (scan_ms * 1e-3) / 1e12:.2f} nominal), active fraction {act_frac:.4f}, "
+        f"regulator {reg_ms:.1f} ms")
 
     plain = None
     parity = {"batched_vs_plain_l_max": None}
@@ -214,8 +228,12 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": None, "kernel": "k_batch_persistent<N,BLOCK,1> (scan stage)",
-                     "kernel_ms": round(scan_ms, 3), "flops": sf,
-                     "algorithmic_bytes": scan_bytes(batch, nt, n_valid)},
+                     "kernel_ms": round(scan_ms, 3), "required_flops": rf,
+                     "nominal_flops": sf,
+                     "nominal_tflops": sf / (scan_ms * 1e-3) / 1e12,
+                     "active_fraction": act_frac,
+                     "algorithmic_bytes": scan_bytes(batch, nt, n_valid),
+                     "hbm_gbs": scan_bytes(batch, nt, n_valid) / (scan_ms * 1e-3) / 1e9},
         "regulator": {"ms": round(reg_ms, 3),
                       "tflops": regulator_flops(batch, n_valid) / (reg_ms * 1e-3) / 1e12},
         "cpu_baseline": cpu,

@@ -35,6 +35,17 @@ extern "C" {
 #define PYCWB_ERR_ARG 1
 #define PYCWB_ERR_HIP 2
 
+/* flags of the regulator / scan entry points.  Both are exact: results are
+ * bit-identical with and without them, given finite inputs (checked by the
+ * caller).
+ *   PYCWB_SKIP_MASKED_PIXELS: a pixel whose energy at a direction's delays is
+ *     below the network threshold adds nothing to any sum, so its evaluation
+ *     stops after the energy test;
+ *   PYCWB_REGULATOR_EARLY_EXIT: a direction's DPF sweep stops once its partial
+ *     quality already exceeds gamma with every remaining term >= 0. */
+#define PYCWB_SKIP_MASKED_PIXELS 1
+#define PYCWB_REGULATOR_EARLY_EXIT 2
+
 const char* pycwb_version(void);
 
 /* Text of the last PYCWB_ERR_HIP raised on the calling thread (HIP error name
@@ -59,15 +70,19 @@ int pycwb_td_relayout(int n_ifo, int n_pix, int n_delay, const float* td00, cons
 int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, const float* FX,
                         const float* rms, double gamma_regulator,
                         double network_energy_threshold, float reg0,
-                        const int64_t* sky_valid_indices, int64_t n_valid, float* REG,
-                        double* reg_f64, void* workspace, size_t workspace_bytes,
+                        const int64_t* sky_valid_indices, int64_t n_valid, int flags,
+                        float* REG, double* reg_f64, void* workspace, size_t workspace_bytes,
                         void* stream);
 
 /* Sky scan of one cluster.  Replaces scan_sky_for_best_fit
  * (pycwb/modules/likelihoodWP/sky_scan.py:17-192; called at likelihood.py:319).
  * Reads the amplitude pairs from td_packed ((n_pix, n_ifo, n_delay) float
  * pairs) or, when td_packed is NULL, from the workspace filled by
- * pycwb_td_relayout (workspace >= 256 bytes suffices with td_packed).  maps may be NULL
+ * pycwb_td_relayout (workspace >= 256 bytes suffices with td_packed).
+ * sky_order (may be NULL) is a permutation of valid-list positions: the order
+ * in which lanes take directions (directions sharing delay taps side by side
+ * keep PYCWB_SKIP_MASKED_PIXELS branches coherent); results are indexed by
+ * sky pixel and tie-broken by valid-list position either way.  maps may be NULL
  * (argmax only).  Writes *l_max (int64) and *sky_stat_max (float64) on the
  * device.  A direction with no pixel above the energy threshold is treated as
  * gated (the reference divides 0/0 there). */
@@ -75,8 +90,9 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
                          const float* FX, const int32_t* ml, const float* rms,
                          const float* td_packed, const float* REG, double netCC,
                          double network_energy_threshold, const int64_t* sky_valid_indices,
-                         int64_t n_valid, float* maps, int64_t* l_max, double* sky_stat_max,
-                         void* workspace, size_t workspace_bytes, void* stream);
+                         int64_t n_valid, const int32_t* sky_order, int flags, float* maps,
+                         int64_t* l_max, double* sky_stat_max, void* workspace,
+                         size_t workspace_bytes, void* stream);
 
 /* Batched regulator + scan + argmax over many clusters, as one hipGraph of
  * persistent work-queue kernels.  Replaces the per-cluster loop of
@@ -115,6 +131,8 @@ typedef struct pycwb_batch_desc {
   const float* FP;
   const float* FX;
   const int32_t* ml;
+  const int32_t* sky_order; /* NULL or (n_networks, n_valid) valid-list positions */
+  int flags;                /* PYCWB_SKIP_MASKED_PIXELS | PYCWB_REGULATOR_EARLY_EXIT */
   double gamma_regulator;
   float reg0;
   double netCC;
@@ -135,7 +153,10 @@ typedef struct pycwb_batch pycwb_batch;
 int pycwb_batch_create(const pycwb_batch_desc* desc, pycwb_batch** out);
 int pycwb_batch_launch(pycwb_batch* batch, void* stream);
 /* One stage of the batch outside the graph (0 init, 1 regulator, 2 scan,
- * 3 finish), for per-stage timing; stages must run in order. */
+ * 3 finish), for per-stage timing; stages must run in order.  Stage 4, run
+ * right after stage 0 instead of 1-3, writes into the per-cluster counts the
+ * number of (direction, pixel) pairs above the energy threshold (the pairs
+ * the scan evaluates in full); it is a measurement aid. */
 int pycwb_batch_launch_stage(pycwb_batch* batch, int stage, void* stream);
 int pycwb_batch_destroy(pycwb_batch* batch);
 

@@ -17,6 +17,9 @@ LIB_PATH = os.path.join(_HERE, "libpycwb_amd.so")
 PYCWB_OK = 0
 PYCWB_ERR_ARG = 1
 PYCWB_ERR_HIP = 2
+SKIP_MASKED_PIXELS = 1
+REGULATOR_EARLY_EXIT = 2
+DEFAULT_FLAGS = SKIP_MASKED_PIXELS | REGULATOR_EARLY_EXIT
 
 _lock = threading.Lock()
 _lib = None
@@ -53,6 +56,8 @@ class BatchDesc(C.Structure):
         ("FP", vp),
         ("FX", vp),
         ("ml", vp),
+        ("sky_order", vp),
+        ("flags", i32),
         ("gamma_regulator", f64),
         ("reg0", f32),
         ("netCC", f64),
@@ -85,10 +90,10 @@ SIGNATURES = {
     "pycwb_last_error": (C.c_char_p, []),
     "pycwb_sky_scan_workspace_bytes": (sz, [i32, i32, i32]),
     "pycwb_td_relayout": (i32, [i32, i32, i32, vp, vp, vp, sz, vp]),
-    "pycwb_dpf_regulator": (i32, [i32, i32, i64, vp, vp, vp, f64, f64, f32, vp, i64, vp, vp,
-                                  vp, sz, vp]),
+    "pycwb_dpf_regulator": (i32, [i32, i32, i64, vp, vp, vp, f64, f64, f32, vp, i64, i32, vp,
+                                  vp, vp, sz, vp]),
     "pycwb_sky_scan_plain": (i32, [i32, i32, i64, i32, vp, vp, vp, vp, vp, vp, f64, f64, vp,
-                                   i64, vp, vp, vp, vp, sz, vp]),
+                                   i64, vp, i32, vp, vp, vp, vp, sz, vp]),
     "pycwb_batch_create": (i32, [C.POINTER(BatchDesc), C.POINTER(vp)]),
     "pycwb_batch_launch": (i32, [vp, vp]),
     "pycwb_batch_launch_stage": (i32, [vp, i32, vp]),

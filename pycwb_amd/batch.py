@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import _lib, synth
+from . import _lib, sky, synth
 from .device import lib, ptr, require_gpu, stream_ptr
 
 
@@ -37,6 +37,19 @@ class NetTablesDev:
     nifo_d: torch.Tensor       # int32 (n_net,)
     En_d: torch.Tensor         # float64 (n_net,)
     K: list = field(default_factory=list)
+    ml_host: list = field(default_factory=list)   # per network (n, S) int32
+
+    def sky_order(self, valid_h: np.ndarray, device) -> torch.Tensor:
+        """(n_net, n_valid) valid-list positions in delay-tuple order per network."""
+        key = ("order", valid_h.tobytes())
+        if getattr(self, "_order_key", None) != key:
+            o = np.stack([sky.delay_order(m, valid_h) for m in self.ml_host])
+            self._order = torch.from_numpy(np.ascontiguousarray(o)).to(device)
+            self._order_key = key
+        return self._order
+
+    def finite(self) -> bool:
+        return bool(torch.isfinite(self.FP).all()) and bool(torch.isfinite(self.FX).all())
 
 
 def net_tables(nets, acore: float = math.sqrt(2.0), device=None) -> NetTablesDev:
@@ -59,7 +72,8 @@ def net_tables(nets, acore: float = math.sqrt(2.0), device=None) -> NetTablesDev
         off=torch.tensor(offs, dtype=torch.int64, device=device),
         FP=t(fp, torch.float32), FX=t(fx, torch.float32), ml=t(ml, torch.int32),
         nifo_d=torch.tensor([n.n_ifo for n in nets], dtype=torch.int32, device=device),
-        En_d=torch.tensor(En, dtype=torch.float64, device=device), K=[n.K for n in nets])
+        En_d=torch.tensor(En, dtype=torch.float64, device=device), K=[n.K for n in nets],
+        ml_host=[np.ascontiguousarray(n.ml, np.int32) for n in nets])
 
 
 @dataclass
@@ -186,7 +200,8 @@ class BatchedSkyScan:
 
     def __init__(self, batch: ClusterBatch, nets: NetTablesDev, valid: torch.Tensor,
                  gamma_regulator: float, delta_regulator: float, netCC: float,
-                 block: int = 256, grid: int | None = None, use_graph: bool = True):
+                 block: int = 256, grid: int | None = None, use_graph: bool = True,
+                 flags: int | None = None, ordered: bool = True):
         dev = batch.td.device
         self.batch, self.nets, self.valid = batch, nets, valid
         C_ = batch.n_clusters
@@ -218,6 +233,13 @@ class BatchedSkyScan:
         d.rms, d.td = ptr(batch.rms), ptr(batch.td)
         d.net_nifo, d.net_off, d.net_En = ptr(nets.nifo_d), ptr(nets.off), ptr(nets.En_d)
         d.FP, d.FX, d.ml = ptr(nets.FP), ptr(nets.FX), ptr(nets.ml)
+        self.sky_order = nets.sky_order(valid.cpu().numpy(), dev) if ordered else None
+        d.sky_order = ptr(self.sky_order)
+        if flags is None:
+            finite = nets.finite() and bool(torch.isfinite(batch.rms).all())
+            flags = _lib.DEFAULT_FLAGS if finite else _lib.REGULATOR_EARLY_EXIT
+        d.flags = int(flags)
+        self.flags = d.flags
         d.gamma_regulator = float(gamma_regulator)
         d.reg0 = float(np.float32(delta_regulator * np.sqrt(2)))
         d.netCC = float(netCC)
@@ -245,6 +267,15 @@ class BatchedSkyScan:
         return {"l_max": self.l_max.cpu().numpy(), "sky_stat_max": self.stat_max.cpu().numpy(),
                 "REG1": self.REG1.cpu().numpy(), "regulator": self.reg_f64.cpu().numpy()}
 
+    def active_pairs(self) -> np.ndarray:
+        """Per cluster, the (direction, pixel) pairs above the energy threshold
+        (those the scan evaluates in full).  Overwrites the regulator counts, so
+        call it after reading results()."""
+        self.launch_stage(0)
+        self.launch_stage(4)
+        torch.cuda.current_stream().synchronize()
+        return self.counts.cpu().numpy().astype(np.uint32).astype(np.int64)
+
     def close(self) -> None:
         if self.handle is not None and self.handle.value:
             lib().pycwb_batch_destroy(self.handle)
@@ -261,7 +292,8 @@ class PlainLoop:
     """The same clusters, one plain launch family (regulator, scan) each."""
 
     def __init__(self, batch: ClusterBatch, nets: NetTablesDev, valid: torch.Tensor,
-                 gamma_regulator: float, delta_regulator: float, netCC: float):
+                 gamma_regulator: float, delta_regulator: float, netCC: float,
+                 flags: int | None = None, ordered: bool = True):
         dev = batch.td.device
         self.batch, self.nets, self.valid = batch, nets, valid
         C_ = batch.n_clusters
@@ -272,6 +304,11 @@ class PlainLoop:
         self.reg_f64 = torch.zeros(C_, dtype=torch.float64, device=dev)
         self.l_max = torch.zeros(C_, dtype=torch.int64, device=dev)
         self.stat_max = torch.zeros(C_, dtype=torch.float64, device=dev)
+        self.order = nets.sky_order(valid.cpu().numpy(), dev) if ordered else None
+        if flags is None:
+            finite = nets.finite() and bool(torch.isfinite(batch.rms).all())
+            flags = _lib.DEFAULT_FLAGS if finite else _lib.REGULATOR_EARLY_EXIT
+        self.flags = int(flags)
         offs = nets.off.cpu().numpy()
         self.net_views = []
         S = nets.n_sky
@@ -295,12 +332,13 @@ class PlainLoop:
             td = b.td.data_ptr() + 8 * int(b.td_off[c])
             REG = self.REG.data_ptr() + 12 * c
             rc = L.pycwb_dpf_regulator(n, P, S, ptr(FP), ptr(FX), rms, self.gamma,
-                                       nets.En[k], self.reg0, vp, nv, REG,
+                                       nets.En[k], self.reg0, vp, nv, self.flags, REG,
                                        self.reg_f64.data_ptr() + 8 * c, ws, wsn, st)
             _lib.check(rc, "pycwb_dpf_regulator")
+            order = None if self.order is None else self.order[k].data_ptr()
             rc = L.pycwb_sky_scan_plain(n, P, S, 2 * K + 1, ptr(FP), ptr(FX), ptr(ml), rms, td,
-                                        REG, self.netCC, nets.En[k], vp, nv, None,
-                                        self.l_max.data_ptr() + 8 * c,
+                                        REG, self.netCC, nets.En[k], vp, nv, order, self.flags,
+                                        None, self.l_max.data_ptr() + 8 * c,
                                         self.stat_max.data_ptr() + 8 * c, ws, wsn, st)
             _lib.check(rc, "pycwb_sky_scan_plain")
 

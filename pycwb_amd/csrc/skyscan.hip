@@ -103,11 +103,15 @@ __device__ __forceinline__ void block_count(unsigned v, unsigned* dst) {
 // ---------------------------------------------------------------------------
 
 // DPF quality of the direction valid[k]; returns 1 if quality > gamma.
+// Every term fx / (ni + 1e-9) is >= 0, so the float32 partial sums are
+// non-decreasing and NN <= P; once sqrt(NI_partial / (P + 0.01)) > gamma the
+// final quality sqrt(NI / (NN + 0.01)) exceeds gamma too (correctly rounded
+// division and sqrt are monotone), and the sweep can stop with the same count.
 template <int N>
 __device__ __forceinline__ unsigned regulator_lane(const float* __restrict__ FP,
                                                    const float* __restrict__ FX,
                                                    const float* __restrict__ rms, int P,
-                                                   int64_t s, double gamma) {
+                                                   int64_t s, double gamma, bool early) {
   float fp0[N], fx0[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -116,11 +120,13 @@ __device__ __forceinline__ unsigned regulator_lane(const float* __restrict__ FP,
   }
   float NI = 0.f;
   unsigned NN = 0;
+  const double bound_den = (double)P + 0.01;
   for (int j = 0; j < P; ++j) {
     float r[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) r[i] = rms[(int64_t)j * N + i];
     regulator_pixel<N>(r, fp0, fx0, NI, NN);
+    if (early && (j & 15) == 15 && __builtin_sqrt((double)NI / bound_den) > gamma) return 1u;
   }
   const double q = __builtin_sqrt((double)NI / ((double)NN + 0.01));
   return q > gamma ? 1u : 0u;
@@ -132,7 +138,7 @@ __device__ __forceinline__ unsigned long long scan_lane(
     const float* __restrict__ FP, const float* __restrict__ FX, const int32_t* __restrict__ ml,
     int64_t S, const float* __restrict__ rms, const float2* __restrict__ td, int P, int K,
     float rr, float RR, double En, double netCC, int64_t s, int64_t k, bool active,
-    float* __restrict__ maps) {
+    bool skip_masked, float* __restrict__ maps) {
   const int nd = 2 * K + 1;
   float fp0[N], fx0[N];
   int tap[N];
@@ -163,7 +169,7 @@ __device__ __forceinline__ unsigned long long scan_lane(
       v90[i] = cur[i].y;
       r[i] = rms[(int64_t)j * N + i];
     }
-    scan_pixel<N>(v00, v90, r, fp0, fx0, rr, RR, En, acc);
+    scan_pixel<N>(v00, v90, r, fp0, fx0, rr, RR, En, skip_masked, acc);
 #pragma unroll
     for (int i = 0; i < N; ++i) cur[i] = nxt[i];
   }
@@ -214,10 +220,11 @@ template <int N, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_regulator_plain(
     const float* __restrict__ FP, const float* __restrict__ FX, const float* __restrict__ rms,
     int P, const int64_t* __restrict__ valid, int64_t n_valid, double gamma,
-    unsigned* __restrict__ count) {
+    unsigned* __restrict__ count, int flags) {
   const int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   unsigned v = 0;
-  if (k < n_valid) v = regulator_lane<N>(FP, FX, rms, P, valid[k], gamma);
+  if (k < n_valid)
+    v = regulator_lane<N>(FP, FX, rms, P, valid[k], gamma, (flags & PYCWB_REGULATOR_EARLY_EXIT) != 0);
   block_count<BLOCK>(v, count);
 }
 
@@ -238,12 +245,17 @@ __global__ __launch_bounds__(BLOCK) void k_scan_plain(
     const float* __restrict__ FP, const float* __restrict__ FX, const int32_t* __restrict__ ml,
     int64_t S, const float* __restrict__ rms, const float2* __restrict__ td, int P, int K,
     const float* __restrict__ REG, double En, double netCC, const int64_t* __restrict__ valid,
-    int64_t n_valid, float* __restrict__ maps, unsigned long long* __restrict__ best) {
-  const int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = k < n_valid;
-  const int64_t s = valid[active ? k : 0];
+    int64_t n_valid, const int32_t* __restrict__ perm, int flags, float* __restrict__ maps,
+    unsigned long long* __restrict__ best) {
+  const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = t < n_valid;
+  // lane t evaluates valid-list position k (delay-tuple order when perm is given)
+  int64_t k = active ? (perm ? (int64_t)perm[t] : t) : 0;
+  k = k < 0 ? 0 : (k >= n_valid ? n_valid - 1 : k);
+  const int64_t s = valid[k];
   const unsigned long long key =
-      scan_lane<N>(FP, FX, ml, S, rms, td, P, K, REG[0], REG[1], En, netCC, s, k, active, maps);
+      scan_lane<N>(FP, FX, ml, S, rms, td, P, K, REG[0], REG[1], En, netCC, s, k, active,
+                   (flags & PYCWB_SKIP_MASKED_PIXELS) != 0, maps);
   block_argmax<BLOCK>(key, best);
 }
 
@@ -291,6 +303,8 @@ struct BatchDev {
   const float* FP;
   const float* FX;
   const int32_t* ml;
+  const int32_t* perm;      // per network: valid-list positions in delay-tuple order
+  int flags;
   double gamma;
   float rr;
   double netCC;
@@ -325,24 +339,58 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev b, int firs
     if (item >= n_items) break;  // every block reaches this once the tickets run out
     const int c = b.order[first + (int)(item / (unsigned)b.tiles_per_cluster)];
     const int tile = (int)(item % (unsigned)b.tiles_per_cluster);
-    const int64_t k = (int64_t)tile * BLOCK + threadIdx.x;
-    const bool active = k < b.n_valid;
-    const int64_t s = b.valid[active ? k : 0];
+    const int64_t t = (int64_t)tile * BLOCK + threadIdx.x;
+    const bool active = t < b.n_valid;
     const int net = b.net[c];
+    int64_t k = active ? (b.perm ? (int64_t)b.perm[(int64_t)net * b.n_valid + t] : t) : 0;
+    k = k < 0 ? 0 : (k >= b.n_valid ? b.n_valid - 1 : k);
+    const int64_t s = b.valid[k];
     const float* FP = b.FP + b.net_off[net];
     const float* FX = b.FX + b.net_off[net];
     const float* rms = b.rms + b.rms_off[c];
     const int P = b.npix[c];
-    if (PHASE == 0) {
-      const unsigned v = active ? regulator_lane<N>(FP, FX, rms, P, s, b.gamma) : 0u;
+    if (PHASE == 2) {
+      // energy test only: how many (direction, pixel) pairs take the full path
+      const int K = b.K[c], nd = 2 * K + 1;
+      const int32_t* ml = b.ml + b.net_off[net];
+      const float2* td = b.td + b.td_off[c];
+      const double En = b.net_En[net];
+      int tap[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        int t = ml[(int64_t)i * b.S + s] + K;
+        tap[i] = i * nd + (t < 0 ? 0 : (t >= nd ? nd - 1 : t));
+      }
+      unsigned cnt = 0;
+      if (active) {
+        for (int j = 0; j < P; ++j) {
+          float v00[N], v90[N];
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            const float2 v = td[(int64_t)j * N * nd + tap[i]];
+            v00[i] = v.x;
+            v90[i] = v.y;
+          }
+          float e, et;
+          int m;
+          pixel_energy<N>(v00, v90, En, e, m, et);
+          cnt += pixel_contributes(e, m) ? 1u : 0u;
+        }
+      }
+      block_count<BLOCK>(cnt, &b.counts[c]);
+    } else if (PHASE == 0) {
+      const unsigned v = active ? regulator_lane<N>(FP, FX, rms, P, s, b.gamma,
+                                                    (b.flags & PYCWB_REGULATOR_EARLY_EXIT) != 0)
+                                : 0u;
       block_count<BLOCK>(v, &b.counts[c]);
     } else {
       const double En = b.net_En[net];
       const float RR = (float)regulator_value(b.n_valid, b.counts[c], En);
       const int32_t* ml = b.ml + b.net_off[net];
       const float2* td = b.td + b.td_off[c];
-      const unsigned long long key = scan_lane<N>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR,
-                                                  En, b.netCC, s, k, active, nullptr);
+      const unsigned long long key =
+          scan_lane<N>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR, En, b.netCC, s, k, active,
+                       (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0, nullptr);
       block_argmax<BLOCK>(key, &b.best[c]);
     }
   }
@@ -372,13 +420,13 @@ __global__ void k_batch_finish(BatchDev b) {
 template <int BLOCK>
 int launch_regulator_plain(int n, const float* FP, const float* FX, const float* rms, int P,
                            const int64_t* valid, int64_t n_valid, double gamma,
-                           unsigned* count, hipStream_t st) {
+                           unsigned* count, int flags, hipStream_t st) {
   const unsigned grid = (unsigned)((n_valid + BLOCK - 1) / BLOCK);
   switch (n) {
 #define CASE(NN)                                                                          \
   case NN:                                                                                \
     hipLaunchKernelGGL((k_regulator_plain<NN, BLOCK>), dim3(grid), dim3(BLOCK), 0, st, FP, \
-                       FX, rms, P, valid, n_valid, gamma, count);                         \
+                       FX, rms, P, valid, n_valid, gamma, count, flags);                  \
     break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
@@ -392,13 +440,15 @@ template <int BLOCK>
 int launch_scan_plain(int n, const float* FP, const float* FX, const int32_t* ml, int64_t S,
                       const float* rms, const float2* td, int P, int K, const float* REG,
                       double En, double netCC, const int64_t* valid, int64_t n_valid,
-                      float* maps, unsigned long long* best, hipStream_t st) {
+                      const int32_t* perm, int flags, float* maps, unsigned long long* best,
+                      hipStream_t st) {
   const unsigned grid = (unsigned)((n_valid + BLOCK - 1) / BLOCK);
   switch (n) {
 #define CASE(NN)                                                                          \
   case NN:                                                                                \
     hipLaunchKernelGGL((k_scan_plain<NN, BLOCK>), dim3(grid), dim3(BLOCK), 0, st, FP, FX, \
-                       ml, S, rms, td, P, K, REG, En, netCC, valid, n_valid, maps, best); \
+                       ml, S, rms, td, P, K, REG, En, netCC, valid, n_valid, perm, flags,  \
+                       maps, best);                                                        \
     break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
@@ -500,8 +550,8 @@ int pycwb_td_relayout(int n_ifo, int n_pix, int n_delay, const float* td00, cons
 int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, const float* FX,
                         const float* rms, double gamma_regulator,
                         double network_energy_threshold, float reg0,
-                        const int64_t* sky_valid_indices, int64_t n_valid, float* REG,
-                        double* reg_f64, void* workspace, size_t workspace_bytes,
+                        const int64_t* sky_valid_indices, int64_t n_valid, int flags,
+                        float* REG, double* reg_f64, void* workspace, size_t workspace_bytes,
                         void* stream) {
   clear_stale_error();
   if (!valid_args(n_ifo, n_pix, n_sky, n_valid)) return PYCWB_ERR_ARG;
@@ -512,7 +562,7 @@ int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, co
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (float*)w.count, (int64_t)1, 0.f);
   HIP_TRY(hipGetLastError());
   int rc = launch_regulator_plain<256>(n_ifo, FP, FX, rms, n_pix, sky_valid_indices, n_valid,
-                                       gamma_regulator, w.count, st);
+                                       gamma_regulator, w.count, flags, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_regulator_finish, dim3(1), dim3(64), 0, st, w.count, n_valid,
                      network_energy_threshold, reg0, REG, reg_f64);
@@ -524,8 +574,9 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
                          const float* FX, const int32_t* ml, const float* rms,
                          const float* td_packed, const float* REG, double netCC,
                          double network_energy_threshold, const int64_t* sky_valid_indices,
-                         int64_t n_valid, float* maps, int64_t* l_max, double* sky_stat_max,
-                         void* workspace, size_t workspace_bytes, void* stream) {
+                         int64_t n_valid, const int32_t* sky_order, int flags, float* maps,
+                         int64_t* l_max, double* sky_stat_max, void* workspace,
+                         size_t workspace_bytes, void* stream) {
   clear_stale_error();
   if (!valid_args(n_ifo, n_pix, n_sky, n_valid) || n_delay < 1 || !(n_delay & 1))
     return PYCWB_ERR_ARG;
@@ -547,7 +598,7 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
   const int K = n_delay / 2;
   int rc = launch_scan_plain<256>(n_ifo, FP, FX, ml, n_sky, rms, td, n_pix, K, REG,
                                   network_energy_threshold, netCC, sky_valid_indices, n_valid,
-                                  maps, w.best, st);
+                                  sky_order, flags, maps, w.best, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_argmax_finish, dim3(1), dim3(64), 0, st, w.best, sky_valid_indices,
                      l_max, sky_stat_max);
@@ -609,6 +660,8 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
   v.FP = d->FP;
   v.FX = d->FX;
   v.ml = d->ml;
+  v.perm = d->sky_order;
+  v.flags = d->flags;
   v.gamma = d->gamma_regulator;
   v.rr = d->reg0;
   v.netCC = d->netCC;
@@ -658,7 +711,7 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
 }
 
 int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
-  if (!b || stage < 0 || stage > 3) return PYCWB_ERR_ARG;
+  if (!b || stage < 0 || stage > 4) return PYCWB_ERR_ARG;
   clear_stale_error();
   hipStream_t st = (hipStream_t)stream;
   const BatchDev& d = b->dev;
@@ -674,20 +727,25 @@ int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
     HIP_TRY(hipGetLastError());
     return PYCWB_OK;
   }
-  const int phase = stage - 1;
+  // stage 4: count pass (energy test only) into counts[]; uses the regulator
+  // tickets, so it runs after stage 0 and before anything that reads counts.
+  const int phase = stage == 4 ? 2 : stage - 1;
   for (int g = 0; g < b->n_groups; ++g) {
-    unsigned* ticket = d.tickets + 2 * g + phase;
+    unsigned* ticket = d.tickets + 2 * g + (phase == 1 ? 1 : 0);
     int rc = PYCWB_ERR_ARG;
     const int n = b->group_nifo[g], first = b->group_first[g], cnt = b->group_count[g];
-    if (b->block == 64)
-      rc = phase ? launch_group<64, 1>(n, d, b->grid, first, cnt, ticket, st)
-                 : launch_group<64, 0>(n, d, b->grid, first, cnt, ticket, st);
-    else if (b->block == 128)
-      rc = phase ? launch_group<128, 1>(n, d, b->grid, first, cnt, ticket, st)
-                 : launch_group<128, 0>(n, d, b->grid, first, cnt, ticket, st);
-    else if (b->block == 256)
-      rc = phase ? launch_group<256, 1>(n, d, b->grid, first, cnt, ticket, st)
-                 : launch_group<256, 0>(n, d, b->grid, first, cnt, ticket, st);
+#define PHASES(BLK)                                                                \
+  rc = phase == 2 ? launch_group<BLK, 2>(n, d, b->grid, first, cnt, ticket, st)   \
+       : phase ? launch_group<BLK, 1>(n, d, b->grid, first, cnt, ticket, st)      \
+               : launch_group<BLK, 0>(n, d, b->grid, first, cnt, ticket, st);
+    if (b->block == 64) {
+      PHASES(64)
+    } else if (b->block == 128) {
+      PHASES(128)
+    } else if (b->block == 256) {
+      PHASES(256)
+    }
+#undef PHASES
     if (rc) return rc;
   }
   return PYCWB_OK;

@@ -122,22 +122,36 @@ struct PixelEval {
   double ll, ccd;  // signal energy and correlation factor (float64)
 };
 
-// load_data_from_td, DPF, avx_GW_ps, avx_ort_ps and avx_stat_ps of one pixel.
+// load_data_from_td of one pixel (sky_stat.py:74-87): energy, mask, masked energy.
 template <int N>
-__device__ __forceinline__ void pixel_eval(const float (&v00)[N], const float (&v90)[N],
-                                           const float (&r)[N], const float (&fp0)[N],
-                                           const float (&fx0)[N], float rr, float RR,
-                                           double En, PixelEval<N>& o) {
-  // --- load_data_from_td (sky_stat.py:74-87) ---
+__device__ __forceinline__ void pixel_energy(const float (&v00)[N], const float (&v90)[N],
+                                             double En, float& e, int& m, float& et) {
   float aa = 0.f, AA = 0.f;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     aa += v00[i] * v00[i];
     AA += v90[i] * v90[i];
   }
-  const float e = aa + AA + 1e-12f;
-  const int m = (double)e > En ? 1 : 0;
-  const float et = e * (float)m;
+  e = aa + AA + 1e-12f;
+  m = (double)e > En ? 1 : 0;
+  et = e * (float)m;
+}
+
+// A pixel below the energy threshold (m == 0) with finite energy adds exactly
+// nothing to any per-direction sum: fd = Fd = +0, so the signal packet is
+// +-0, ll = ec = rn = +0, gn = -0, ccd = +0 and the updated mask is -1.
+// Given finite noise weights, antenna patterns and regulators (checked by the
+// host), skipping such a pixel leaves every sum bit-identical.
+__device__ __forceinline__ bool pixel_contributes(float e, int m) {
+  return m != 0 || !isfinite(e);
+}
+
+// Everything after load_data_from_td for one pixel.
+template <int N>
+__device__ __forceinline__ void pixel_rest(const float (&v00)[N], const float (&v90)[N],
+                                           const float (&r)[N], const float (&fp0)[N],
+                                           const float (&fx0)[N], float rr, float RR, float e,
+                                           int m, float et, PixelEval<N>& o) {
   o.e = e;
   o.m = m;
   o.et = et;
@@ -236,15 +250,32 @@ __device__ __forceinline__ void pixel_eval(const float (&v00)[N], const float (&
   o.ccd = (double)o.ec / (a2 + (double)A2);
 }
 
+// load_data_from_td, DPF, avx_GW_ps, avx_ort_ps and avx_stat_ps of one pixel.
+template <int N>
+__device__ __forceinline__ void pixel_eval(const float (&v00)[N], const float (&v90)[N],
+                                           const float (&r)[N], const float (&fp0)[N],
+                                           const float (&fx0)[N], float rr, float RR,
+                                           double En, PixelEval<N>& o) {
+  float e, et;
+  int m;
+  pixel_energy<N>(v00, v90, En, e, m, et);
+  pixel_rest<N>(v00, v90, r, fp0, fx0, rr, RR, e, m, et, o);
+}
+
 // One pixel of the scan: pixel_eval plus the per-direction sums
-// (sky_stat.py:398-405, sky_scan.py:148-153).
+// (sky_stat.py:398-405, sky_scan.py:148-153).  With skip_masked, pixels that
+// cannot contribute (pixel_contributes) stop after the energy test.
 template <int N>
 __device__ __forceinline__ void scan_pixel(const float (&v00)[N], const float (&v90)[N],
                                            const float (&r)[N], const float (&fp0)[N],
                                            const float (&fx0)[N], float rr, float RR,
-                                           double En, SkySums& acc) {
+                                           double En, bool skip_masked, SkySums& acc) {
+  float e, et;
+  int m;
+  pixel_energy<N>(v00, v90, En, e, m, et);
+  if (skip_masked && !pixel_contributes(e, m)) return;
   PixelEval<N> o;
-  pixel_eval<N>(v00, v90, r, fp0, fx0, rr, RR, En, o);
+  pixel_rest<N>(v00, v90, r, fp0, fx0, rr, RR, e, m, et, o);
   acc.M += o.m;
   acc.EE += o.et;
   acc.Lr += o.ll * o.ccd;

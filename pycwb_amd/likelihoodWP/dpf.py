@@ -12,6 +12,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from .. import _lib
 from ..device import as_dev, require_gpu
 from .sky_scan import PlainClusterScan
 
@@ -35,6 +36,7 @@ def calculate_dpf(FP, FX, rms, n_sky, n_ifo, gamma_regulator, network_energy_thr
     sc = PlainClusterScan(n_ifo, n_pix, 1, dev)
     sc.regulator(as_dev(FP, torch.float32, dev), as_dev(FX, torch.float32, dev),
                  as_dev(rms, torch.float32, dev), n_sky, gamma_regulator,
-                 network_energy_threshold, 0.0, torch.from_numpy(valid).to(dev))
+                 network_energy_threshold, 0.0, torch.from_numpy(valid).to(dev),
+                 flags=_lib.REGULATOR_EARLY_EXIT)
     torch.cuda.current_stream().synchronize()
     return float(sc.reg_f64.item())

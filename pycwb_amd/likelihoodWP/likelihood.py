@@ -38,7 +38,7 @@ from ..skystats import SkyStatistics as _GpuSkyStats
 from ..skystats import SCALARS, self_xtalk
 from .detection_statistics import get_likelihood_rejection_reason
 from .sky_mask import compute_sky_valid_indices, sky_valid_indices_for_cluster
-from .sky_scan import MAP_NAMES, PlainClusterScan
+from .sky_scan import MAP_NAMES, PlainClusterScan, scan_flags
 from .typing import SkyMapStatistics, SkyStatistics
 
 logger = logging.getLogger(__name__)
@@ -136,7 +136,7 @@ def _device_net(setup: dict, big: bool, device) -> tuple:
             ml=t(ml, torch.int32).reshape(-1),
             nifo_d=torch.tensor([n_ifo], dtype=torch.int32, device=device),
             En_d=torch.tensor([En], dtype=torch.float64, device=device),
-            K=[int(np.abs(ml).max())])
+            K=[int(np.abs(ml).max())], ml_host=[np.ascontiguousarray(ml, np.int32)])
         setup[key] = nets
     return setup[key]
 
@@ -303,10 +303,12 @@ def evaluate_cluster_likelihood(nIFO, cluster, config, MRAcatalog=None, strains=
     reg0 = float(np.float32(setup["delta_regulator"] * np.sqrt(2)))
     n_sky = nets.n_sky
     scan = PlainClusterScan(nIFO, n_pix, nd, dev)
-    scan.regulator(nets.FP, nets.FX, rms_d, n_sky, setup["gamma_regulator"], En, reg0, valid)
+    flags = scan_flags(rms_d, nets.FP, nets.FX)
+    scan.regulator(nets.FP, nets.FX, rms_d, n_sky, setup["gamma_regulator"], En, reg0, valid,
+                   flags=flags)
     maps = torch.empty((len(MAP_NAMES), n_sky), dtype=torch.float32, device=dev)
     scan.scan(nets.FP, nets.FX, nets.ml, rms_d, n_sky, setup["netCC"], En, valid, maps,
-              td_packed=packed)
+              td_packed=packed, sky_order=nets.sky_order(valid_h, dev)[0], flags=flags)
     batch = _single_batch(n_pix, nIFO, K, rms_d, packed)
     st = _GpuSkyStats(batch, nets, [(lookup, xt)])
     st.run(scan.l_max, scan.REG[1:2], reg0, setup["xgb_rho_mode"])
@@ -435,10 +437,11 @@ def _evaluate_batch(config, setup, items, xtalk, nIFO, dev) -> dict:
         scan = PlainClusterScan(nIFO, P, 2 * K + 1, dev)
         rms_c = batch.cluster_rms(c).contiguous()
         td_c = batch.cluster_td(c).contiguous()
-        scan.regulator(nets.FP, nets.FX, rms_c, n_sky, setup["gamma_regulator"], En, reg0, valid)
+        scan.regulator(nets.FP, nets.FX, rms_c, n_sky, setup["gamma_regulator"], En, reg0, valid,
+                       flags=bs.flags)
         maps = torch.empty((len(MAP_NAMES), n_sky), dtype=torch.float32, device=dev)
         scan.scan(nets.FP, nets.FX, nets.ml, rms_c, n_sky, setup["netCC"], En, valid, maps,
-                  td_packed=td_c)
+                  td_packed=td_c, sky_order=bs.sky_order[0], flags=bs.flags)
         s64 = maps[6].double()
         e = torch.exp(s64 - s64.max())
         m = maps.cpu().numpy()

@@ -12,7 +12,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .. import _lib
+from .. import _lib, sky
 from ..device import as_dev, lib, ptr, require_gpu, stream_ptr
 
 MAP_NAMES = (
@@ -51,6 +51,13 @@ def check_scan_inputs(n_ifo, n_pix, n_sky, FP, FX, rms, td00, td90, ml, sky_vali
     return n_delay, valid
 
 
+def scan_flags(*arrays) -> int:
+    """Exact-skip flags, enabled when the weights, patterns and regulators are finite."""
+    ok = all(bool(torch.isfinite(a).all()) if isinstance(a, torch.Tensor)
+             else bool(np.isfinite(np.asarray(a)).all()) for a in arrays)
+    return _lib.DEFAULT_FLAGS if ok else _lib.REGULATOR_EARLY_EXIT
+
+
 class PlainClusterScan:
     """One cluster's regulator + scan on the GPU (one launch family per call)."""
 
@@ -69,21 +76,23 @@ class PlainClusterScan:
                                      ptr(self.ws), self.ws.numel(), stream_ptr(stream))
         _lib.check(rc, "pycwb_td_relayout")
 
-    def regulator(self, FP, FX, rms, n_sky, gamma, En, reg0, valid, stream=None) -> None:
+    def regulator(self, FP, FX, rms, n_sky, gamma, En, reg0, valid, stream=None,
+                  flags=_lib.DEFAULT_FLAGS) -> None:
         rc = lib().pycwb_dpf_regulator(self.n_ifo, self.n_pix, int(n_sky), ptr(FP), ptr(FX),
                                        ptr(rms), float(gamma), float(En), float(reg0), ptr(valid),
-                                       int(valid.numel()), ptr(self.REG), ptr(self.reg_f64),
-                                       ptr(self.ws), self.ws.numel(), stream_ptr(stream))
+                                       int(valid.numel()), int(flags), ptr(self.REG),
+                                       ptr(self.reg_f64), ptr(self.ws), self.ws.numel(),
+                                       stream_ptr(stream))
         _lib.check(rc, "pycwb_dpf_regulator")
 
     def scan(self, FP, FX, ml, rms, n_sky, netCC, En, valid, maps=None, stream=None,
-             td_packed=None) -> None:
+             td_packed=None, sky_order=None, flags=_lib.DEFAULT_FLAGS) -> None:
         rc = lib().pycwb_sky_scan_plain(self.n_ifo, self.n_pix, int(n_sky), self.n_delay,
                                         ptr(FP), ptr(FX), ptr(ml), ptr(rms), ptr(td_packed),
                                         ptr(self.REG), float(netCC), float(En), ptr(valid),
-                                        int(valid.numel()), ptr(maps), ptr(self.l_max),
-                                        ptr(self.stat_max), ptr(self.ws), self.ws.numel(),
-                                        stream_ptr(stream))
+                                        int(valid.numel()), ptr(sky_order), int(flags),
+                                        ptr(maps), ptr(self.l_max), ptr(self.stat_max),
+                                        ptr(self.ws), self.ws.numel(), stream_ptr(stream))
         _lib.check(rc, "pycwb_sky_scan_plain")
 
 
@@ -106,10 +115,15 @@ def scan_sky_for_best_fit(n_ifo, n_pix, n_sky, FP, FX, rms, td00, td90, ml, REG,
     t00, t90 = as_dev(td00, torch.float32, dev), as_dev(td90, torch.float32, dev)
     valid = torch.from_numpy(valid_h).to(dev)
     sc = PlainClusterScan(n_ifo, n_pix, n_delay, dev)
-    sc.REG.copy_(as_dev(np.asarray(REG, dtype=np.float32).reshape(-1)[:3], torch.float32, dev))
+    REGh = np.asarray(REG, dtype=np.float32).reshape(-1)[:3]
+    sc.REG.copy_(as_dev(REGh, torch.float32, dev))
     sc.load_td(t00, t90)
     maps = torch.empty((len(MAP_NAMES), int(n_sky)), dtype=torch.float32, device=dev)
-    sc.scan(FPd, FXd, mld, rmsd, n_sky, netCC, network_energy_threshold, valid, maps)
+    order = torch.from_numpy(sky.delay_order(np.asarray(ml.cpu() if isinstance(ml, torch.Tensor)
+                                                        else ml), valid_h)).to(dev)
+    flags = scan_flags(rmsd, FPd, FXd, REGh)
+    sc.scan(FPd, FXd, mld, rmsd, n_sky, netCC, network_energy_threshold, valid, maps,
+            sky_order=order, flags=flags)
     torch.cuda.current_stream().synchronize()
     m = maps.cpu().numpy()
     return (int(sc.l_max.item()), *[m[i] for i in range(len(MAP_NAMES))],

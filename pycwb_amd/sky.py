@@ -209,3 +209,16 @@ def max_network_delay(ifos, ref_ifo) -> float:
     dets = [Detector(i) for i in ifos]
     ref = next((d for d in dets if d.name == ref_ifo), dets[0]).vertex
     return max(float(np.linalg.norm(d.vertex - ref)) / C_LIGHT for d in dets)
+
+
+def delay_order(ml: np.ndarray, valid: np.ndarray) -> np.ndarray:
+    """Valid-list positions sorted by delay tuple ``ml[:, valid[k]]`` (then by k).
+
+    Directions that share every delay tap read the same delayed amplitudes, so
+    placing them side by side in a wavefront keeps the per-pixel energy test of
+    the sky scan uniform across its lanes.
+    """
+    ml = np.asarray(ml)
+    valid = np.asarray(valid, dtype=np.int64)
+    keys = [np.arange(valid.size)] + [ml[i, valid] for i in reversed(range(ml.shape[0]))]
+    return np.lexsort(keys).astype(np.int32)

@@ -83,3 +83,53 @@ def test_batched_with_sky_mask(oracle):
                                       rms, td00, td90, net.ml, REG, 0.0, 0.5, En, vh)
         assert r["l_max"][c] == l_o and r["sky_stat_max"][c] == s_o
         assert r["l_max"][c] in set(vh.tolist())
+
+
+def test_exact_skip_and_order_are_bit_identical():
+    """The masked-pixel skip, the regulator early exit and the delay-tuple lane
+    order change nothing in the results."""
+    from pycwb_amd import _lib
+    from pycwb_amd.batch import BatchedSkyScan, PlainLoop, net_tables, synth_batch
+
+    nets = _nets(3, 12)
+    sizes = np.array([3, 40, 250, 17, 90], dtype=np.int64)
+    dev = torch.device("cuda", 0)
+    b = synth_batch(nets, sizes, np.arange(5) % 2, seed=123, snr=5.0, device=dev)
+    nt = net_tables(nets, device=dev)
+    valid = torch.arange(nets[0].n_sky, dtype=torch.int64, device=dev)
+    outs = []
+    for flags, ordered in [(0, False), (_lib.DEFAULT_FLAGS, True), (_lib.SKIP_MASKED_PIXELS, False),
+                           (_lib.REGULATOR_EARLY_EXIT, True)]:
+        bs = BatchedSkyScan(b, nt, valid, 0.1667, 0.5, 0.5, block=128, grid=23, flags=flags,
+                            ordered=ordered)
+        bs.launch()
+        outs.append(bs.results())
+        pl = PlainLoop(b, nt, valid, 0.1667, 0.5, 0.5, flags=flags, ordered=ordered)
+        pl.launch()
+        outs.append(pl.results())
+    for r in outs[1:]:
+        for k in ("l_max", "sky_stat_max", "regulator"):
+            assert np.array_equal(r[k], outs[0][k]), k
+
+
+def test_plain_scan_with_non_finite_amplitudes(oracle):
+    """A NaN and an inf delayed amplitude: the energy test keeps those pixels
+    on the full path, and every map still equals the oracle's."""
+    from pycwb_amd.likelihoodWP.sky_scan import scan_sky_for_best_fit
+
+    net = synth.make_network(synth.NETWORK_3, order=2, K=4)
+    cl = synth.make_cluster(net, 20, seed=8, snr=6.0)
+    td00 = cl["td00"].copy()
+    td00[2, 1, 3] = np.nan
+    td00[5, 0, 7] = np.inf
+    REG = np.array([0.5 * np.sqrt(2), 1.0, 0], np.float32)
+    valid = np.arange(net.n_sky)
+    g = scan_sky_for_best_fit(3, 20, net.n_sky, net.FP_t, net.FX_t, cl["rms"], td00, cl["td90"],
+                              net.ml, REG, 0.0, 0.5, 12.0, valid)
+    l_o, maps_o, s_o = oracle.scan_sky(3, 20, net.n_sky, net.FP_t, net.FX_t, cl["rms"], td00,
+                                       cl["td90"], net.ml, REG, 0.0, 0.5, 12.0, valid)
+    mg = np.stack(g[1:12])
+    both_nan = np.isnan(mg) & np.isnan(maps_o)
+    assert np.array_equal(np.where(both_nan, 0, mg).view(np.uint32),
+                          np.where(both_nan, 0, maps_o).view(np.uint32))
+    assert g[0] == l_o and (g[12] == s_o or (np.isnan(g[12]) and np.isnan(s_o)))
