@@ -86,6 +86,36 @@ def cpu_baseline(threads, order=7, P=1024, repeats=2):
                          f"best of {repeats}, {best:.2f} s"
 
 
+def oracle_parity(batch, nets, nt, rb, lm_gpu, gamma, delta, netCC, reg0, n_check=6):
+    """l_max bit-exact and Lm relative error against the oracle for the
+    n_check smallest clusters of each network (whole sky each)."""
+    from oracle import oracle as o
+
+    picks = []
+    for k in range(len(nets)):
+        ids = np.nonzero(batch.net == k)[0]
+        picks += ids[np.argsort(batch.n_pix[ids], kind="stable")][:n_check // len(nets)].tolist()
+    valid = np.arange(nets[0].n_sky, dtype=np.int64)
+    l_ok, lm_err = True, 0.0
+    for c in picks:
+        net = nets[batch.net[c]]
+        td = batch.cluster_td(c).cpu().numpy()
+        td00 = np.ascontiguousarray(td[..., 0].transpose(2, 1, 0))
+        td90 = np.ascontiguousarray(td[..., 1].transpose(2, 1, 0))
+        rms = batch.cluster_rms(c).cpu().numpy()
+        En = nt.En[batch.net[c]]
+        reg = o.calculate_dpf(net.FP_t, net.FX_t, rms, net.n_sky, net.n_ifo, gamma, En, valid)
+        REG = np.array([reg0, reg, 0.0], np.float32)
+        l_o, _, _ = o.scan_sky(net.n_ifo, int(batch.n_pix[c]), net.n_sky, net.FP_t, net.FX_t,
+                               rms, td00, td90, net.ml, REG, netCC, delta, En, valid)
+        st = o.sky_statistics(l_o, net.n_ifo, int(batch.n_pix[c]), net.n_sky, net.FP_t,
+                              net.FX_t, rms, td00, td90, net.ml, REG, En)
+        l_ok &= int(rb["l_max"][c]) == l_o
+        lm_err = max(lm_err, abs(lm_gpu[c] - st["Lm"]) / max(abs(st["Lm"]), 1e-30))
+    return {"oracle_clusters": len(picks), "oracle_l_max_bit_exact": bool(l_ok),
+            "oracle_Lm_max_rel_err": lm_err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +129,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--parity-clusters", type=int, default=6)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -111,6 +142,7 @@ def main():
 
     from pycwb_amd import autotune, synth
     from pycwb_amd.batch import BatchedSkyScan, PlainLoop, net_tables, synth_batch
+    from pycwb_amd.skystats import SkyStatistics
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -138,20 +170,27 @@ def main():
         tuned = autotune.tune_batched(nets, nt, valid, gamma, delta, netCC, log=log)
         block, grid = tuned["block"], tuned["grid"]
     bs = BatchedSkyScan(batch, nt, valid, gamma, delta, netCC, block=block, grid=grid)
+    stats = SkyStatistics(batch, nt)   # self-only cross-talk
+    reg0 = float(np.float32(delta * np.sqrt(2)))
     log(f"batched: block={bs.block} grid={bs.grid} ({tuned.get('source')})")
+
+    def step():
+        bs.launch(stream)                              # regulator + scan + argmax (hipGraph)
+        stats.run(bs.l_max, bs.REG1, reg0, stream=stream)  # Lm, Ec, rho, ... at l_max
 
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
-        bs.launch(stream)
+        step()
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
     for _ in range(args.steps):
-        bs.launch(stream)
+        step()
     ev1.record(stream)
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / args.steps
     rb = bs.results()
+    lm_gpu = stats.scalars()["Lm"]
     log(f"batched step {ms:.1f} ms -> {evals / ms * 1e3:.4e} evals/s")
 
     # dominant kernel: the scan stage, timed alone on the same stream
@@ -196,6 +235,12 @@ def main():
             "stage_vs_graph_l_max": bool(np.array_equal(rb["l_max"], rs["l_max"])),
         }
         log(f"plain loop {pms:.1f} ms -> {evals / pms * 1e3:.4e} evals/s; parity {parity}")
+
+    # parity against the oracle on the smallest clusters (full sky each)
+    if not args.no_cpu:
+        parity.update(oracle_parity(batch, nets, nt, rb, lm_gpu, gamma, delta, netCC, reg0,
+                                    n_check=args.parity_clusters))
+        log(f"oracle parity: {parity}")
 
     cpu = None
     if not args.no_cpu:
