@@ -4,8 +4,9 @@
  *
  * Every pointer argument is a device pointer (HBM) unless stated otherwise;
  * `stream` is a hipStream_t passed as void*.  Calls enqueue work on `stream` and
- * return without synchronising.  No entry point keeps global mutable state, so
- * concurrent calls on different streams with different buffers are safe.
+ * return without synchronising.  The only state kept between calls is the
+ * text of the last error, which is per thread; concurrent calls on different
+ * streams with different buffers are safe.
  * Return value: PYCWB_OK, or PYCWB_ERR_ARG (shapes / pointers rejected before
  * any launch), or PYCWB_ERR_HIP (a HIP runtime call failed).
  *
@@ -212,6 +213,50 @@ typedef struct pycwb_stats_desc {
 size_t pycwb_sky_stats_scratch32(int n_ifo, int n_pix);
 size_t pycwb_sky_stats_scratch64(int n_ifo, int n_pix);
 int pycwb_sky_statistics(const pycwb_stats_desc* desc, void* stream);
+
+/* Continuous-direction sky statistic and its gradients, float64, one
+ * workgroup per cluster.  Extends the sky scan (pycwb/modules/likelihoodWP/
+ * sky_scan.py:131-141, the statistic AA = L * Co maximised by the grid argmax)
+ * to off-grid directions: delays tau_i = (R_i - R_ref).n(theta, phi) * rate / c
+ * are linearly interpolated in the td tables (instead of rint at
+ * pycwb/types/detector.py:1299-1303) and F+/Fx come from the response tensor
+ * (detector.py:272-378) at psi = 0, gha = -phi; REG is fixed and the netCC
+ * gate is not applied.  No counterpart exists in the reference; it serves
+ * sky-location refinement off the HEALPix grid.
+ *
+ * theta = colatitude, phi = Earth-fixed longitude (radians), per cluster.
+ * geo: per network 8 x 12 float64: for each detector the delay vector
+ * (R_i - R_ref) * rate / c (3) then the response tensor row-major (9).
+ * value[c] = AA, grad[2c..2c+1] = (dAA/dtheta, dAA/dphi).
+ * grad_td (optional): float32 pairs in the td layout; on return holds
+ * grad_scale[c] * dAA/d(amplitude) at the two taps each (pixel, detector)
+ * reads and is left untouched elsewhere (callers zero it first).
+ * A cluster whose network does not have n_ifo detectors gets NaN outputs. */
+typedef struct pycwb_grad_desc {
+  int n_clusters;            /* clusters evaluated by this call */
+  const int32_t* clusters;   /* NULL: 0 .. n_clusters-1; else their indices */
+  int n_ifo;                 /* detectors of every evaluated cluster (1..5) */
+  const int32_t* n_pix;
+  const int32_t* net;
+  const int32_t* K;
+  const int64_t* rms_off;
+  const int64_t* td_off;     /* in float pairs */
+  const float* rms;
+  const float* td;
+  const double* geo;
+  const int32_t* net_nifo;
+  const double* net_En;
+  const double* REG1;        /* per cluster */
+  double reg0;               /* delta * sqrt(2) */
+  const double* theta;
+  const double* phi;
+  double* value;
+  double* grad;
+  float* grad_td;            /* NULL: angles only */
+  const double* grad_scale;  /* NULL: 1 */
+} pycwb_grad_desc;
+
+int pycwb_sky_grad(const pycwb_grad_desc* desc, void* stream);
 
 /* WDM forward transform t2w of n_series float64 series (row stride ldx), all
  * with the same filter.  Replaces WDM<T>::t2w (cwb-core/WDM.cc:1385-1534) as

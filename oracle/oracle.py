@@ -215,3 +215,67 @@ def xtalk_sum(p, q, xtalks, lookup, mk):
     lk = _c(lookup, np.int32)
     mk = _c(mk, np.float32)
     return fn(_p(p), _p(q), _p(lk), _p(xt) if xt.size else vp(), _p(mk), n, P)
+
+
+# ---- continuous-direction sky statistic (grad_oracle.c) ----
+
+CS_STEP = 1e-30   # complex step
+
+
+def direction_terms(geo, theta, phi):
+    """(tau, fp, fx) float64 of the n detectors in ``geo`` ((n, 12) float64)."""
+    lib = load()
+    fn = lib.oracle_direction_terms
+    fn.restype = None
+    fn.argtypes = [C.c_int, vp, C.c_double, C.c_double, vp, vp, vp]
+    geo = _c(geo, np.float64).reshape(-1, 12)
+    n = geo.shape[0]
+    tau, fp, fx = np.zeros(n), np.zeros(n), np.zeros(n)
+    fn(n, _p(geo), theta, phi, _p(tau), _p(fp), _p(fx))
+    return tau, fp, fx
+
+
+class SkyObjective:
+    """AA(theta, phi; amplitudes) of one cluster in float64, with complex-step
+    derivatives.  rms (P, n) float32, td (P, n, 2K+1, 2) float32, geo (n, 12)."""
+
+    def __init__(self, rms, td, geo, reg0, reg1, En):
+        self.rms = _c(rms, np.float32)
+        self.td = _c(td, np.float32)
+        self.P, self.n, nd, _ = self.td.shape
+        self.K = (nd - 1) // 2
+        self.geo = _c(geo, np.float64).reshape(self.n, 12)
+        self.reg0, self.reg1, self.En = float(reg0), float(reg1), float(En)
+        lib = load()
+        self._fn = lib.oracle_sky_objective
+        self._fn.restype = None
+        self._fn.argtypes = ([C.c_int] * 3 + [vp] * 3 + [C.c_double] * 7 + [C.c_int64] +
+                             [C.c_double] * 2 + [vp])
+        self._at = lib.oracle_sky_objective_at
+        self._at.restype = C.c_double
+        self._at.argtypes = [C.c_int] * 3 + [vp] * 5 + [C.c_double] * 3
+
+    def _call(self, th, thi, ph, phi_, idx=-1, dre=0.0, dim=0.0):
+        out = np.zeros(2)
+        self._fn(self.n, self.P, self.K, _p(self.rms), _p(self.td), _p(self.geo), self.reg0,
+                 self.reg1, self.En, th, thi, ph, phi_, idx, dre, dim, _p(out))
+        return out
+
+    def value(self, theta, phi, td_index=-1, td_delta=0.0):
+        return float(self._call(theta, 0.0, phi, 0.0, td_index, td_delta)[0])
+
+    def grad(self, theta, phi):
+        """(dAA/dtheta, dAA/dphi) by complex step."""
+        gt = self._call(theta, CS_STEP, phi, 0.0)[1] / CS_STEP
+        gp = self._call(theta, 0.0, phi, CS_STEP)[1] / CS_STEP
+        return np.array([gt, gp])
+
+    def grad_td(self, theta, phi, td_index):
+        """dAA/d(td flat float entry) by complex step."""
+        return float(self._call(theta, 0.0, phi, 0.0, td_index, 0.0, CS_STEP)[1] / CS_STEP)
+
+    def at(self, tau, fp, fx):
+        """AA at given delays (taps) and patterns."""
+        tau, fp, fx = (_c(a, np.float64) for a in (tau, fp, fx))
+        return float(self._at(self.n, self.P, self.K, _p(self.rms), _p(self.td), _p(tau),
+                              _p(fp), _p(fx), self.reg0, self.reg1, self.En))

@@ -84,6 +84,14 @@ class StatsDesc(C.Structure):
                 ("s64_off", vp), ("scal", vp), ("S_snr", vp)]
 
 
+class GradDesc(C.Structure):
+    _fields_ = [("n_clusters", C.c_int), ("clusters", vp), ("n_ifo", C.c_int), ("n_pix", vp),
+                ("net", vp), ("K", vp), ("rms_off", vp), ("td_off", vp), ("rms", vp),
+                ("td", vp), ("geo", vp), ("net_nifo", vp), ("net_En", vp), ("REG1", vp),
+                ("reg0", C.c_double), ("theta", vp), ("phi", vp), ("value", vp), ("grad", vp),
+                ("grad_td", vp), ("grad_scale", vp)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -102,6 +110,7 @@ SIGNATURES = {
     "pycwb_sky_stats_scratch64": (sz, [i32, i32]),
     "pycwb_sky_statistics": (i32, [C.POINTER(StatsDesc), vp]),
     "pycwb_wdm_t2w": (i32, [vp, i64, i64, i32, i32, i32, vp, i32, vp, i32, vp, vp]),
+    "pycwb_sky_grad": (i32, [C.POINTER(GradDesc), vp]),
 }
 
 

@@ -1,0 +1,152 @@
+"""Sky-statistic gradient kernel against the float64 oracle, finite differences,
+the autograd wrapper, and off-grid refinement."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from pycwb_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(order, K, sizes, net_ids, seed, snr=8.0, ifos=(synth.NETWORK_3, synth.NETWORK_4)):
+    from pycwb_amd.batch import BatchedSkyScan, net_tables, synth_batch
+    from pycwb_amd.skygrad import SkyGradient
+
+    nets = [synth.make_network(f, order=order, K=K) for f in ifos]
+    dev = torch.device("cuda", 0)
+    b = synth_batch(nets, np.asarray(sizes, np.int64), np.asarray(net_ids), seed=seed, snr=snr,
+                    device=dev)
+    nt = net_tables(nets, device=dev)
+    valid = torch.arange(nets[0].n_sky, dtype=torch.int64, device=dev)
+    bs = BatchedSkyScan(b, nt, valid, 0.5 * 0.5 * 2 / 3, 0.5, 0.5, block=64, grid=64)
+    bs.launch()
+    r = bs.results()
+    eng = SkyGradient(b, nt, nets, r["REG1"], bs.desc.reg0)
+    th0 = np.array([math.pi / 2 - nets[k].lat[l] for k, l in zip(net_ids, r["l_max"])])
+    ph0 = np.array([nets[k].phi[l] for k, l in zip(net_ids, r["l_max"])])
+    return nets, nt, b, r, eng, th0, ph0
+
+
+def _oracle_obj(oracle, nets, nt, b, r, c, net_id, reg0):
+    from pycwb_amd.skygrad import network_geometry
+
+    net = nets[net_id]
+    td = b.cluster_td(c).cpu().numpy()
+    rms = b.cluster_rms(c).cpu().numpy()
+    geo = network_geometry(net.ifos, net.td_rate)[:net.n_ifo]
+    return oracle.SkyObjective(rms, td, geo, reg0, float(r["REG1"][c]), nt.En[net_id])
+
+
+def test_value_and_angle_gradient_match_oracle(oracle):
+    sizes = [40, 7, 130, 64, 300, 23, 90, 1]
+    ids = [0, 1, 0, 1, 0, 0, 1, 0]
+    print("case grad value/angles vs oracle", flush=True)
+    nets, nt, b, r, eng, th0, ph0 = _setup(4, 16, sizes, ids, seed=31)
+    rng = np.random.default_rng(5)
+    th = th0 + rng.normal(scale=0.01, size=th0.size)
+    ph = ph0 + rng.normal(scale=0.01, size=ph0.size)
+    v, g, _ = eng.run(torch.from_numpy(th), torch.from_numpy(ph))
+    v, g = v.cpu().numpy(), g.cpu().numpy()
+    for c in range(len(sizes)):
+        obj = _oracle_obj(oracle, nets, nt, b, r, c, ids[c], eng.reg0)
+        vo = obj.value(th[c], ph[c])
+        go = obj.grad(th[c], ph[c])
+        assert v[c] == pytest.approx(vo, rel=1e-10, abs=1e-10), c
+        scale = np.abs(go).max() + 1e-6
+        assert np.abs(g[c] - go).max() <= 1e-8 * scale, (c, g[c], go)
+
+
+def test_angle_gradient_central_differences():
+    """Central differences of the kernel's own value, step 1e-4 rad."""
+    print("case grad FD 1e-4", flush=True)
+    sizes = [50, 120, 80, 200, 33, 150, 64, 97, 41, 180, 75, 60]
+    ids = [0] * len(sizes)
+    nets, nt, b, r, eng, th0, ph0 = _setup(5, 24, sizes, ids, seed=8)
+    rng = np.random.default_rng(2)
+    th = torch.from_numpy(th0 + rng.normal(scale=0.005, size=th0.size))
+    ph = torch.from_numpy(ph0 + rng.normal(scale=0.005, size=ph0.size))
+    h = 1e-4
+    _, g, _ = eng.run(th, ph)
+    fd_t = (eng.run(th + h, ph)[0] - eng.run(th - h, ph)[0]) / (2 * h)
+    fd_p = (eng.run(th, ph + h)[0] - eng.run(th, ph - h)[0]) / (2 * h)
+    fd = torch.stack([fd_t, fd_p], dim=1).cpu().numpy()
+    g = g.cpu().numpy()
+    err = np.abs(g - fd).max(axis=1) / (np.abs(g).max(axis=1) + 1e-6)
+    # O(h^2) truncation; a step that crosses a tap boundary or flips a pixel
+    # mask sees a kink, so a minority of clusters may differ more
+    assert np.mean(err < 1e-2) >= 0.75, err
+    assert np.median(err) < 1e-3, err
+
+
+def test_amplitude_gradient_matches_oracle(oracle):
+    print("case grad amplitudes vs oracle", flush=True)
+    sizes = [60, 25, 140]
+    ids = [0, 1, 0]
+    nets, nt, b, r, eng, th0, ph0 = _setup(4, 12, sizes, ids, seed=44, snr=10.0)
+    th = torch.from_numpy(th0 + 0.004)
+    ph = torch.from_numpy(ph0 - 0.003)
+    scale_c = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64)
+    _, _, gtd = eng.run(th, ph, want_td=True, grad_scale=scale_c)
+    gtd = gtd.cpu().numpy()
+    rng = np.random.default_rng(1)
+    for c in range(len(sizes)):
+        obj = _oracle_obj(oracle, nets, nt, b, r, c, ids[c], eng.reg0)
+        P, n, nd = int(sizes[c]), nets[ids[c]].n_ifo, 2 * 12 + 1
+        base = 2 * int(b.td_off[c])
+        gc = gtd[base:base + P * n * nd * 2]
+        tol = 1e-5 * (np.abs(gc).max() + 1e-12)
+        nz = np.nonzero(gc)[0]
+        # at most two taps per (pixel, detector, phase) are touched
+        assert nz.size <= P * n * 2 * 2
+        pick = np.concatenate([rng.choice(nz, size=min(15, nz.size), replace=False),
+                               rng.integers(gc.size, size=5)]) if nz.size else \
+            rng.integers(gc.size, size=5)
+        for idx in pick:
+            go = float(scale_c[c]) * obj.grad_td(float(th[c]), float(ph[c]), int(idx))
+            assert abs(gc[idx] - go) <= tol, (c, idx, gc[idx], go)
+
+
+def test_autograd_function():
+    from pycwb_amd.skygrad import sky_statistic
+
+    print("case grad autograd", flush=True)
+    sizes = [70, 30, 110, 45]
+    ids = [0, 1, 1, 0]
+    nets, nt, b, r, eng, th0, ph0 = _setup(4, 12, sizes, ids, seed=3)
+    dev = b.td.device
+    th = torch.tensor(th0 + 0.002, device=dev, requires_grad=True)
+    ph = torch.tensor(ph0 - 0.001, device=dev, requires_grad=True)
+    td = b.td.clone().requires_grad_(True)
+    w = torch.tensor([1.0, 0.5, -1.5, 2.0], dtype=torch.float64, device=dev)
+    val = sky_statistic(th, ph, td, eng)
+    (w * val).sum().backward()
+    v_ref, g_ref, gtd_ref = eng.run(th.detach(), ph.detach(), want_td=True, grad_scale=w)
+    assert torch.equal(val.detach(), v_ref)
+    assert torch.equal(th.grad, w * g_ref[:, 0])
+    assert torch.equal(ph.grad, w * g_ref[:, 1])
+    assert torch.equal(td.grad, gtd_ref)
+
+
+def test_refinement_improves_clusters():
+    from pycwb_amd.skygrad import refine_sky
+
+    print("case grad refinement order 6", flush=True)
+    sizes = synth.cluster_sizes(48, seed=12, lo=16, hi=512)
+    ids = np.zeros(sizes.size, dtype=np.int64)
+    nets, nt, b, r, eng, th0, ph0 = _setup(6, 32, sizes, ids, seed=12, snr=6.0,
+                                           ifos=(synth.NETWORK_3,))
+    out = refine_sky(eng, th0, ph0, steps=20)
+    v0, v = out["value0"].cpu().numpy(), out["value"].cpu().numpy()
+    assert np.all(v >= v0)
+    assert np.mean(v > v0) >= 0.95, (v0, v)
+    # moves stay inside the cap around the start
+    from pycwb_amd.skygrad import _unit
+
+    c0 = _unit(torch.from_numpy(th0), torch.from_numpy(ph0))
+    c1 = _unit(out["theta"].cpu(), out["phi"].cpu())
+    ang = torch.acos((c0 * c1).sum(-1).clamp(-1, 1)).numpy()
+    assert np.all(ang <= 3 * math.sqrt(4 * math.pi / nets[0].n_sky) + 1e-9)
