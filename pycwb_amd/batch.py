@@ -87,6 +87,7 @@ class ClusterBatch:
     rms: torch.Tensor          # float32 flat
     td: torch.Tensor           # float32 flat (pairs)
     l0: np.ndarray = None      # injection directions (synthetic batches)
+    truth: np.ndarray = None   # (C, 2) off-grid injection (theta, phi), smooth batches
 
     @property
     def n_clusters(self):
@@ -172,6 +173,93 @@ def synth_batch(nets, sizes, net_ids, seed: int, snr: float = 3.0, device=None,
                         K=K.astype(np.int32), nifo=nifo.astype(np.int32),
                         rms_off=rms_off.astype(np.int64), td_off=td_off.astype(np.int64),
                         rms=rms, td=td, l0=l0)
+
+
+def synth_batch_smooth(nets, sizes, net_ids, seed: int, snr: float = 6.0, tap_sigma: float = 2.0,
+                       device=None, signal_fraction: float = 200 / 1024) -> ClusterBatch:
+    """Synthetic batch whose delayed amplitudes vary smoothly with the delay.
+
+    A td table samples one pixel's amplitude at successive time shifts, so a
+    real one is correlated along the tap axis over the pixel's duration.  Here
+    the noise is white Gaussian smoothed along the taps by a Gaussian of
+    sigma tap_sigma / sqrt(2) (unit variance per tap, correlation
+    exp(-d^2 / (2 tap_sigma^2)) between taps d apart), and each cluster's
+    signal comes from a direction drawn uniformly on the sphere (off the
+    grid): detector i carries r_i (F+_i h+ + Fx_i hx) * exp(-(k - u_i)^2 /
+    (2 tap_sigma^2)) at tap k, u_i = K + continuous delay.  All clusters must
+    share K.  ``truth`` holds the injected (theta, phi).
+    """
+    from .skygrad import network_geometry
+
+    device = device or require_gpu()
+    sizes = np.asarray(sizes, dtype=np.int64)
+    net_ids = np.asarray(net_ids, dtype=np.int64)
+    Ks = {nets[k].K for k in set(net_ids.tolist())}
+    if len(Ks) != 1:
+        raise ValueError("synth_batch_smooth needs one K for every cluster")
+    K = Ks.pop()
+    nd = 2 * K + 1
+    b = synth_batch(nets, sizes, net_ids, seed, snr=0.0, device=device, signal_fraction=0.0)
+    rows = b.td.numel() // (2 * nd)
+    hw = int(math.ceil(4 * tap_sigma))
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 1)
+    xs = np.arange(-hw, hw + 1, dtype=np.float64)
+    ker = np.exp(-xs ** 2 / (tap_sigma ** 2))        # sigma_n = tap_sigma / sqrt(2)
+    ker = ker / math.sqrt(float(np.sum(ker ** 2)))
+    td = b.td.view(rows, nd, 2)
+    chunk = 1 << 18
+    for r0 in range(0, rows, chunk):
+        r1 = min(rows, r0 + chunk)
+        w = torch.randn(r1 - r0, nd + 2 * hw, 2, generator=g, device=device)
+        sm = torch.zeros(r1 - r0, nd, 2, device=device)
+        for k in range(ker.size):  # Gaussian smoothing along the taps
+            sm.add_(w[:, k:k + nd], alpha=float(ker[k]))
+        td[r0:r1] = sm
+    rms_h = b.rms.cpu().numpy()
+    rng = np.random.default_rng(seed)
+    truth = np.zeros((sizes.size, 2))
+    geos = {}
+    taps = np.arange(nd, dtype=np.float64)
+    idx_all, val_all = [], []
+    for c in range(sizes.size):
+        net = nets[net_ids[c]]
+        n, P = net.n_ifo, int(sizes[c])
+        if net_ids[c] not in geos:
+            geos[net_ids[c]] = network_geometry(net.ifos, net.td_rate)[:n]
+        geo = geos[net_ids[c]]
+        th, ph = math.acos(rng.uniform(-1.0, 1.0)), rng.uniform(0.0, 2 * math.pi)
+        truth[c] = th, ph
+        nvec = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+        u = np.clip(geo[:, :3] @ nvec + K, 0.0, nd - 1.0)
+        fp, fx = np.zeros(n), np.zeros(n)
+        for i in range(n):
+            x = np.array([math.sin(ph), -math.cos(ph), 0.0])
+            y = np.array([-math.cos(ph) * math.cos(th), -math.sin(ph) * math.cos(th),
+                          math.sin(th)])
+            D = geo[i, 3:].reshape(3, 3)
+            fp[i] = x @ D @ x - y @ D @ y
+            fx[i] = x @ D @ y + y @ D @ x
+        ns = max(1, round(P * signal_fraction))
+        sig = rng.choice(P, size=min(ns, P), replace=False)
+        h = rng.standard_normal((sig.size, 4))
+        r = rms_h[b.rms_off[c]: b.rms_off[c] + P * n].reshape(P, n)[sig].astype(np.float64)
+        v00 = r * (fp[None, :] * h[:, :1] + fx[None, :] * h[:, 1:2])
+        v90 = r * (fp[None, :] * h[:, 2:3] + fx[None, :] * h[:, 3:4])
+        e = np.sqrt(np.sum(v00 ** 2 + v90 ** 2, axis=1, keepdims=True))
+        scale = np.where(e > 0, snr / np.where(e > 0, e, 1), 0)
+        v00, v90 = v00 * scale, v90 * scale
+        prof = np.exp(-(taps[None, :] - u[:, None]) ** 2 / (2 * tap_sigma ** 2))  # (n, nd)
+        base = b.td_off[c] + (sig[:, None] * n + np.arange(n)[None, :]) * nd      # (s, n)
+        idx = base[:, :, None] + np.arange(nd)[None, None, :]
+        idx_all += [(2 * idx).ravel(), (2 * idx + 1).ravel()]
+        val_all += [(v00[:, :, None] * prof[None]).ravel(), (v90[:, :, None] * prof[None]).ravel()]
+    if idx_all:
+        b.td.index_add_(0, torch.from_numpy(np.concatenate(idx_all)).to(device),
+                        torch.from_numpy(np.concatenate(val_all).astype(np.float32)).to(device))
+    b.l0 = None
+    b.truth = truth
+    return b
 
 
 def lpt_order(batch: ClusterBatch):

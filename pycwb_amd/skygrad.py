@@ -161,6 +161,13 @@ def sky_statistic(theta, phi, td, engine: SkyGradient):
     return SkyStatistic.apply(theta, phi, td, engine)
 
 
+def angular_distance(theta1, phi1, theta2, phi2):
+    """Great-circle angle (radians) between directions, numpy float64."""
+    c = (np.sin(theta1) * np.sin(theta2) * np.cos(np.asarray(phi1) - phi2) +
+         np.cos(theta1) * np.cos(theta2))
+    return np.arccos(np.clip(c, -1.0, 1.0))
+
+
 def _unit(theta, phi):
     st = torch.sin(theta)
     return torch.stack([st * torch.cos(phi), st * torch.sin(phi), torch.cos(theta)], dim=-1)
