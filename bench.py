@@ -248,6 +248,10 @@ def main():
     ap.add_argument("--refine-steps", type=int, default=20)
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-wdm", action="store_true")
+    ap.add_argument("--only", choices=("refine", "wdm"), default=None,
+                    help="run one side leg alone and print its JSON (for profiling)")
+    ap.add_argument("--copy-calib-gib", type=float, default=0.0,
+                    help="first copy this many GiB device-to-device (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -264,6 +268,22 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calib_gib > 0:
+        n = int(args.copy_calib_gib * (1 << 30)) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n * 4} bytes read and written")
+        del src, dst
+    if args.only == "refine":
+        print(json.dumps({"refinement": refinement_leg(
+            args.refine_clusters, args.order, args.K, args.refine_steps,
+            torch.cuda.current_stream(), log)}), flush=True)
+        return 0
+    if args.only == "wdm":
+        print(json.dumps({"wdm": wdm_leg(torch.cuda.current_stream(), log)}), flush=True)
+        return 0
     t0 = time.perf_counter()
     nets = [synth.make_network(synth.NETWORK_3, order=args.order, K=args.K),
             synth.make_network(synth.NETWORK_4, order=args.order, K=args.K)]

@@ -36,16 +36,21 @@ extern "C" {
 #define PYCWB_ERR_ARG 1
 #define PYCWB_ERR_HIP 2
 
-/* flags of the regulator / scan entry points.  Both are exact: results are
- * bit-identical with and without them, given finite inputs (checked by the
- * caller).
+/* flags of the regulator / scan entry points.  All are exact: results are
+ * bit-identical with and without them, given finite noise weights and sky
+ * tables (checked by the caller).
  *   PYCWB_SKIP_MASKED_PIXELS: a pixel whose energy at a direction's delays is
  *     below the network threshold adds nothing to any sum, so its evaluation
  *     stops after the energy test;
  *   PYCWB_REGULATOR_EARLY_EXIT: a direction's DPF sweep stops once its partial
- *     quality already exceeds gamma with every remaining term >= 0. */
+ *     quality already exceeds gamma with every remaining term >= 0;
+ *   PYCWB_TILE_ENERGY_BOUND (with PYCWB_SKIP_MASKED_PIXELS): each wavefront of
+ *     64 directions bounds every pixel's energy over the taps its directions
+ *     read and skips pixels whose bound is below the threshold for all of
+ *     them, without loading their amplitudes per direction. */
 #define PYCWB_SKIP_MASKED_PIXELS 1
 #define PYCWB_REGULATOR_EARLY_EXIT 2
+#define PYCWB_TILE_ENERGY_BOUND 4
 
 const char* pycwb_version(void);
 

@@ -19,7 +19,8 @@ PYCWB_ERR_ARG = 1
 PYCWB_ERR_HIP = 2
 SKIP_MASKED_PIXELS = 1
 REGULATOR_EARLY_EXIT = 2
-DEFAULT_FLAGS = SKIP_MASKED_PIXELS | REGULATOR_EARLY_EXIT
+TILE_ENERGY_BOUND = 4
+DEFAULT_FLAGS = SKIP_MASKED_PIXELS | REGULATOR_EARLY_EXIT | TILE_ENERGY_BOUND
 
 _lock = threading.Lock()
 _lib = None

@@ -38,12 +38,14 @@ class NetTablesDev:
     En_d: torch.Tensor         # float64 (n_net,)
     K: list = field(default_factory=list)
     ml_host: list = field(default_factory=list)   # per network (n, S) int32
+    side_host: list = field(default_factory=list)  # per network (S,) bool or None
 
     def sky_order(self, valid_h: np.ndarray, device) -> torch.Tensor:
         """(n_net, n_valid) valid-list positions in delay-tuple order per network."""
         key = ("order", valid_h.tobytes())
         if getattr(self, "_order_key", None) != key:
-            o = np.stack([sky.delay_order(m, valid_h) for m in self.ml_host])
+            sides = self.side_host or [None] * len(self.ml_host)
+            o = np.stack([sky.delay_order(m, valid_h, sd) for m, sd in zip(self.ml_host, sides)])
             self._order = torch.from_numpy(np.ascontiguousarray(o)).to(device)
             self._order_key = key
         return self._order
@@ -73,7 +75,8 @@ def net_tables(nets, acore: float = math.sqrt(2.0), device=None) -> NetTablesDev
         FP=t(fp, torch.float32), FX=t(fx, torch.float32), ml=t(ml, torch.int32),
         nifo_d=torch.tensor([n.n_ifo for n in nets], dtype=torch.int32, device=device),
         En_d=torch.tensor(En, dtype=torch.float64, device=device), K=[n.K for n in nets],
-        ml_host=[np.ascontiguousarray(n.ml, np.int32) for n in nets])
+        ml_host=[np.ascontiguousarray(n.ml, np.int32) for n in nets],
+        side_host=[sky.mirror_side(n.ifos, n.phi, n.lat) for n in nets])
 
 
 @dataclass

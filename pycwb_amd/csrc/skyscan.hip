@@ -132,16 +132,205 @@ __device__ __forceinline__ unsigned regulator_lane(const float* __restrict__ FP,
   return q > gamma ? 1u : 0u;
 }
 
-// Sky scan of direction s; returns the argmax key for position k.
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Widest summed tap range (over detectors) for which the tile energy bound is
+// evaluated; wider ranges scan every pixel.
+constexpr int kMaxBoundSpan = 32;
+
+// Tile energy bound of pixel j: U = sum_i max_{t in [lo_i, hi_i]} (v00^2 + v90^2)
+// at taps t of detector i.  Every direction of the wavefront reads taps inside
+// these ranges, so its pixel energy e = (sum v00^2) + (sum v90^2) + 1e-12 is at
+// most U up to float32 rounding (relative 1e-6 at n <= 8, all terms >= 0) plus
+// the 1e-12 addend.  Returns false when U * (1 + 1e-5) + 1e-11 <= En: then no
+// direction of the wavefront has e > En, and with finite amplitudes the pixel
+// contributes nothing (pixel_contributes).  A NaN or inf amplitude in range
+// makes U non-finite and the pixel a candidate.
 template <int N>
+__device__ __forceinline__ bool pixel_candidate(const float2* __restrict__ row, int nd,
+                                                const int (&lo)[N], const int (&hi)[N],
+                                                double En) {
+  float U = 0.f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    float mx = 0.f;
+    for (int t = lo[i]; t <= hi[i]; ++t) {
+      const float2 v = row[i * nd + t];
+      const float E = v.x * v.x + v.y * v.y;
+      mx = (E > mx || E != E) ? E : mx;  // NaN sticks
+    }
+    U += mx;
+  }
+  return !((double)U * (1.0 + 1e-5) + 1e-11 <= En);
+}
+
+// Wavefront-level ordering point for LDS data shared between lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Compaction of the full-path evaluations of one wavefront.  Lane d owns
+// direction d.  For each pixel, the lanes whose energy test passes enqueue a
+// (pixel, owner) pair with the delayed amplitudes they loaded; once 64 pairs
+// wait, every lane evaluates one of them (pixel_rest) and stores its
+// contributions, and each owner adds its own contributions to its sums in
+// queue order, which is its pixel order.  Pairs beyond the first 64 move to
+// the front and wait for the next round.  The sums therefore see exactly the
+// terms of the per-lane sweep in the same order (bit-identical results) while
+// the expensive chain runs with every lane busy.
+constexpr int kQueue = 128;  // pair slots: at most 63 waiting + 64 of one pixel
+
+template <int N>
+struct WaveQueue {
+  float qv[kQueue][2 * N];        // v00 (0..N) and v90 (N..2N) of each pair
+  int qjd[kQueue];                // (pixel << 6) | owner lane
+  unsigned long long eb[kQueue];  // owners of each queued pixel event
+  int ebase[kQueue];              // first pair slot of each event
+  float pat[64][2 * N];           // F+ (0..N) and Fx (N..2N) of each lane's direction
+  float c_et[64], c_ec[64], c_gn[64], c_rn[64], c_fpet[64], c_fxet[64];
+  double c_llcc[64], c_ll[64];
+  int c_flags[64];                // bit 0: m, bit 1: ec > 1e-9, bit 2: mk > 0
+};
+
+// Evaluate slots [0, cnt), add them to their owners' sums, drop them from the queue.
+template <int N>
+__device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, int& E, int lane,
+                                            const float* __restrict__ rms, int P, float rr,
+                                            float RR, double En, SkySums& acc) {
+  if (lane < cnt) {
+    const int jd = q.qjd[lane];
+    int j = jd >> 6;
+    j = j < 0 ? 0 : (j >= P ? P - 1 : j);
+    const int d = jd & 63;
+    float v00[N], v90[N], r[N], fp0[N], fx0[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      v00[i] = q.qv[lane][i];
+      v90[i] = q.qv[lane][N + i];
+      fp0[i] = q.pat[d][i];
+      fx0[i] = q.pat[d][N + i];
+      r[i] = rms[(int64_t)j * N + i];
+    }
+    float e, et;
+    int m;
+    pixel_energy<N>(v00, v90, En, e, m, et);
+    PixelEval<N> o;
+    pixel_rest<N>(v00, v90, r, fp0, fx0, rr, RR, e, m, et, o);
+    q.c_et[lane] = o.et;
+    q.c_ec[lane] = o.ec;
+    q.c_gn[lane] = o.gn;
+    q.c_rn[lane] = o.rn;
+    q.c_fpet[lane] = o.d.fp * o.et;
+    q.c_fxet[lane] = o.d.fx * o.et;
+    q.c_llcc[lane] = o.ll * o.ccd;
+    q.c_ll[lane] = o.ll;
+    q.c_flags[lane] = o.m | (o.ec > 1e-9f ? 2 : 0) | (o.mk > 0.f ? 4 : 0);
+  }
+  wave_sync();
+  const unsigned long long below = lanemask_lt(lane);
+  int kdrop = 0;
+  for (int e = 0; e < E; ++e) {
+    const int base = q.ebase[e];
+    if (base >= cnt) break;
+    const unsigned long long bm = q.eb[e];
+    if (base + __popcll(bm) <= cnt) kdrop = e + 1;
+    if ((bm >> lane) & 1ull) {
+      const int slot = base + __popcll(bm & below);
+      if (slot < cnt) {
+        const int f = q.c_flags[slot];
+        const float et = q.c_et[slot];
+        acc.M += f & 1;
+        acc.EE += et;
+        acc.Lr += q.c_llcc[slot];
+        acc.LL += q.c_ll[slot];
+        acc.EC += q.c_ec[slot];
+        acc.GN += q.c_gn[slot];
+        acc.RN += q.c_rn[slot];
+        acc.Mp += (f & 2) ? 1.0 : 0.0;
+        if (f & 4) {
+          acc.sde += et;
+          acc.pwe += q.c_fpet[slot];
+          acc.cwe += q.c_fxet[slot];
+        }
+      }
+    }
+  }
+  // move the waiting pairs and events to the front
+  const bool mv = lane + cnt < T;
+  float v[2 * N];
+  int jd = 0;
+  if (mv) {
+#pragma unroll
+    for (int i = 0; i < 2 * N; ++i) v[i] = q.qv[lane + cnt][i];
+    jd = q.qjd[lane + cnt];
+  }
+  const int ne = E - kdrop;
+  unsigned long long eb0 = 0ull, eb1 = 0ull;
+  int bs0 = 0, bs1 = 0;
+  if (lane < ne) {
+    eb0 = q.eb[kdrop + lane];
+    bs0 = q.ebase[kdrop + lane];
+  }
+  if (lane + 64 < ne) {
+    eb1 = q.eb[kdrop + lane + 64];
+    bs1 = q.ebase[kdrop + lane + 64];
+  }
+  wave_sync();
+  if (mv) {
+#pragma unroll
+    for (int i = 0; i < 2 * N; ++i) q.qv[lane][i] = v[i];
+    q.qjd[lane] = jd;
+  }
+  if (lane < ne) {
+    bs0 -= cnt;
+    for (; bs0 < 0; ++bs0) eb0 &= eb0 - 1ull;  // the partly evaluated event
+    q.eb[lane] = eb0;
+    q.ebase[lane] = bs0;
+  }
+  if (lane + 64 < ne) {
+    q.eb[lane + 64] = eb1;
+    q.ebase[lane + 64] = bs1 - cnt;
+  }
+  E = ne;
+  T -= cnt;
+  wave_sync();
+}
+
+// Sky scan of direction s; returns the argmax key for position k.
+// Pixels are taken in chunks of 64.  With tile_bound (which needs skip_masked)
+// the wavefront first marks the chunk's candidate pixels with the tile energy
+// bound and every lane then tests only those, in increasing order.  With
+// skip_masked the pairs passing the energy test go through the wavefront's
+// queue (queue_flush); otherwise every lane evaluates every pixel itself.
+template <int N, int WAVES>
 __device__ __forceinline__ unsigned long long scan_lane(
     const float* __restrict__ FP, const float* __restrict__ FX, const int32_t* __restrict__ ml,
     int64_t S, const float* __restrict__ rms, const float2* __restrict__ td, int P, int K,
     float rr, float RR, double En, double netCC, int64_t s, int64_t k, bool active,
-    bool skip_masked, float* __restrict__ maps) {
+    bool skip_masked, bool tile_bound, float* __restrict__ maps) {
+  __shared__ WaveQueue<N> queues[WAVES];
   const int nd = 2 * K + 1;
+  const int lane = (int)(threadIdx.x & 63);
+  WaveQueue<N>& q = queues[threadIdx.x >> 6];
   float fp0[N], fx0[N];
-  int tap[N];
+  int tap[N], lo[N], hi[N];
+  int span = 0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     fp0[i] = FP[s * N + i];
@@ -149,29 +338,89 @@ __device__ __forceinline__ unsigned long long scan_lane(
     int t = ml[(int64_t)i * S + s] + K;
     t = t < 0 ? 0 : (t >= nd ? nd - 1 : t);  // host validates |ml| <= K
     tap[i] = i * nd + t;
+    lo[i] = wave_min_int(t);
+    hi[i] = wave_max_int(t);
+    span += hi[i] - lo[i] + 1;
+    q.pat[lane][i] = fp0[i];
+    q.pat[lane][N + i] = fx0[i];
   }
+  const bool use_bound = skip_masked && tile_bound && span <= kMaxBoundSpan;  // wave-uniform
+  int T = 0, E = 0;  // queued pairs and pixel events (wave-uniform)
+  wave_sync();
   SkySums acc;
   sums_init(acc);
   const int64_t pstride = (int64_t)N * nd;
-  // software pipeline: the next pixel's amplitudes load while this one computes
-  float2 cur[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i) cur[i] = td[tap[i]];
-  for (int j = 0; j < P; ++j) {
-    float2 nxt[N];
-    const int jn = j + 1 < P ? j + 1 : j;
-#pragma unroll
-    for (int i = 0; i < N; ++i) nxt[i] = td[(int64_t)jn * pstride + tap[i]];
-    float v00[N], v90[N], r[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      v00[i] = cur[i].x;
-      v90[i] = cur[i].y;
-      r[i] = rms[(int64_t)j * N + i];
+  for (int j0 = 0; j0 < P; j0 += 64) {
+    const int left = P - j0;
+    unsigned long long cand = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
+    if (use_bound) {
+      const int j = j0 + lane;
+      const bool c = j < P && pixel_candidate<N>(td + (int64_t)j * pstride, nd, lo, hi, En);
+      cand = __ballot(c);
     }
-    scan_pixel<N>(v00, v90, r, fp0, fx0, rr, RR, En, skip_masked, acc);
+    if (!cand) continue;
+    // software pipeline: the next candidate's amplitudes load while this one computes
+    int b = __ffsll((long long)cand) - 1;
+    cand &= cand - 1ull;
+    float2 cur[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) cur[i] = nxt[i];
+    for (int i = 0; i < N; ++i) cur[i] = td[(int64_t)(j0 + b) * pstride + tap[i]];
+    for (;;) {
+      const int j = j0 + b;
+      const bool more = cand != 0ull;
+      const int bn = more ? __ffsll((long long)cand) - 1 : b;
+      cand &= cand - 1ull;
+      float2 nxt[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) nxt[i] = td[(int64_t)(j0 + bn) * pstride + tap[i]];
+      float v00[N], v90[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        v00[i] = cur[i].x;
+        v90[i] = cur[i].y;
+      }
+      if (skip_masked) {
+        float e, et;
+        int m;
+        pixel_energy<N>(v00, v90, En, e, m, et);
+        const bool contrib = pixel_contributes(e, m);
+        const unsigned long long bm = __ballot(contrib);
+        if (bm) {
+          if (contrib) {
+            const int slot = T + __popcll(bm & lanemask_lt(lane));
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              q.qv[slot][i] = v00[i];
+              q.qv[slot][N + i] = v90[i];
+            }
+            q.qjd[slot] = (j << 6) | lane;
+          }
+          if (lane == 0) {
+            q.eb[E] = bm;
+            q.ebase[E] = T;
+          }
+          E += 1;
+          T += __popcll(bm);
+          if (T >= 64) {
+            wave_sync();
+            queue_flush<N>(q, 64, T, E, lane, rms, P, rr, RR, En, acc);
+          }
+        }
+      } else {
+        float r[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = rms[(int64_t)j * N + i];
+        scan_pixel<N>(v00, v90, r, fp0, fx0, rr, RR, En, false, acc);
+      }
+      if (!more) break;
+      b = bn;
+#pragma unroll
+      for (int i = 0; i < N; ++i) cur[i] = nxt[i];
+    }
+  }
+  if (T > 0) {
+    wave_sync();
+    queue_flush<N>(q, T, T, E, lane, rms, P, rr, RR, En, acc);
   }
   SkyStat st;
   sky_epilogue(acc, N, netCC, st);
@@ -254,8 +503,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_plain(
   k = k < 0 ? 0 : (k >= n_valid ? n_valid - 1 : k);
   const int64_t s = valid[k];
   const unsigned long long key =
-      scan_lane<N>(FP, FX, ml, S, rms, td, P, K, REG[0], REG[1], En, netCC, s, k, active,
-                   (flags & PYCWB_SKIP_MASKED_PIXELS) != 0, maps);
+      scan_lane<N, BLOCK / 64>(FP, FX, ml, S, rms, td, P, K, REG[0], REG[1], En, netCC, s, k,
+                               active,
+                   (flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
+                   (flags & PYCWB_TILE_ENERGY_BOUND) != 0, maps);
   block_argmax<BLOCK>(key, best);
 }
 
@@ -389,8 +640,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev b, int firs
       const int32_t* ml = b.ml + b.net_off[net];
       const float2* td = b.td + b.td_off[c];
       const unsigned long long key =
-          scan_lane<N>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR, En, b.netCC, s, k, active,
-                       (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0, nullptr);
+          scan_lane<N, BLOCK / 64>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR, En, b.netCC, s,
+                                   k, active,
+                       (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
+                       (b.flags & PYCWB_TILE_ENERGY_BOUND) != 0, nullptr);
       block_argmax<BLOCK>(key, &b.best[c]);
     }
   }

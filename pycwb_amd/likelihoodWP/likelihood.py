@@ -114,6 +114,7 @@ def prepare_likelihood_inputs(config, strains, nIFO, ml=None, FP=None, FX=None, 
         "n_sky_big_cluster": None, "big_cluster_healpix_order": big_cluster_healpix_order,
         "phi_geo_arr_big_cluster": None, "latitude_arr_big_cluster": None,
         "sky_valid_indices_big": None,
+        "_ifos": list(getattr(config, "ifo", []) or []),
     }
     out.update(big)
     return out
@@ -136,9 +137,24 @@ def _device_net(setup: dict, big: bool, device) -> tuple:
             ml=t(ml, torch.int32).reshape(-1),
             nifo_d=torch.tensor([n_ifo], dtype=torch.int32, device=device),
             En_d=torch.tensor([En], dtype=torch.float64, device=device),
-            K=[int(np.abs(ml).max())], ml_host=[np.ascontiguousarray(ml, np.int32)])
+            K=[int(np.abs(ml).max())], ml_host=[np.ascontiguousarray(ml, np.int32)],
+            side_host=[_mirror_side(setup, big)])
         setup[key] = nets
     return setup[key]
+
+
+def _mirror_side(setup: dict, big: bool):
+    """Mirror sheet of each grid direction (scan lane order only), when the
+    detector names are known."""
+    ifos = setup.get("_ifos")
+    if not ifos:
+        return None
+    phi = setup["phi_geo_arr_big_cluster" if big else "phi_geo_arr"]
+    lat = setup["latitude_arr_big_cluster" if big else "latitude_arr"]
+    try:
+        return sky.mirror_side(list(ifos), np.asarray(phi), np.asarray(lat))
+    except ValueError:  # detector not in the built-in table
+        return None
 
 
 def _noise_weights(pixel_arrays) -> np.ndarray:

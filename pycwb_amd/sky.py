@@ -211,14 +211,33 @@ def max_network_delay(ifos, ref_ifo) -> float:
     return max(float(np.linalg.norm(d.vertex - ref)) / C_LIGHT for d in dets)
 
 
-def delay_order(ml: np.ndarray, valid: np.ndarray) -> np.ndarray:
+def mirror_side(ifos, phi: np.ndarray, lat: np.ndarray) -> np.ndarray | None:
+    """Side of each direction relative to the plane through the first three
+    detector vertices (None for fewer than three detectors).  Mirror images
+    through that plane share the delays of those three detectors."""
+    if len(ifos) < 3:
+        return None
+    v = [Detector(i).vertex if isinstance(i, str) else i.vertex for i in ifos[:3]]
+    nrm = np.cross(v[1] - v[0], v[2] - v[0])
+    cl = np.cos(lat)
+    nh = np.stack([cl * np.cos(phi), cl * np.sin(phi), np.sin(lat)])
+    return (nrm @ nh) > 0
+
+
+def delay_order(ml: np.ndarray, valid: np.ndarray, side: np.ndarray | None = None) -> np.ndarray:
     """Valid-list positions sorted by delay tuple ``ml[:, valid[k]]`` (then by k).
 
     Directions that share every delay tap read the same delayed amplitudes, so
     placing them side by side in a wavefront keeps the per-pixel energy test of
-    the sky scan uniform across its lanes.
+    the sky scan uniform across its lanes and the taps a wavefront reads within
+    a narrow range.  With four or more detectors the two mirror sheets of the
+    first three detectors' plane (``side``, per sky direction) are sorted
+    apart: they share the leading delays but not the others, and interleaving
+    them would widen the later detectors' tap ranges.
     """
     ml = np.asarray(ml)
     valid = np.asarray(valid, dtype=np.int64)
     keys = [np.arange(valid.size)] + [ml[i, valid] for i in reversed(range(ml.shape[0]))]
+    if side is not None and ml.shape[0] >= 4:
+        keys.append(np.asarray(side)[valid])
     return np.lexsort(keys).astype(np.int32)

@@ -86,8 +86,8 @@ def test_batched_with_sky_mask(oracle):
 
 
 def test_exact_skip_and_order_are_bit_identical():
-    """The masked-pixel skip, the regulator early exit and the delay-tuple lane
-    order change nothing in the results."""
+    """The masked-pixel skip, the tile energy bound, the regulator early exit
+    and the delay-tuple lane order change nothing in the results."""
     from pycwb_amd import _lib
     from pycwb_amd.batch import BatchedSkyScan, PlainLoop, net_tables, synth_batch
 
@@ -98,9 +98,14 @@ def test_exact_skip_and_order_are_bit_identical():
     nt = net_tables(nets, device=dev)
     valid = torch.arange(nets[0].n_sky, dtype=torch.int64, device=dev)
     outs = []
-    for flags, ordered in [(0, False), (_lib.DEFAULT_FLAGS, True), (_lib.SKIP_MASKED_PIXELS, False),
-                           (_lib.REGULATOR_EARLY_EXIT, True)]:
-        bs = BatchedSkyScan(b, nt, valid, 0.1667, 0.5, 0.5, block=128, grid=23, flags=flags,
+    skip_bound = _lib.SKIP_MASKED_PIXELS | _lib.TILE_ENERGY_BOUND
+    for flags, ordered, block in [(0, False, 128), (_lib.DEFAULT_FLAGS, True, 128),
+                                  (_lib.SKIP_MASKED_PIXELS, False, 128),
+                                  (_lib.REGULATOR_EARLY_EXIT, True, 128),
+                                  (skip_bound, True, 64), (skip_bound, False, 64),
+                                  (_lib.DEFAULT_FLAGS, True, 256)]:
+        print(f"case flags={flags} ordered={ordered} block={block}", flush=True)
+        bs = BatchedSkyScan(b, nt, valid, 0.1667, 0.5, 0.5, block=block, grid=23, flags=flags,
                             ordered=ordered)
         bs.launch()
         outs.append(bs.results())
