@@ -51,6 +51,12 @@ extern "C" {
 #define PYCWB_SKIP_MASKED_PIXELS 1
 #define PYCWB_REGULATOR_EARLY_EXIT 2
 #define PYCWB_TILE_ENERGY_BOUND 4
+/*   PYCWB_REGULATOR_BOUNDS (with PYCWB_REGULATOR_EARLY_EXIT): per-pixel bounds
+ *     on the DPF quality terms, from the cluster's noise-weight range and the
+ *     direction's antenna patterns, decide a direction's count before or
+ *     during its sweep whenever the final sum is bracketed on one side of
+ *     the threshold (margins cover float32 and float64 rounding). */
+#define PYCWB_REGULATOR_BOUNDS 8
 
 const char* pycwb_version(void);
 
@@ -152,6 +158,7 @@ typedef struct pycwb_batch_desc {
   int block;      /* sky directions per workgroup: 64, 128 or 256 */
   int grid;       /* persistent workgroups */
   int use_graph;  /* capture the launches in a hipGraph */
+  double* weight_stats; /* scratch, 16 doubles per cluster, or NULL (no regulator bounds) */
 } pycwb_batch_desc;
 
 typedef struct pycwb_batch pycwb_batch;

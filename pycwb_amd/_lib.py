@@ -20,7 +20,9 @@ PYCWB_ERR_HIP = 2
 SKIP_MASKED_PIXELS = 1
 REGULATOR_EARLY_EXIT = 2
 TILE_ENERGY_BOUND = 4
-DEFAULT_FLAGS = SKIP_MASKED_PIXELS | REGULATOR_EARLY_EXIT | TILE_ENERGY_BOUND
+REGULATOR_BOUNDS = 8
+DEFAULT_FLAGS = SKIP_MASKED_PIXELS | REGULATOR_EARLY_EXIT | TILE_ENERGY_BOUND | REGULATOR_BOUNDS
+WSTATS_PER_CLUSTER = 16
 
 _lock = threading.Lock()
 _lib = None
@@ -71,7 +73,7 @@ class BatchDesc(C.Structure):
         ("reg_f64", vp),
         ("block", i32),
         ("grid", i32),
-        ("use_graph", i32),
+        ("use_graph", i32), ("weight_stats", vp),
     ]
 
 

@@ -310,6 +310,7 @@ class BatchedSkyScan:
         self.l_max = torch.zeros(C_, dtype=torch.int64, device=dev)
         self.stat_max = torch.zeros(C_, dtype=torch.float64, device=dev)
         self.reg_f64 = torch.zeros(C_, dtype=torch.float64, device=dev)
+        self.wstats = torch.zeros(C_ * _lib.WSTATS_PER_CLUSTER, dtype=torch.float64, device=dev)
         d = _lib.BatchDesc()
         d.n_clusters = C_
         d.n_sky = nets.n_sky
@@ -340,6 +341,7 @@ class BatchedSkyScan:
         d.block = int(block)
         d.grid = int(grid or default_grid(block))
         d.use_graph = 1 if use_graph else 0
+        d.weight_stats = ptr(self.wstats)
         self.desc = d
         self.block, self.grid = d.block, d.grid
         h = C.c_void_p()

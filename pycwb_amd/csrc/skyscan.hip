@@ -102,31 +102,192 @@ __device__ __forceinline__ void block_count(unsigned v, unsigned* dst) {
 // Per-tile bodies shared by the plain and batched kernels.
 // ---------------------------------------------------------------------------
 
-// DPF quality of the direction valid[k]; returns 1 if quality > gamma.
-// Every term fx / (ni + 1e-9) is >= 0, so the float32 partial sums are
-// non-decreasing and NN <= P; once sqrt(NI_partial / (P + 0.01)) > gamma the
-// final quality sqrt(NI / (NN + 0.01)) exceeds gamma too (correctly rounded
-// division and sqrt are monotone), and the sweep can stop with the same count.
+// Noise-weight statistics of a cluster for the regulator bounds, float64:
+// [0] min_j R2_j, [1] max_j R2_j with R2_j = sum_i r_ji^2, [2 + i] min_j
+// r_ji^2 / R2_j (i < n).  kWStats doubles per cluster.
+constexpr int kWStats = 16;
+
+template <int BLK>
+__device__ void weight_stats_block(const float* __restrict__ rms, int P, int n,
+                                   double* __restrict__ out) {
+  __shared__ double s_red[BLK / 64][2 + 8];
+  double lo = __builtin_inf(), hi = 0.0, wmin[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) wmin[i] = __builtin_inf();
+  for (int j = threadIdx.x; j < P; j += BLK) {
+    double r2[8], R2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i < n) {
+        const double r = (double)rms[(int64_t)j * n + i];
+        r2[i] = r * r;
+        R2 += r2[i];
+      }
+    }
+    lo = R2 < lo ? R2 : lo;
+    hi = R2 > hi ? R2 : hi;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i < n) {
+        const double w = r2[i] / R2;
+        wmin[i] = (w < wmin[i] || w != w) ? w : wmin[i];  // NaN sticks
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double a = __shfl_xor(lo, off, 64), b = __shfl_xor(hi, off, 64);
+    lo = (a < lo || a != a) ? a : lo;
+    hi = (b > hi || b != b) ? b : hi;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double c = __shfl_xor(wmin[i], off, 64);
+      wmin[i] = (c < wmin[i] || c != c) ? c : wmin[i];
+    }
+  }
+  if (lane == 0) {
+    s_red[wave][0] = lo;
+    s_red[wave][1] = hi;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s_red[wave][2 + i] = wmin[i];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 + 8) {
+    const int f = threadIdx.x;
+    double v = s_red[0][f];
+    for (int w = 1; w < BLK / 64; ++w) {
+      const double x = s_red[w][f];
+      if (f == 1) v = (x > v || x != x) ? x : v;
+      else v = (x < v || x != x) ? x : v;
+    }
+    out[f] = v;
+  }
+  for (int f = 2 + 8 + (int)threadIdx.x; f < kWStats; f += BLK) out[f] = 0.0;
+  __syncthreads();
+}
+
+// Bounds on one direction's regulator terms t_j = fx_j / (ni_j + 1e-9).
+// With M_j = sum_i r_ji^2 v_i v_i^T (v_i = (F+_i, Fx_i)), exact arithmetic
+// gives fx_j = lambda_min(M_j) and 1/n <= ni_j <= 1 (Cauchy-Schwarz on the
+// dominant-frame vector); the float32 pipeline stays within 1e-4 * R2max *
+// max|v|^2 of lambda_min (its rounding is ~1e-6 of fp <= R2 max|v|^2).
+//   lower: lambda_min is concave in the weights, and every pixel's normalised
+//     weights lie in the simplex {w_i >= wmin_i}, so lambda_min(M_j) >= R2min
+//     times the minimum of lambda_min over that simplex's n vertices;
+//   upper: lambda_min(M(w)) <= u^T M(w) u <= max_i (u.v_i)^2 for any unit u,
+//     so lambda_min(M_j) <= R2max * min over sampled u of max_i (u.v_i)^2.
+//     Used only when fp is bounded away from 0 (fp^2 >= 1e-3), which also
+//     makes every pixel count in NN.
+struct RegBounds {
+  double t_lb;  // >= 0
+  double t_ub;  // < 0: no upper bound
+};
+
+template <int N>
+__device__ __forceinline__ RegBounds regulator_bounds(const float (&fp0)[N], const float (&fx0)[N],
+                                                      const double* __restrict__ ws) {
+  const double R2min = ws[0], R2max = ws[1];
+  double a[N], b[N], v2max = 0.0, sw = 0.0, fpw = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    a[i] = fp0[i];
+    b[i] = fx0[i];
+    const double v2 = a[i] * a[i] + b[i] * b[i];
+    v2max = v2 > v2max ? v2 : v2max;
+    sw += ws[2 + i];
+    fpw += ws[2 + i] * v2;
+  }
+  const double slack = 1e-4 * R2max * v2max;
+  double C = __builtin_inf();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double g11 = 0.0, g12 = 0.0, g22 = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double w = i == k ? 1.0 - (sw - ws[2 + k]) : ws[2 + i];
+      g11 += w * a[i] * a[i];
+      g12 += w * a[i] * b[i];
+      g22 += w * b[i] * b[i];
+    }
+    const double h = 0.5 * (g11 + g22), det = g11 * g22 - g12 * g12;
+    const double disc = h * h - det;
+    const double lm = h - __builtin_sqrt(disc > 0.0 ? disc : 0.0);
+    C = lm < C ? lm : C;
+  }
+  RegBounds r;
+  r.t_lb = (R2min * C - slack) / (1.0 + 2e-5);
+  if (!(r.t_lb > 0.0)) r.t_lb = 0.0;
+  r.t_ub = -1.0;
+  const double fp_lb = 0.5 * R2min * fpw * (1.0 - 1e-5);
+  if (fp_lb * fp_lb >= 1e-3) {
+    double B = __builtin_inf();
+    for (int q = 0; q < 32; ++q) {
+      double su, cu;
+      sincos(q * (3.14159265358979323846 / 32.0), &su, &cu);
+      double mx = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double p = cu * a[i] + su * b[i];
+        mx = p * p > mx ? p * p : mx;
+      }
+      B = mx < B ? mx : B;
+    }
+    r.t_ub = (double)N * (1.0 + 2e-5) * (R2max * B + slack);
+  }
+  if (!(r.t_ub == r.t_ub)) r.t_ub = -1.0;  // NaN statistics: no upper bound
+  return r;
+}
+
+// DPF quality of the direction s; returns 1 if quality > gamma.
+// The reference sums NI = sum_j fx_j / (ni_j + 1e-9) in float32 over j = 0..P-1
+// and counts sqrt(NI / (NN + 0.01)) > gamma (dpf.py:25-34, 444-453).  Every
+// term is >= 0, so with the partial sum NI_k after k pixels and per-term bounds
+// t_lb <= t_j <= t_ub the final sum lies in
+// [(NI_k + (P-k) t_lb)(1 - P u'), (NI_k + (P-k) t_ub)(1 + P u')], u' = 1.2e-7
+// covering float32 rounding of the remaining additions.  The sweep stops at a
+// checkpoint (every 16 pixels, and before the first) as soon as that interval
+// lies entirely above gamma^2 (P + 0.01) (NN <= P, so the count is 1) or,
+// with an upper bound (which guarantees NN = P), entirely below (count 0);
+// the margins 1e-9 cover the float64 division and square root.  Without
+// bounds (t_lb = 0, no t_ub) this is the early exit on the partial sum alone.
 template <int N>
 __device__ __forceinline__ unsigned regulator_lane(const float* __restrict__ FP,
                                                    const float* __restrict__ FX,
                                                    const float* __restrict__ rms, int P,
-                                                   int64_t s, double gamma, bool early) {
+                                                   int64_t s, double gamma, bool early,
+                                                   const double* __restrict__ ws) {
   float fp0[N], fx0[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     fp0[i] = FP[s * N + i];
     fx0[i] = FX[s * N + i];
   }
+  RegBounds bd;
+  bd.t_lb = 0.0;
+  bd.t_ub = -1.0;
+  if (ws) bd = regulator_bounds<N>(fp0, fx0, ws);
+  const double den = (double)P + 0.01;
+  const double thr_hi = gamma * gamma * den * (1.0 + 1e-9);
+  const double thr_lo = gamma * gamma * den * (1.0 - 1e-9);
+  const double sum_lo = 1.0 - (double)P * 1.2e-7, sum_hi = 1.0 + (double)P * 1.2e-7;
   float NI = 0.f;
   unsigned NN = 0;
-  const double bound_den = (double)P + 0.01;
+  if (early) {
+    const double rest = (double)P;
+    if (rest * bd.t_lb * sum_lo > thr_hi) return 1u;
+    if (bd.t_ub >= 0.0 && rest * bd.t_ub * sum_hi < thr_lo) return 0u;
+  }
   for (int j = 0; j < P; ++j) {
     float r[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) r[i] = rms[(int64_t)j * N + i];
     regulator_pixel<N>(r, fp0, fx0, NI, NN);
-    if (early && (j & 15) == 15 && __builtin_sqrt((double)NI / bound_den) > gamma) return 1u;
+    if (early && (j & 15) == 15) {
+      const double rest = (double)(P - 1 - j);
+      if (((double)NI + rest * bd.t_lb) * sum_lo > thr_hi) return 1u;
+      if (bd.t_ub >= 0.0 && ((double)NI + rest * bd.t_ub) * sum_hi < thr_lo) return 0u;
+    }
   }
   const double q = __builtin_sqrt((double)NI / ((double)NN + 0.01));
   return q > gamma ? 1u : 0u;
@@ -469,12 +630,19 @@ template <int N, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_regulator_plain(
     const float* __restrict__ FP, const float* __restrict__ FX, const float* __restrict__ rms,
     int P, const int64_t* __restrict__ valid, int64_t n_valid, double gamma,
-    unsigned* __restrict__ count, int flags) {
+    unsigned* __restrict__ count, int flags, const double* __restrict__ ws) {
   const int64_t k = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   unsigned v = 0;
   if (k < n_valid)
-    v = regulator_lane<N>(FP, FX, rms, P, valid[k], gamma, (flags & PYCWB_REGULATOR_EARLY_EXIT) != 0);
+    v = regulator_lane<N>(FP, FX, rms, P, valid[k], gamma,
+                          (flags & PYCWB_REGULATOR_EARLY_EXIT) != 0,
+                          (flags & PYCWB_REGULATOR_BOUNDS) ? ws : nullptr);
   block_count<BLOCK>(v, count);
+}
+
+__global__ __launch_bounds__(256) void k_weight_stats_plain(const float* __restrict__ rms, int P,
+                                                            int n, double* __restrict__ out) {
+  weight_stats_block<256>(rms, P, n, out);
 }
 
 __global__ void k_regulator_finish(const unsigned* __restrict__ count, int64_t n_valid,
@@ -566,6 +734,7 @@ struct BatchDev {
   int64_t* l_max;
   double* stat_max;
   double* reg_f64;
+  double* wstats;           // per cluster kWStats doubles, or NULL (no regulator bounds)
 };
 
 __global__ void k_batch_init(BatchDev b, int n_tickets) {
@@ -575,6 +744,12 @@ __global__ void k_batch_init(BatchDev b, int n_tickets) {
     b.counts[c] = 0u;
     b.best[c] = 0ull;
   }
+}
+
+__global__ __launch_bounds__(256) void k_batch_weights(BatchDev b) {
+  const int c = blockIdx.x;
+  weight_stats_block<256>(b.rms + b.rms_off[c], b.npix[c], b.net_nifo[b.net[c]],
+                          b.wstats + (int64_t)c * kWStats);
 }
 
 template <int N, int BLOCK, int PHASE>
@@ -630,8 +805,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev b, int firs
       }
       block_count<BLOCK>(cnt, &b.counts[c]);
     } else if (PHASE == 0) {
+      const double* ws = (b.wstats && (b.flags & PYCWB_REGULATOR_BOUNDS))
+                             ? b.wstats + (int64_t)c * kWStats : nullptr;
       const unsigned v = active ? regulator_lane<N>(FP, FX, rms, P, s, b.gamma,
-                                                    (b.flags & PYCWB_REGULATOR_EARLY_EXIT) != 0)
+                                                    (b.flags & PYCWB_REGULATOR_EARLY_EXIT) != 0,
+                                                    ws)
                                 : 0u;
       block_count<BLOCK>(v, &b.counts[c]);
     } else {
@@ -673,13 +851,13 @@ __global__ void k_batch_finish(BatchDev b) {
 template <int BLOCK>
 int launch_regulator_plain(int n, const float* FP, const float* FX, const float* rms, int P,
                            const int64_t* valid, int64_t n_valid, double gamma,
-                           unsigned* count, int flags, hipStream_t st) {
+                           unsigned* count, int flags, const double* ws, hipStream_t st) {
   const unsigned grid = (unsigned)((n_valid + BLOCK - 1) / BLOCK);
   switch (n) {
 #define CASE(NN)                                                                          \
   case NN:                                                                                \
     hipLaunchKernelGGL((k_regulator_plain<NN, BLOCK>), dim3(grid), dim3(BLOCK), 0, st, FP, \
-                       FX, rms, P, valid, n_valid, gamma, count, flags);                  \
+                       FX, rms, P, valid, n_valid, gamma, count, flags, ws);              \
     break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
@@ -738,6 +916,7 @@ size_t plain_ws_bytes(int n, int P, int nd) {
 struct PlainWs {
   unsigned long long* best;  // argmax key
   unsigned* count;           // regulator count
+  double* wstats;            // noise-weight statistics (kWStats doubles at byte 64)
   float2* td;
 };
 
@@ -745,9 +924,11 @@ PlainWs plain_ws(void* ws) {
   PlainWs w;
   w.best = (unsigned long long*)ws;
   w.count = (unsigned*)((char*)ws + 8);
+  w.wstats = (double*)((char*)ws + 64);
   w.td = (float2*)((char*)ws + kCounterBytes);
   return w;
 }
+static_assert(64 + kWStats * sizeof(double) <= kCounterBytes, "plain workspace layout");
 
 bool valid_args(int n, int P, int64_t S, int64_t n_valid) {
   return n >= 1 && n <= kMaxIfo && P >= 1 && S >= 1 && n_valid >= 1 && n_valid <= S &&
@@ -814,8 +995,13 @@ int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, co
   PlainWs w = plain_ws(workspace);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (float*)w.count, (int64_t)1, 0.f);
   HIP_TRY(hipGetLastError());
+  if (flags & PYCWB_REGULATOR_BOUNDS) {
+    hipLaunchKernelGGL(k_weight_stats_plain, dim3(1), dim3(256), 0, st, rms, n_pix, n_ifo,
+                       w.wstats);
+    HIP_TRY(hipGetLastError());
+  }
   int rc = launch_regulator_plain<256>(n_ifo, FP, FX, rms, n_pix, sky_valid_indices, n_valid,
-                                       gamma_regulator, w.count, flags, st);
+                                       gamma_regulator, w.count, flags, w.wstats, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_regulator_finish, dim3(1), dim3(64), 0, st, w.count, n_valid,
                      network_energy_threshold, reg0, REG, reg_f64);
@@ -925,6 +1111,7 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
   v.l_max = d->l_max;
   v.stat_max = d->sky_stat_max;
   v.reg_f64 = d->reg_f64;
+  v.wstats = d->weight_stats;
   b->block = d->block;
   b->grid = d->grid;
   b->n_groups = d->n_groups;
@@ -972,6 +1159,10 @@ int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
     hipLaunchKernelGGL(k_batch_init, dim3(grid_for(d.n_clusters, 4096)), dim3(256), 0, st, d,
                        2 * b->n_groups);
     HIP_TRY(hipGetLastError());
+    if (d.wstats && (d.flags & PYCWB_REGULATOR_BOUNDS)) {
+      hipLaunchKernelGGL(k_batch_weights, dim3((unsigned)d.n_clusters), dim3(256), 0, st, d);
+      HIP_TRY(hipGetLastError());
+    }
     return PYCWB_OK;
   }
   if (stage == 3) {

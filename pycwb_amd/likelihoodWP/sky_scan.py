@@ -58,6 +58,13 @@ def scan_flags(*arrays) -> int:
     return _lib.DEFAULT_FLAGS if ok else _lib.REGULATOR_EARLY_EXIT
 
 
+def _dense(*tensors) -> None:
+    """The kernels index raw pointers: every tensor must be contiguous."""
+    for t in tensors:
+        if t is not None and not t.is_contiguous():
+            raise ValueError("device arrays passed to the C ABI must be contiguous")
+
+
 class PlainClusterScan:
     """One cluster's regulator + scan on the GPU (one launch family per call)."""
 
@@ -72,12 +79,14 @@ class PlainClusterScan:
         self.stat_max = torch.zeros(1, dtype=torch.float64, device=self.device)
 
     def load_td(self, td00: torch.Tensor, td90: torch.Tensor, stream=None) -> None:
+        _dense(td00, td90)
         rc = lib().pycwb_td_relayout(self.n_ifo, self.n_pix, self.n_delay, ptr(td00), ptr(td90),
                                      ptr(self.ws), self.ws.numel(), stream_ptr(stream))
         _lib.check(rc, "pycwb_td_relayout")
 
     def regulator(self, FP, FX, rms, n_sky, gamma, En, reg0, valid, stream=None,
                   flags=_lib.DEFAULT_FLAGS) -> None:
+        _dense(FP, FX, rms, valid)
         rc = lib().pycwb_dpf_regulator(self.n_ifo, self.n_pix, int(n_sky), ptr(FP), ptr(FX),
                                        ptr(rms), float(gamma), float(En), float(reg0), ptr(valid),
                                        int(valid.numel()), int(flags), ptr(self.REG),
@@ -87,6 +96,7 @@ class PlainClusterScan:
 
     def scan(self, FP, FX, ml, rms, n_sky, netCC, En, valid, maps=None, stream=None,
              td_packed=None, sky_order=None, flags=_lib.DEFAULT_FLAGS) -> None:
+        _dense(FP, FX, ml, rms, valid, maps, td_packed, sky_order)
         rc = lib().pycwb_sky_scan_plain(self.n_ifo, self.n_pix, int(n_sky), self.n_delay,
                                         ptr(FP), ptr(FX), ptr(ml), ptr(rms), ptr(td_packed),
                                         ptr(self.REG), float(netCC), float(En), ptr(valid),

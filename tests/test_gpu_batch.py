@@ -103,6 +103,8 @@ def test_exact_skip_and_order_are_bit_identical():
                                   (_lib.SKIP_MASKED_PIXELS, False, 128),
                                   (_lib.REGULATOR_EARLY_EXIT, True, 128),
                                   (skip_bound, True, 64), (skip_bound, False, 64),
+                                  (_lib.REGULATOR_EARLY_EXIT | _lib.REGULATOR_BOUNDS, True, 64),
+                                  (_lib.REGULATOR_EARLY_EXIT | _lib.REGULATOR_BOUNDS, False, 256),
                                   (_lib.DEFAULT_FLAGS, True, 256)]:
         print(f"case flags={flags} ordered={ordered} block={block}", flush=True)
         bs = BatchedSkyScan(b, nt, valid, 0.1667, 0.5, 0.5, block=block, grid=23, flags=flags,

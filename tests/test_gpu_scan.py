@@ -138,3 +138,40 @@ def test_plain_scan_full_order7_cluster(oracle):
                                           valid)
     _assert_maps_equal(np.stack(g[1:12]), maps_o)
     assert g[0] == l_o and g[12] == smax_o
+
+
+@pytest.mark.parametrize("ifos,spread", [(synth.NETWORK_3, 0.1), (synth.NETWORK_3, 3.0),
+                                         (synth.NETWORK_4, 1.0), (synth.NETWORK_5, 0.5)])
+def test_regulator_with_bounds_matches_oracle(oracle, ifos, spread):
+    """The regulator's bound-based decisions leave the count unchanged, also
+    for strongly unbalanced noise weights (log-normal rms of the given spread)
+    and several thresholds on both sides of the typical quality."""
+    import torch
+
+    from pycwb_amd import _lib
+    from pycwb_amd.device import require_gpu
+    from pycwb_amd.likelihoodWP.sky_scan import PlainClusterScan
+
+    print(f"case regulator bounds {len(ifos)} det spread {spread}", flush=True)
+    net = synth.make_network(ifos, order=4, K=16)
+    n = net.n_ifo
+    rng = np.random.default_rng(int(spread * 10) + n)
+    dev = require_gpu()
+    valid = np.arange(net.n_sky)
+    for P in (1, 17, 160, 700):
+        sigma = np.exp(rng.normal(0.0, spread, size=(n, P)))
+        rms = synth.noise_weights(sigma)
+        rms_d = torch.from_numpy(rms).to(dev)
+        FP = torch.from_numpy(np.ascontiguousarray(net.FP_t)).to(dev)
+        FX = torch.from_numpy(np.ascontiguousarray(net.FX_t)).to(dev)
+        vd = torch.from_numpy(valid).to(dev)
+        for gamma in (0.02, 0.1667, 0.3, 0.6):
+            sc = PlainClusterScan(n, P, 33, dev)
+            for flags in (_lib.DEFAULT_FLAGS, 0):
+                sc.regulator(FP, FX, rms_d, net.n_sky, gamma, 4.0 * n,
+                             float(np.float32(0.5 * np.sqrt(2))), vd, flags=flags)
+                torch.cuda.synchronize()
+                reg_g = float(sc.reg_f64.item())
+                reg_o = oracle.calculate_dpf(net.FP_t, net.FX_t, rms, net.n_sky, n, gamma,
+                                             4.0 * n, valid)
+                assert reg_g == reg_o, (P, gamma, flags)
