@@ -119,6 +119,23 @@ def oracle_parity(batch, nets, nt, rb, lm_gpu, gamma, delta, netCC, reg0, n_chec
             "oracle_Lm_max_rel_err": lm_err}
 
 
+def profiled_traffic(clusters, order, K):
+    """HBM bytes per scan-stage launch of the default workload, from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs, each
+    scaled by a 1 GiB copy); None for other workloads or without the file."""
+    path = os.path.join(ROOT, "profiles", "batched_step_4096clusters_order7.md")
+    if (clusters, order, K) != (4096, 7, 64):
+        return None, None
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_scan_stage:"):
+                    return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def refinement_leg(n_clusters, order, K, steps, stream, log):
     """Gradient refinement: 3-detector clusters, grid argmax, then projected
     gradient ascent of the continuous statistic (forward + angle gradient per
@@ -393,6 +410,7 @@ def main():
                                     stream, log)
     wdm = None if args.no_wdm else wdm_leg(stream, log)
 
+    traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
         "metric": "sky x cluster-pixel evals/s, batched vs plain GPU at Lm rel.err<=1e-5",
         "value": evals / ms * 1e3,
@@ -416,7 +434,8 @@ def main():
         "parity": parity,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": None, "kernel": "k_batch_persistent<N,BLOCK,1> (scan stage)",
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "k_batch_persistent<N,BLOCK,1> (scan stage)",
                      "kernel_ms": round(scan_ms, 3), "required_flops": rf,
                      "nominal_flops": sf,
                      "nominal_tflops": sf / (scan_ms * 1e-3) / 1e12,
