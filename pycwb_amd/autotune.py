@@ -3,9 +3,12 @@
 Tunes the two shape parameters of the persistent work-queue kernels:
   block : sky directions per workgroup (64, 128, 256) -- the sky-tile size;
   grid  : persistent workgroups (waves per SIMD of resident work).
-It times the scan stage alone with HIP events on a seeded calibration batch
-drawn from the same cluster-size distribution, keeps the fastest shape, and
-caches the choice per (device, detector counts, K, directions) in
+It times the regulator and scan stages together (the persistent kernels
+the shape affects) with HIP events on a seeded calibration batch drawn from
+the same cluster-size distribution, keeps the fastest shape (among shapes
+within 2 % of the fastest, the largest grid: more resident waves hide the
+regulator's latency better on the full batch), and caches the choice per
+(device, detector counts, K, directions) in
 ``$XDG_CACHE_HOME/pycwb_amd/autotune.json``.
 """
 
@@ -50,14 +53,15 @@ def _store(d):
         pass
 
 
-def time_scan_stage(bs, reps: int = 2) -> float:
+def time_stages(bs, reps: int = 2) -> float:
+    """Best-of-reps time of the regulator + scan stages (ms)."""
     stream = torch.cuda.current_stream()
     best = float("inf")
     for _ in range(reps):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         bs.launch_stage(0, stream)
-        bs.launch_stage(1, stream)
         a.record(stream)
+        bs.launch_stage(1, stream)
         bs.launch_stage(2, stream)
         b.record(stream)
         bs.launch_stage(3, stream)
@@ -66,9 +70,9 @@ def time_scan_stage(bs, reps: int = 2) -> float:
     return best
 
 
-def tune_batched(net_list, nets, valid, gamma, delta, netCC, n_calib: int = 48,
+def tune_batched(net_list, nets, valid, gamma, delta, netCC, n_calib: int = 96,
                  use_cache: bool = True, log=None) -> dict:
-    """Returns {"block", "grid", "source", "ms"} for the batched scan."""
+    """Returns {"block", "grid", "source", "ms"} for the batched regulator + scan."""
     from .batch import BatchedSkyScan, synth_batch
 
     key = _key(nets, int(valid.numel()))
@@ -92,12 +96,14 @@ def tune_batched(net_list, nets, valid, gamma, delta, netCC, n_calib: int = 48,
             bs.launch_stage(1)
             bs.launch_stage(2)
             bs.launch_stage(3)
-            ms = time_scan_stage(bs)
+            ms = time_stages(bs)
             bs.close()
             results.append((ms, block, grid))
             if log:
-                log(f"autotune block={block} grid={grid}: scan {ms:.2f} ms")
-    ms, block, grid = min(results)
+                log(f"autotune block={block} grid={grid}: regulator+scan {ms:.2f} ms")
+    fastest = min(r[0] for r in results)
+    ms, block, grid = max((r for r in results if r[0] <= 1.02 * fastest),
+                          key=lambda r: (r[2] * r[1], -r[0]))
     r = {"block": block, "grid": grid, "ms": ms, "source": "tuned",
          "table": [[b, g, m] for m, b, g in results]}
     cache[key] = {k: v for k, v in r.items() if k != "source"}

@@ -752,8 +752,17 @@ __global__ __launch_bounds__(256) void k_batch_weights(BatchDev b) {
                           b.wstats + (int64_t)c * kWStats);
 }
 
+// Waves per SIMD the compiler must leave room for (caps VGPRs): measured on
+// MI355X, the regulator runs fastest at 5 and the scan at 4 (a few spilled
+// registers cost less than the lost occupancy).
+template <int PHASE>
+struct PhaseOccupancy {
+  static constexpr int waves = PHASE == 0 ? 5 : (PHASE == 1 ? 4 : 8);
+};
+
 template <int N, int BLOCK, int PHASE>
-__global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchDev b, int first, int count,
+__global__ __launch_bounds__(BLOCK, PhaseOccupancy<PHASE>::waves) void k_batch_persistent(
+    BatchDev b, int first, int count,
                                                             unsigned* ticket) {
   __shared__ unsigned s_item;
   const unsigned n_items = (unsigned)count * (unsigned)b.tiles_per_cluster;
