@@ -279,3 +279,47 @@ class SkyObjective:
         tau, fp, fx = (_c(a, np.float64) for a in (tau, fp, fx))
         return float(self._at(self.n, self.P, self.K, _p(self.rms), _p(self.td), _p(tau),
                               _p(fp), _p(fx), self.reg0, self.reg1, self.En))
+
+
+# ---- time-delay amplitudes (td_oracle.c) ----
+
+def td_fourier(beta, which):
+    """which 0: Cos2 table, 1: SinCos table (cosine-series coefficients)."""
+    lib = load()
+    lib.td_oracle_size.restype = C.c_int
+    lib.td_oracle_size.argtypes = [C.c_int, C.c_int]
+    n = lib.td_oracle_size(beta, which)
+    out = np.zeros(n)
+    lib.td_oracle_fourier.restype = C.c_int
+    lib.td_oracle_fourier.argtypes = [C.c_int, C.c_int, vp]
+    lib.td_oracle_fourier(beta, which, _p(out))
+    return out
+
+
+def td_filters(M, K, beta, n_coeffs, L):
+    """(T0, Tx) float64 (2J+1, 2 n_coeffs+1), J = M L."""
+    lib = load()
+    fn = lib.td_oracle_filters
+    fn.restype = None
+    fn.argtypes = [C.c_int] * 5 + [vp, vp]
+    J = M * L
+    T0 = np.zeros((2 * J + 1, 2 * n_coeffs + 1))
+    Tx = np.zeros_like(T0)
+    fn(M, K, beta, n_coeffs, L, _p(T0), _p(Tx))
+    return T0, Tx
+
+
+def td_vectors(pixel_indices, p00, p90, T0, Tx, M, n_coeffs, K, J):
+    """(P, 4K+2) float32 rows [a00(-K..K), a90(-K..K)]."""
+    lib = load()
+    fn = lib.td_oracle_vectors
+    fn.restype = None
+    fn.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                   vp]
+    pix = _c(pixel_indices, np.int32)
+    p00, p90 = _c(p00, np.float32), _c(p90, np.float32)
+    T0, Tx = _c(T0, np.float64), _c(Tx, np.float64)
+    out = np.zeros((pix.size, 4 * K + 2), dtype=np.float32)
+    fn(_p(pix), pix.size, _p(p00), _p(p90), p00.shape[0], _p(T0), _p(Tx), M, n_coeffs, K, J,
+       _p(out))
+    return out

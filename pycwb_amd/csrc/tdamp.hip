@@ -1,0 +1,159 @@
+// Time-delay amplitude extraction on MI355X: the td00/td90 tables of the
+// likelihood from the WDM maps.  Restates batch_get_td_vecs and
+// _get_pixel_amplitude_nb (pycwb/utils/td_vector_kernels.py:16-199, cWB's
+// WDM<T>::getTDamp): for pixel (n, m) and delay dT in [-K, K] (steps of
+// tau/L, J = M L per time bin) the amplitude is the same-band filter sum of
+// layer m plus the two cross-band sums of layers m -+ 1, over the 2 n_c + 1
+// time bins around n, with the whole-bin shift dT = s J + r (truncation
+// toward zero) and, for odd s, the quadrature swap with the (n + m) sign.
+// Products float32 x float64 and sums in float64 as the Numba kernel; the
+// result is rounded to float32.
+// One thread per (pixel, delay): threads of one pixel read the same plane
+// values (broadcast) and neighbouring filter rows (cached).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "skyscan_lane.hh"
+#include "pycwb_amd.h"
+
+namespace {
+
+struct TdArgs {
+  const int32_t* pix;
+  int64_t n_pixels;
+  const float* p00;
+  const float* p90;
+  int64_t n_rows;  // rows of the zero-padded planes (n_time + 2 n_c)
+  int M;
+  const double* T0;
+  const double* Tx;
+  int nc;
+  int J;
+  int K;
+  float* out;
+  int64_t row_stride;  // floats between consecutive pixels' outputs
+  int interleave;      // 0: [a00(-K..K), a90(-K..K)]; 1: (a00, a90) pairs
+};
+
+// even / odd filter sums of one layer; rows outside the planes read as zero
+__device__ __forceinline__ void band_sums(const float* __restrict__ plane, int64_t n_rows, int M1,
+                                          int n, int m, const double* __restrict__ T, int W,
+                                          double& se, double& so) {
+  double e = 0.0, o = 0.0;
+  for (int k = 0; k < W; ++k) {
+    const int64_t row = (int64_t)n + k;
+    const double v = (row >= 0 && row < n_rows) ? (double)plane[row * M1 + m] : 0.0;
+    const double t = v * T[k];
+    if (k & 1) o += t;
+    else e += t;
+  }
+  se = e;
+  so = o;
+}
+
+__device__ double pixel_amp(const TdArgs& a, int n, int m, int dT, const float* __restrict__ plane,
+                            bool quad) {
+  const int M = a.M, M1 = M + 1, W = 2 * a.nc + 1;
+  const bool odd_n = n & 1, odd_mn = (m + n) & 1;
+  const bool cancel_even = odd_n ^ quad;
+  const bool edge = m == 0 || m == M;
+  const double* t0 = a.T0 + (int64_t)(dT + a.J) * W;
+  const double* tx = a.Tx + (int64_t)(dT + a.J) * W;
+  const double dt = (double)dT * (double)M / (double)a.J;
+  double se, so;
+  band_sums(plane, a.n_rows, M1, n, m, t0, W, se, so);
+  if (edge && cancel_even) se = 0.0;
+  if (edge && !cancel_even) so = 0.0;
+  const double ph = m * M_PI * dt / M;
+  double r = odd_n ? se * cos(ph) - so * sin(ph) : se * cos(ph) + so * sin(ph);
+  if (m > 0) {
+    double le, lo;
+    band_sums(plane, a.n_rows, M1, n, m - 1, tx, W, le, lo);
+    if (m == 1 && cancel_even) le = 0.0;
+    if (m == 1 && !cancel_even) lo = 0.0;
+    const double lp = (2.0 * m - 1.0) * dt * M_PI / (2.0 * M);
+    double t = odd_n ? (le - lo) * sin(lp) : (le + lo) * sin(lp);
+    if (m == 1 || m == M) t *= 1.4142135623730951;
+    r = odd_mn ? r - t : r + t;
+  }
+  if (m < M) {
+    double he, ho;
+    band_sums(plane, a.n_rows, M1, n, m + 1, tx, W, he, ho);
+    if (m == M - 1 && cancel_even) he = 0.0;
+    if (m == M - 1 && !cancel_even) ho = 0.0;
+    const double hp = (2.0 * m + 1.0) * dt * M_PI / (2.0 * M);
+    double t = odd_n ? (he + ho) * sin(hp) : (he - ho) * sin(hp);
+    if (m == 0 || m == M - 1) t *= 1.4142135623730951;
+    r = odd_mn ? r - t : r + t;
+  }
+  if (edge && cancel_even) return 0.0;
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_td_vectors(TdArgs a) {
+  const int half = 2 * a.K + 1;
+  const int64_t total = a.n_pixels * half;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t p = e / half;
+    const int ki = (int)(e % half);
+    const int idx = a.pix[p];
+    const int n = idx / (a.M + 1), m = idx % (a.M + 1);
+    const int dT = ki - a.K;
+    const int sh = dT >= 0 ? dT / a.J : -((-dT) / a.J);
+    const int sub = dT - sh * a.J, ne = n - sh;
+    double a00, a90;
+    if (sh % 2 != 0) {
+      if ((n + m) % 2 != 0) {
+        a00 = -pixel_amp(a, ne, m, sub, a.p90, true);
+        a90 = pixel_amp(a, ne, m, sub, a.p00, false);
+      } else {
+        a00 = pixel_amp(a, ne, m, sub, a.p90, true);
+        a90 = -pixel_amp(a, ne, m, sub, a.p00, false);
+      }
+    } else {
+      a00 = pixel_amp(a, ne, m, sub, a.p00, false);
+      a90 = pixel_amp(a, ne, m, sub, a.p90, true);
+    }
+    float* o = a.out + p * a.row_stride;
+    if (a.interleave) {
+      o[2 * ki] = (float)a00;
+      o[2 * ki + 1] = (float)a90;
+    } else {
+      o[ki] = (float)a00;
+      o[half + ki] = (float)a90;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pycwb_td_vectors(const int32_t* pixel_index, int64_t n_pixels, const float* padded00,
+                     const float* padded90, int64_t n_rows, int M, const double* T0,
+                     const double* Tx, int n_coeffs, int J, int K, float* out, int64_t row_stride,
+                     int interleave, void* stream) {
+  if (n_pixels < 0 || M < 1 || n_coeffs < 0 || J < 1 || K < 0 || n_rows < 1) return PYCWB_ERR_ARG;
+  if (n_pixels == 0) return PYCWB_OK;
+  if (!pixel_index || !padded00 || !padded90 || !T0 || !Tx || !out) return PYCWB_ERR_ARG;
+  if (K > 64 * J || row_stride < (int64_t)(4 * K + 2)) return PYCWB_ERR_ARG;
+  (void)hipGetLastError();
+  TdArgs a;
+  a.pix = pixel_index; a.n_pixels = n_pixels; a.p00 = padded00; a.p90 = padded90;
+  a.n_rows = n_rows; a.M = M; a.T0 = T0; a.Tx = Tx; a.nc = n_coeffs; a.J = J; a.K = K;
+  a.out = out; a.row_stride = row_stride; a.interleave = interleave ? 1 : 0;
+  const int64_t total = n_pixels * (2 * K + 1);
+  int64_t grid = (total + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(k_td_vectors, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    pycwb::record_error("k_td_vectors", hipGetErrorString(e));
+    return PYCWB_ERR_HIP;
+  }
+  return PYCWB_OK;
+}
+
+}  // extern "C"

@@ -26,23 +26,156 @@ import torch
 from . import _lib
 from .device import lib, ptr, require_gpu, stream_ptr
 
-# Number of Fourier terms per beta order used by the reference's coefficient table.
+# Number of Fourier terms per beta order used by the reference's coefficient
+# tables (Cos, Cos2, SinCos of cwb-core/FourierCoefficients.icc).
 COS_SIZE = {2: 401, 3: 501, 4: 151, 5: 126, 6: 81, 7: 61}
+COS2_SIZE = {2: 101, 3: 61, 4: 46, 5: 36, 6: 36, 7: 31}
+SINCOS_SIZE = {2: 601, 3: 801, 4: 151, 5: 151, 6: 81, 7: 81}
+
+
+def _cosine_series(beta: int, n: int, which: str) -> np.ndarray:
+    """a_0 = int_0^1 f, a_j = sqrt(2) int_0^1 f cos(j pi x) for f = g (which="cos"),
+    g^2 ("cos2") or sin(pi nu) = 2 g sqrt(1 - g^2) ("sincos"), with
+    g = cos(pi/2 nu), nu = I_x(beta, beta)."""
+    x, w = np.polynomial.legendre.leggauss(2048 if which != "cos" else 1024)
+    x, w = 0.5 * (x + 1.0), 0.5 * w
+    m = 2 * beta - 1
+    nu = sum(math.comb(m, k) * x ** k * (1 - x) ** (m - k) for k in range(beta, m + 1))
+    if which == "cos":
+        f = np.cos(0.5 * np.pi * nu)
+    elif which == "cos2":
+        f = np.cos(0.5 * np.pi * nu) ** 2
+    else:
+        f = np.sin(np.pi * nu)
+    j = np.arange(n)[:, None]
+    a = (np.cos(j * np.pi * x[None, :]) * (w * f)[None, :]).sum(axis=1)
+    a[1:] *= math.sqrt(2.0)
+    # nu(1 - x) = 1 - nu(x): g^2 - 1/2 is odd and sin(pi nu) even about x = 1/2,
+    # so their even (j > 0) and odd coefficients vanish exactly
+    if which == "cos2":
+        a[2::2] = 0.0
+    elif which == "sincos":
+        a[1::2] = 0.0
+    return a
 
 
 @lru_cache(maxsize=None)
 def edge_fourier(beta: int) -> np.ndarray:
-    """a_0 = int_0^1 g, a_j = sqrt(2) int_0^1 g cos(j pi x), g = cos(pi/2 I_x(beta, beta))."""
-    n = COS_SIZE[beta]
-    x, w = np.polynomial.legendre.leggauss(1024)
-    x, w = 0.5 * (x + 1.0), 0.5 * w
-    m = 2 * beta - 1
-    nu = sum(math.comb(m, k) * x ** k * (1 - x) ** (m - k) for k in range(beta, m + 1))
-    g = np.cos(0.5 * np.pi * nu)
-    j = np.arange(n)[:, None]
-    a = (np.cos(j * np.pi * x[None, :]) * (w * g)[None, :]).sum(axis=1)
-    a[1:] *= math.sqrt(2.0)
-    return a
+    """Cosine-series coefficients of the band edge g = cos(pi/2 I_x(beta, beta))."""
+    return _cosine_series(beta, COS_SIZE[beta], "cos")
+
+
+@lru_cache(maxsize=None)
+def edge_fourier_cos2(beta: int) -> np.ndarray:
+    """Cosine-series coefficients of g^2 (the reference's Cos2 table)."""
+    return _cosine_series(beta, COS2_SIZE[beta], "cos2")
+
+
+@lru_cache(maxsize=None)
+def edge_fourier_sincos(beta: int) -> np.ndarray:
+    """Cosine-series coefficients of 2 g sqrt(1 - g^2) (the reference's SinCos table)."""
+    return _cosine_series(beta, SINCOS_SIZE[beta], "sincos")
+
+
+def _td_integral1(M: int, K: int, beta: int, n: int, L: int) -> np.ndarray:
+    """WDM<T>::getTDFilter1 (cwb-core/WDM.cc:551-594): n*L samples, step tau/L."""
+    four = edge_fourier_cos2(beta).copy()
+    nf = four.size
+    B = math.pi / K
+    A = (K - M) * math.pi / 2.0 / K / M
+    K2 = float(K) * K
+    gNorm = 2.0 * M / math.pi
+    out = np.empty(n * L)
+    out[0] = (A + four[0] * B) * gNorm
+    four[0] /= math.sqrt(2.0)
+    i = np.arange(1, n * L)
+    di = i / float(L)
+    jj = np.arange(nf)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        terms = di[:, None] / (di[:, None] ** 2 - (jj[None, :] ** 2) * K2) * four[None, :]
+    terms = np.where(jj[None, :] * K * L == i[:, None], 0.0, terms)
+    odd = (jj & 1).astype(bool)
+    s_even = np.cumsum(np.where(odd[None, :], 0.0, terms), axis=1)[:, -1]
+    s_odd = np.cumsum(np.where(odd[None, :], terms, 0.0), axis=1)[:, -1]
+    intAB = np.zeros(i.size)
+    sel = (i % (K * L) == 0) & (i // (K * L) < nf)
+    intAB[sel] = four[i[sel] // (K * L)] * B / 2 * np.cos(di[sel] * A)
+    intAB += 2 * (s_even * np.sin(di * B / 2) * np.cos(di * np.pi / (2 * M))
+                  - s_odd * np.sin(di * np.pi / (2 * M)) * np.cos(di * B / 2))
+    out[1:] = gNorm * (np.sin(di * A) / di + math.sqrt(2.0) * intAB)
+    return out
+
+
+def _td_integral2(M: int, K: int, beta: int, n: int, L: int) -> np.ndarray:
+    """WDM<T>::getTDFilter2 (cwb-core/WDM.cc:596-636): n*L samples, step tau/L."""
+    four = edge_fourier_sincos(beta).copy()
+    nf = four.size
+    B = math.pi / K
+    K2 = float(K) * K
+    gNorm = M / math.pi
+    out = np.empty(n * L)
+    out[0] = four[0] * B * gNorm
+    four[0] /= math.sqrt(2.0)
+    i = np.arange(1, n * L)
+    di = i / float(L)
+    jj = np.arange(0, nf, 2)  # the reference runs j = 0, 2, .. <= nf (nf is odd)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        terms = di[:, None] / (di[:, None] ** 2 - (jj[None, :] ** 2) * K2) * four[jj][None, :]
+    terms = np.where(jj[None, :] * K * L == i[:, None], 0.0, terms)
+    ssum = np.cumsum(terms, axis=1)[:, -1] * (2 * np.sin(di * B / 2))
+    sel = (i % (2 * K * L) == 0) & (i // (K * L) <= nf)
+    idx = i[sel] // (K * L)
+    sign = np.where((i[sel] // (2 * K * L)) & 1, -1.0, 1.0)
+    ssum[sel] += sign * four[np.minimum(idx, nf - 1)] * B / 2
+    out[1:] = gNorm * math.sqrt(2.0) * ssum
+    return out
+
+
+@dataclass
+class TDFilterBank:
+    """Time-delay filter tables (the wdm_wavelet TDFilterBank fields PycWB reads).
+
+    T0, Tx: float64 (2J+1, 2 n_coeffs+1), row i + J for delay i in [-J, J]
+    (steps of tau / L), column j + n_coeffs for time-bin offset j.
+    """
+    T0: np.ndarray
+    Tx: np.ndarray
+    n_coeffs: int
+    M: int
+    L: int
+
+    @property
+    def max_delay(self) -> int:
+        return self.M * self.L
+
+
+def td_filters(M: int, K: int, beta: int, n_coeffs: int, L: int) -> TDFilterBank:
+    """WDM<T>::setTDFilter (cwb-core/WDM.cc:638-700) as dense tables."""
+    n = M * (n_coeffs + 1) + 1
+    f1 = _td_integral1(M, K, beta, n, L)
+    f2 = _td_integral2(M, K, beta, n, L)
+    J = M * L
+    T0 = np.zeros((2 * J + 1, 2 * n_coeffs + 1))
+    Tx = np.zeros((2 * J + 1, 2 * n_coeffs + 1))
+    c = n_coeffs
+    for i in range(-J, J + 1):
+        r = i + J
+        T0[r, c] = f1[abs(i)]
+        Tx[r, c] = f2[abs(i)] / 2.0
+        for j in range(1, n_coeffs + 1):
+            T0[r, c + j] = f1[i + J * j]
+            T0[r, c - j] = f1[J * j - i]
+            Tx[r, c + j] = f2[i + J * j] / 2.0
+            Tx[r, c - j] = f2[J * j - i] / 2.0
+        for j in range(1, n_coeffs + 1):  # bring the i^l factor to C_l form
+            if j % 4 == 1:
+                Tx[r, c - j] *= -1
+            elif j % 4 == 2:
+                Tx[r, c + j] *= -1
+                Tx[r, c - j] *= -1
+            elif j % 4 == 3:
+                Tx[r, c + j] *= -1
+    return TDFilterBank(T0=T0, Tx=Tx, n_coeffs=int(n_coeffs), M=int(M), L=int(L))
 
 
 def wdm_filter(M: int, K: int, beta: int, precision: int, chunk: int = 4096) -> np.ndarray:
@@ -110,6 +243,13 @@ class TimeFrequencyMap:
     def end_freq(self):
         return (self.data.shape[0] - 1) * self.df
 
+    def prepare_td_inputs(self, td_filters, device=None):
+        """Padded quadrature planes and TD filters on the device
+        (time_frequency_map.py:482-521); see pycwb_amd.tdamp."""
+        from .tdamp import TDBatchInputs
+
+        return TDBatchInputs.from_map(self.data, td_filters, device)
+
 
 class WDM:
     """WDM transform with the surface of wdm_wavelet.WDM used by PycWB."""
@@ -126,7 +266,10 @@ class WDM:
         self._dev = {}
 
     def set_td_filter(self, TDSize, upTDF):
-        raise NotImplementedError("time-delay filters are not part of this package")
+        """Time-delay filters with TDSize coefficients per side and upsampling
+        upTDF (WDM.cc:638-700); stored as ``self.td_filters``."""
+        self.td_filters = td_filters(self.M, self.K, self.beta_order, int(TDSize), int(upTDF))
+        return self.td_filters
 
     def _device(self, device):
         if device not in self._dev:
