@@ -10,10 +10,11 @@ work-queue kernels.  Inputs are resident in HBM before timing starts.
 Also measured in the same run: the plain family (one cluster per launch)
 looped over the same clusters, the roofline of the dominant kernel (the scan
 stage) from HIP events on its stream, and the CPU baseline: the oracle's
-OpenMP restatement of PycWB's Numba scan on a bounded sample.  Two side legs
+OpenMP restatement of PycWB's Numba scan on a bounded sample.  Side legs
 report under their own keys: "refinement" (512 three-detector clusters, grid
 argmax then 20 projected-gradient steps of the continuous sky statistic) and
-"wdm" (forward WDM transform of 5 x 600 s at 4096 Hz at M = 8 .. 512).
+"wdm" (forward WDM transform of 5 x 600 s at 4096 Hz at M = 8 .. 512) and
+"tdamp" (TD-amplitude extraction of 200,000 pixels x 129 delays).
 
 Prints one JSON line.
 """
@@ -247,6 +248,42 @@ def wdm_leg(stream, log, seconds=600, rate=4096, n_det=5):
     return res
 
 
+def tdamp_leg(stream, log, M=64, n_pixels=200000, K=64, nc=12):
+    """TD-amplitude extraction: one detector's WDM map of 600 s at 4096 Hz
+    (M = 64: 38,400 time bins x 65 layers), n_pixels pixels, delays -K..K,
+    TD filters with nc coefficients per side.  Work per (pixel, delay): two
+    phases x three layer sums x (2 nc + 1) multiply-adds in float64."""
+    import torch
+
+    from pycwb_amd.tdamp import TDBatchInputs
+    from pycwb_amd.wdm import td_filters
+
+    dev = torch.device("cuda", 0)
+    n_time = 600 * 4096 // M
+    rng = np.random.default_rng(11)
+    data = rng.normal(size=(M + 1, n_time)) + 1j * rng.normal(size=(M + 1, n_time))
+    inp = TDBatchInputs.from_map(data, td_filters(M, M, 6, nc, 1), dev)
+    pix = rng.choice(n_time * (M + 1), size=n_pixels, replace=False).astype(np.int32)
+    out = inp.extract_td_vecs(pix, K)   # warm
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record(stream)
+    for _ in range(reps):
+        out = inp.extract_td_vecs(pix, K, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    amps = n_pixels * (2 * K + 1)
+    flops = amps * 2 * 3 * (2 * nc + 1) * 2
+    res = {"M": M, "pixels": n_pixels, "K": K, "n_coeffs": nc, "ms": round(ms, 4),
+           "amplitudes_per_s": amps / (ms * 1e-3), "tflops_f64": flops / (ms * 1e-3) / 1e12,
+           "frac_f64_peak": flops / (ms * 1e-3) / 1e12 / 78.6,
+           "out_gbs": out.numel() * 4 / (ms * 1e-3) / 1e9}
+    log(f"tdamp: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,7 +302,8 @@ def main():
     ap.add_argument("--refine-steps", type=int, default=20)
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-wdm", action="store_true")
-    ap.add_argument("--only", choices=("refine", "wdm"), default=None,
+    ap.add_argument("--no-tdamp", action="store_true")
+    ap.add_argument("--only", choices=("refine", "wdm", "tdamp"), default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first copy this many GiB device-to-device (counter calibration)")
@@ -300,6 +338,9 @@ def main():
         return 0
     if args.only == "wdm":
         print(json.dumps({"wdm": wdm_leg(torch.cuda.current_stream(), log)}), flush=True)
+        return 0
+    if args.only == "tdamp":
+        print(json.dumps({"tdamp": tdamp_leg(torch.cuda.current_stream(), log)}), flush=True)
         return 0
     t0 = time.perf_counter()
     nets = [synth.make_network(synth.NETWORK_3, order=args.order, K=args.K),
@@ -409,6 +450,7 @@ def main():
         refinement = refinement_leg(args.refine_clusters, args.order, args.K, args.refine_steps,
                                     stream, log)
     wdm = None if args.no_wdm else wdm_leg(stream, log)
+    tdamp = None if args.no_tdamp else tdamp_leg(stream, log)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
@@ -447,6 +489,7 @@ def main():
         "cpu_baseline": cpu,
         "refinement": refinement,
         "wdm": wdm,
+        "tdamp": tdamp,
     }
     print(json.dumps(out), flush=True)
     bs.close()

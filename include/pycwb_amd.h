@@ -275,7 +275,12 @@ int pycwb_sky_grad(const pycwb_grad_desc* desc, void* stream);
  * called through TDBatchInputs.extract_td_vecs, types/td_batch_inputs.py:45-67).
  * padded00/90: (n_rows, M+1) float32 planes, the map's 00/90 phases zero-padded
  * by n_coeffs time bins on both sides (time_frequency_map.py:482-521);
- * T0, Tx: (2J+1, 2 n_coeffs+1) float64 TD filters (WDM<T>::setTDFilter);
+ * T0, Tx: TD filters (WDM<T>::setTDFilter) stored tap-major, (2 n_coeffs+1,
+ * 2J+1) float64: element [k][dT + J] is tap k - n_coeffs of delay dT (the
+ * transpose of the reference's (2J+1, 2 n_coeffs+1) tables, so that threads of
+ * consecutive delays read consecutive addresses);
+ * trig: 4J (sin, cos) float64 pairs of q pi / (2J), q = 0..4J-1 (the phase
+ * table of WDM<T>::setTDFilter, cwb-core/WDM.cc:688-699);
  * pixel_index[p] = n (M+1) + m; delays dT = -K..K.
  * out + p * row_stride receives, float32, either [a00(-K..K), a90(-K..K)]
  * (interleave = 0, the reference's rows) or (a00, a90) pairs per delay
@@ -283,8 +288,8 @@ int pycwb_sky_grad(const pycwb_grad_desc* desc, void* stream);
  * (whole-bin shifts at the map's ends) read as zero. */
 int pycwb_td_vectors(const int32_t* pixel_index, int64_t n_pixels, const float* padded00,
                      const float* padded90, int64_t n_rows, int M, const double* T0,
-                     const double* Tx, int n_coeffs, int J, int K, float* out,
-                     int64_t row_stride, int interleave, void* stream);
+                     const double* Tx, const double* trig, int n_coeffs, int J, int K,
+                     float* out, int64_t row_stride, int interleave, void* stream);
 
 /* WDM forward transform t2w of n_series float64 series (row stride ldx), all
  * with the same filter.  Replaces WDM<T>::t2w (cwb-core/WDM.cc:1385-1534) as

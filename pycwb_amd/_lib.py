@@ -114,8 +114,8 @@ SIGNATURES = {
     "pycwb_sky_statistics": (i32, [C.POINTER(StatsDesc), vp]),
     "pycwb_wdm_t2w": (i32, [vp, i64, i64, i32, i32, i32, vp, i32, vp, i32, vp, vp]),
     "pycwb_sky_grad": (i32, [C.POINTER(GradDesc), vp]),
-    "pycwb_td_vectors": (i32, [vp, i64, vp, vp, i64, i32, vp, vp, i32, i32, i32, vp, i64, i32,
-                               vp]),
+    "pycwb_td_vectors": (i32, [vp, i64, vp, vp, i64, i32, vp, vp, vp, i32, i32, i32, vp, i64,
+                               i32, vp]),
 }
 
 

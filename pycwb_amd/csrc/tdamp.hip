@@ -28,6 +28,7 @@ struct TdArgs {
   int M;
   const double* T0;
   const double* Tx;
+  const double2* trig;  // (sin, cos) of q pi / (2J), q in [0, 4J)
   int nc;
   int J;
   int K;
@@ -36,54 +37,77 @@ struct TdArgs {
   int interleave;      // 0: [a00(-K..K), a90(-K..K)]; 1: (a00, a90) pairs
 };
 
-// even / odd filter sums of one layer; rows outside the planes read as zero
+// even / odd filter sums of one layer; rows outside the planes read as zero.
+// T points at the delay's entry of tap 0 in a tap-major table (stride D = 2J+1
+// between taps), so threads of consecutive delays read consecutive addresses.
 __device__ __forceinline__ void band_sums(const float* __restrict__ plane, int64_t n_rows, int M1,
-                                          int n, int m, const double* __restrict__ T, int W,
-                                          double& se, double& so) {
+                                          int n, int m, const double* __restrict__ T, int64_t D,
+                                          int W, double& se, double& so) {
   double e = 0.0, o = 0.0;
-  for (int k = 0; k < W; ++k) {
-    const int64_t row = (int64_t)n + k;
-    const double v = (row >= 0 && row < n_rows) ? (double)plane[row * M1 + m] : 0.0;
-    const double t = v * T[k];
-    if (k & 1) o += t;
-    else e += t;
+  const float* col = plane + m;
+  // pairs of taps (even, odd) so the sums keep their sequential order; loads
+  // of several pairs are in flight at once
+  int k = 0;
+#pragma unroll 4
+  for (; k + 1 < W; k += 2) {
+    const int64_t r0 = (int64_t)n + k, r1 = r0 + 1;
+    const double v0 = (r0 >= 0 && r0 < n_rows) ? (double)col[r0 * M1] : 0.0;
+    const double v1 = (r1 >= 0 && r1 < n_rows) ? (double)col[r1 * M1] : 0.0;
+    e += v0 * T[k * D];
+    o += v1 * T[(k + 1) * D];
+  }
+  if (k < W) {
+    const int64_t r0 = (int64_t)n + k;
+    const double v0 = (r0 >= 0 && r0 < n_rows) ? (double)col[r0 * M1] : 0.0;
+    e += v0 * T[k * D];
   }
   se = e;
   so = o;
 }
 
-__device__ double pixel_amp(const TdArgs& a, int n, int m, int dT, const float* __restrict__ plane,
-                            bool quad) {
+// sin and cos of q pi / (2J) for q in [0, 4J): every phase of the filter sums
+// is such a multiple (m pi dt / M = 2 m r pi / 2J, (2m -+ 1) dt pi / 2M =
+// (2m -+ 1) r pi / 2J with dt = r M / J), as WDM<T>::setTDFilter tabulates
+// sinTD / cosTD (cwb-core/WDM.cc:688-699).
+__device__ __forceinline__ double2 phase(const double2* __restrict__ trig, int J4, int q) {
+  int r = q % J4;
+  r += r < 0 ? J4 : 0;
+  return trig[r];  // (sin, cos)
+}
+
+__device__ __forceinline__ double pixel_amp(const TdArgs& a, int n, int m, int dT,
+                                            const float* __restrict__ plane, bool quad) {
   const int M = a.M, M1 = M + 1, W = 2 * a.nc + 1;
   const bool odd_n = n & 1, odd_mn = (m + n) & 1;
   const bool cancel_even = odd_n ^ quad;
   const bool edge = m == 0 || m == M;
-  const double* t0 = a.T0 + (int64_t)(dT + a.J) * W;
-  const double* tx = a.Tx + (int64_t)(dT + a.J) * W;
-  const double dt = (double)dT * (double)M / (double)a.J;
+  const int64_t D = 2 * (int64_t)a.J + 1;
+  const double* t0 = a.T0 + (dT + a.J);
+  const double* tx = a.Tx + (dT + a.J);
   double se, so;
-  band_sums(plane, a.n_rows, M1, n, m, t0, W, se, so);
+  band_sums(plane, a.n_rows, M1, n, m, t0, D, W, se, so);
   if (edge && cancel_even) se = 0.0;
   if (edge && !cancel_even) so = 0.0;
-  const double ph = m * M_PI * dt / M;
-  double r = odd_n ? se * cos(ph) - so * sin(ph) : se * cos(ph) + so * sin(ph);
+  const int J4 = 4 * a.J;
+  const double2 ph = phase(a.trig, J4, 2 * m * dT);
+  double r = odd_n ? se * ph.y - so * ph.x : se * ph.y + so * ph.x;
   if (m > 0) {
     double le, lo;
-    band_sums(plane, a.n_rows, M1, n, m - 1, tx, W, le, lo);
+    band_sums(plane, a.n_rows, M1, n, m - 1, tx, D, W, le, lo);
     if (m == 1 && cancel_even) le = 0.0;
     if (m == 1 && !cancel_even) lo = 0.0;
-    const double lp = (2.0 * m - 1.0) * dt * M_PI / (2.0 * M);
-    double t = odd_n ? (le - lo) * sin(lp) : (le + lo) * sin(lp);
+    const double slp = phase(a.trig, J4, (2 * m - 1) * dT).x;
+    double t = odd_n ? (le - lo) * slp : (le + lo) * slp;
     if (m == 1 || m == M) t *= 1.4142135623730951;
     r = odd_mn ? r - t : r + t;
   }
   if (m < M) {
     double he, ho;
-    band_sums(plane, a.n_rows, M1, n, m + 1, tx, W, he, ho);
+    band_sums(plane, a.n_rows, M1, n, m + 1, tx, D, W, he, ho);
     if (m == M - 1 && cancel_even) he = 0.0;
     if (m == M - 1 && !cancel_even) ho = 0.0;
-    const double hp = (2.0 * m + 1.0) * dt * M_PI / (2.0 * M);
-    double t = odd_n ? (he + ho) * sin(hp) : (he - ho) * sin(hp);
+    const double shp = phase(a.trig, J4, (2 * m + 1) * dT).x;
+    double t = odd_n ? (he + ho) * shp : (he - ho) * shp;
     if (m == 0 || m == M - 1) t *= 1.4142135623730951;
     r = odd_mn ? r - t : r + t;
   }
@@ -97,24 +121,20 @@ __global__ __launch_bounds__(256) void k_td_vectors(TdArgs a) {
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * 256) {
     const int64_t p = e / half;
-    const int ki = (int)(e % half);
+    const int ki = (int)(e - p * half);
     const int idx = a.pix[p];
-    const int n = idx / (a.M + 1), m = idx % (a.M + 1);
+    const int n = idx / (a.M + 1), m = idx - n * (a.M + 1);
     const int dT = ki - a.K;
     const int sh = dT >= 0 ? dT / a.J : -((-dT) / a.J);
     const int sub = dT - sh * a.J, ne = n - sh;
-    double a00, a90;
-    if (sh % 2 != 0) {
-      if ((n + m) % 2 != 0) {
-        a00 = -pixel_amp(a, ne, m, sub, a.p90, true);
-        a90 = pixel_amp(a, ne, m, sub, a.p00, false);
-      } else {
-        a00 = pixel_amp(a, ne, m, sub, a.p90, true);
-        a90 = -pixel_amp(a, ne, m, sub, a.p00, false);
-      }
-    } else {
-      a00 = pixel_amp(a, ne, m, sub, a.p00, false);
-      a90 = pixel_amp(a, ne, m, sub, a.p90, true);
+    // odd whole-bin shifts swap the quadratures (getTDamp): a00 from the 90
+    // plane, a90 from the 00 plane, one of them negated by the (n + m) parity
+    const bool swap = (sh & 1) != 0;
+    double a00 = pixel_amp(a, ne, m, sub, swap ? a.p90 : a.p00, swap);
+    double a90 = pixel_amp(a, ne, m, sub, swap ? a.p00 : a.p90, !swap);
+    if (swap) {
+      if ((n + m) & 1) a00 = -a00;
+      else a90 = -a90;
     }
     float* o = a.out + p * a.row_stride;
     if (a.interleave) {
@@ -133,16 +153,17 @@ extern "C" {
 
 int pycwb_td_vectors(const int32_t* pixel_index, int64_t n_pixels, const float* padded00,
                      const float* padded90, int64_t n_rows, int M, const double* T0,
-                     const double* Tx, int n_coeffs, int J, int K, float* out, int64_t row_stride,
-                     int interleave, void* stream) {
+                     const double* Tx, const double* trig, int n_coeffs, int J, int K, float* out,
+                     int64_t row_stride, int interleave, void* stream) {
   if (n_pixels < 0 || M < 1 || n_coeffs < 0 || J < 1 || K < 0 || n_rows < 1) return PYCWB_ERR_ARG;
   if (n_pixels == 0) return PYCWB_OK;
-  if (!pixel_index || !padded00 || !padded90 || !T0 || !Tx || !out) return PYCWB_ERR_ARG;
+  if (!pixel_index || !padded00 || !padded90 || !T0 || !Tx || !trig || !out) return PYCWB_ERR_ARG;
   if (K > 64 * J || row_stride < (int64_t)(4 * K + 2)) return PYCWB_ERR_ARG;
   (void)hipGetLastError();
   TdArgs a;
   a.pix = pixel_index; a.n_pixels = n_pixels; a.p00 = padded00; a.p90 = padded90;
-  a.n_rows = n_rows; a.M = M; a.T0 = T0; a.Tx = Tx; a.nc = n_coeffs; a.J = J; a.K = K;
+  a.n_rows = n_rows; a.M = M; a.T0 = T0; a.Tx = Tx; a.trig = (const double2*)trig;
+  a.nc = n_coeffs; a.J = J; a.K = K;
   a.out = out; a.row_stride = row_stride; a.interleave = interleave ? 1 : 0;
   const int64_t total = n_pixels * (2 * K + 1);
   int64_t grid = (total + 255) / 256;

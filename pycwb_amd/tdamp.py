@@ -28,11 +28,12 @@ class TDBatchInputs:
     """Padded quadrature planes and TD filter tables on the device."""
     padded00: torch.Tensor   # (n_time + 2 n_coeffs, M+1) float32
     padded90: torch.Tensor
-    T0: torch.Tensor         # (2J+1, 2 n_coeffs+1) float64
+    T0: torch.Tensor         # tap-major (2 n_coeffs+1, 2J+1) float64: the reference's table, transposed
     Tx: torch.Tensor
     M: int
     n_coeffs: int
     J: int
+    trig: torch.Tensor = None  # (4J, 2) float64: sin, cos of q pi / (2J)
 
     @classmethod
     def from_map(cls, data: np.ndarray, td_filters, device=None) -> "TDBatchInputs":
@@ -44,9 +45,13 @@ class TDBatchInputs:
         p00 = np.pad(np.ascontiguousarray(data.real.T), pad).astype(np.float32)
         p90 = np.pad(np.ascontiguousarray(data.imag.T), pad).astype(np.float32)
         up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dt)
+        J = int(td_filters.max_delay)
+        q = np.arange(4 * J) * (np.pi / (2.0 * J))
         return cls(padded00=up(p00, torch.float32), padded90=up(p90, torch.float32),
-                   T0=up(td_filters.T0, torch.float64), Tx=up(td_filters.Tx, torch.float64),
-                   M=int(td_filters.M), n_coeffs=nc, J=int(td_filters.max_delay))
+                   T0=up(np.asarray(td_filters.T0).T, torch.float64),
+                   Tx=up(np.asarray(td_filters.Tx).T, torch.float64),
+                   M=int(td_filters.M), n_coeffs=nc, J=J,
+                   trig=up(np.stack([np.sin(q), np.cos(q)], axis=1), torch.float64))
 
     def _check_pixels(self, pixel_indices) -> torch.Tensor:
         dev = self.padded00.device
@@ -61,7 +66,8 @@ class TDBatchInputs:
     def _launch(self, pix, K, out, row_stride, interleave, stream=None):
         rc = lib().pycwb_td_vectors(ptr(pix), int(pix.numel()), ptr(self.padded00),
                                     ptr(self.padded90), int(self.padded00.shape[0]), self.M,
-                                    ptr(self.T0), ptr(self.Tx), self.n_coeffs, self.J, int(K),
+                                    ptr(self.T0), ptr(self.Tx), ptr(self.trig), self.n_coeffs,
+                                    self.J, int(K),
                                     ptr(out), int(row_stride), int(interleave),
                                     stream_ptr(stream))
         _lib.check(rc, "pycwb_td_vectors")
