@@ -752,16 +752,33 @@ __global__ __launch_bounds__(256) void k_batch_weights(BatchDev b) {
                           b.wstats + (int64_t)c * kWStats);
 }
 
-// Waves per SIMD the compiler must leave room for (caps VGPRs): measured on
-// MI355X, the regulator runs fastest at 5 and the scan at 4 (a few spilled
-// registers cost less than the lost occupancy).
-template <int PHASE>
+// Waves per SIMD the compiler must leave room for (caps VGPRs), per phase and
+// detector count (small: up to three detectors).  Measured on MI355X, the
+// regulator runs fastest at 5 and the scan at 4 for both three and four
+// detectors: at four detectors the scan spills about 20 VGPRs there, and
+// still beats the spill-free 3-wave build by 3 % (occupancy hides more gather
+// latency than the spill traffic costs).
+#ifndef PYCWB_OCC_REG_SMALL
+#define PYCWB_OCC_REG_SMALL 5
+#endif
+#ifndef PYCWB_OCC_REG_LARGE
+#define PYCWB_OCC_REG_LARGE 5
+#endif
+#ifndef PYCWB_OCC_SCAN_SMALL
+#define PYCWB_OCC_SCAN_SMALL 4
+#endif
+#ifndef PYCWB_OCC_SCAN_LARGE
+#define PYCWB_OCC_SCAN_LARGE 4
+#endif
+template <int PHASE, int N>
 struct PhaseOccupancy {
-  static constexpr int waves = PHASE == 0 ? 5 : (PHASE == 1 ? 4 : 8);
+  static constexpr int waves =
+      PHASE == 0 ? (N <= 3 ? PYCWB_OCC_REG_SMALL : PYCWB_OCC_REG_LARGE)
+                 : (PHASE == 1 ? (N <= 3 ? PYCWB_OCC_SCAN_SMALL : PYCWB_OCC_SCAN_LARGE) : 8);
 };
 
 template <int N, int BLOCK, int PHASE>
-__global__ __launch_bounds__(BLOCK, PhaseOccupancy<PHASE>::waves) void k_batch_persistent(
+__global__ __launch_bounds__(BLOCK, (PhaseOccupancy<PHASE, N>::waves)) void k_batch_persistent(
     BatchDev b, int first, int count,
                                                             unsigned* ticket) {
   __shared__ unsigned s_item;
