@@ -291,6 +291,62 @@ int pycwb_td_vectors(const int32_t* pixel_index, int64_t n_pixels, const float* 
                      const double* Tx, const double* trig, int n_coeffs, int J, int K,
                      float* out, int64_t row_stride, int interleave, void* stream);
 
+/* Sub-network cut of a batch of clusters: the loudest pixels of each
+ * supercluster (apply_subnet_cut, pycwb/modules/super_cluster_native/
+ * utils.py:640-755, via _sub_net_cut_prepared_packets, sub_net_cut.py:206-267).
+ * Cluster c uses the layout of pycwb_batch_desc: rms (n_pix[c], n) float32 at
+ * rms + rms_off[c]; td (n_pix[c], n, 2K[c]+1) {00,90} float pairs at
+ * td + 2*td_off[c]; network net[c]: FP/FX (n_sky, n) and ml (n, n_sky) at
+ * offset net_off[net], with |ml| <= K[c]; thresholds net_En[net] =
+ * float32(2 Acore^2 n) and net_Es[net] = float32(2 e2or).
+ * A call evaluates the clusters which[0 .. n_which) (n_which <= 65535), all
+ * with n_ifo detectors (1..8). */
+typedef struct pycwb_subnet_desc {
+  int n_which;
+  const int32_t* which;
+  int n_ifo;
+  int64_t n_sky;
+  const int32_t* n_pix;
+  const int32_t* net;
+  const int32_t* K;
+  const int64_t* rms_off;
+  const int64_t* td_off;     /* in float pairs */
+  const float* rms;
+  const float* td;
+  const int64_t* net_off;
+  const float* net_En;
+  const float* net_Es;
+  const float* FP;
+  const float* FX;
+  const int32_t* ml;
+  double subcut;             /* < 0 disables the sub-network pre-cut */
+  double* tiles;             /* scratch: n_which * pycwb_subnet_tiles(n_sky) * 8 doubles */
+  double* sky;               /* 8 doubles per cluster (indexed by cluster) */
+  /* pycwb_subnet_mra only: cross-talk CSR as in pycwb_stats_desc, scratch, output */
+  const int64_t* lk_off;
+  const int32_t* lookup;
+  const int64_t* xt_off;
+  const float* xt;
+  float* scratch;            /* pycwb_subnet_mra_scratch(n, n_pix[c]) floats at scr_off[c] */
+  const int64_t* scr_off;
+  double* mra;               /* 8 doubles per cluster (indexed by cluster) */
+} pycwb_subnet_desc;
+
+size_t pycwb_subnet_tiles(int64_t n_sky);
+size_t pycwb_subnet_mra_scratch(int n_ifo, int n_pix);
+
+/* Sky loop of the sub-network cut.  Replaces optimze_sky_loc_from_td
+ * (pycwb/modules/super_cluster_native/sub_net_cut.py:404-517).  Writes
+ * sky[8c .. 8c+7] = (l_max, stat, Em, Am, lm, Vm, suball, EE), the reference's
+ * return tuple (zeros when no direction has a positive statistic). */
+int pycwb_subnet_sky(const pycwb_subnet_desc* desc, void* stream);
+
+/* Multi-resolution statistics at l_max = sky[8c].  Replaces
+ * mra_statistics_from_td (sub_net_cut.py:659-724) with sse_MRA_ps (:746-825).
+ * Cross-talk rows must not repeat a neighbour.  Writes mra[8c .. 8c+7] =
+ * (submra, rHo, Eo, Lo, Ls, m, extracted components, pixels above En). */
+int pycwb_subnet_mra(const pycwb_subnet_desc* desc, void* stream);
+
 /* WDM forward transform t2w of n_series float64 series (row stride ldx), all
  * with the same filter.  Replaces WDM<T>::t2w (cwb-core/WDM.cc:1385-1534) as
  * called through wdm_wavelet's t2w at

@@ -1,9 +1,11 @@
 """ORACLE -- TEST INFRASTRUCTURE ONLY.
 
-ctypes binding of ``oracle/liboracle.so`` (cwb_oracle.c, wdm_oracle.c): the CPU
-restatement of PycWB's Numba likelihood sky scan and of cWB's WDM forward
-transform.  Only tests, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline
-leg use it, and only as the checker.
+ctypes binding of ``oracle/liboracle.so``: the CPU restatement of PycWB's Numba
+likelihood sky scan and best-sky statistics (cwb_oracle.c), cWB's WDM forward
+transform (wdm_oracle.c), the continuous-direction statistic (grad_oracle.c),
+the TD filters and amplitudes (td_oracle.c) and the sub-network cut
+(subnet_oracle.c).  Only tests, ``__graft_entry__.smoke()`` and bench.py's
+CPU-baseline leg use it, and only as the checker.
 """
 
 from __future__ import annotations
@@ -323,3 +325,83 @@ def td_vectors(pixel_indices, p00, p90, T0, Tx, M, n_coeffs, K, J):
     fn(_p(pix), pix.size, _p(p00), _p(p90), p00.shape[0], _p(T0), _p(Tx), M, n_coeffs, K, J,
        _p(out))
     return out
+
+
+# ---- sub-network cut (subnet_oracle.c) ----
+
+def _subnet_common(FP, FX, rms, td00, td90, ml):
+    FP, FX = _c(FP, np.float32), _c(FX, np.float32)
+    rms = _c(rms, np.float32)
+    td00, td90 = _c(td00, np.float32), _c(td90, np.float32)
+    ml = _c(ml, np.int32)
+    return FP, FX, rms, td00, td90, ml
+
+
+def sky_loc_from_td(n_ifo, n_pix, n_sky, FP, FX, rms, td00, td90, ml, En, e2or, subcut):
+    """optimze_sky_loc_from_td: (l_max, stat, Em, Am, lm, Vm, suball, EE).
+    FP/FX (S, n), rms (P, n), td00/td90 (nd, n, P), ml (n, S); En float32."""
+    lib = load()
+    fn = lib.subnet_oracle_sky_loc_from_td
+    fn.restype = None
+    fn.argtypes = [C.c_int] * 3 + [vp] * 5 + [C.c_int, vp, C.c_float, C.c_double, C.c_double,
+                                              vp]
+    FP, FX, rms, td00, td90, ml = _subnet_common(FP, FX, rms, td00, td90, ml)
+    out = np.zeros(8)
+    fn(n_ifo, n_pix, n_sky, _p(FP), _p(FX), _p(rms), _p(td00), _p(td90), td00.shape[0], _p(ml),
+       float(np.float32(En)), float(e2or), float(subcut), _p(out))
+    return _sky_tuple(out)
+
+
+def _sky_tuple(out):
+    return (int(out[0]), float(out[1]), float(out[2]), float(out[3]), int(out[4]), int(out[5]),
+            float(out[6]), float(out[7]))
+
+
+def sky_loc(n_ifo, n_pix, n_sky, FP, FX, rms, td00, td90, td_energy, ml, En, e2or, subcut):
+    """optimze_sky_loc (the variant with a per-detector energy table)."""
+    lib = load()
+    fn = lib.subnet_oracle_sky_loc
+    fn.restype = None
+    fn.argtypes = [C.c_int] * 3 + [vp] * 6 + [C.c_int, vp, C.c_float, C.c_double, C.c_double,
+                                              vp]
+    FP, FX, rms, td00, td90, ml = _subnet_common(FP, FX, rms, td00, td90, ml)
+    te = _c(td_energy, np.float32)
+    out = np.zeros(8)
+    fn(n_ifo, n_pix, n_sky, _p(FP), _p(FX), _p(rms), _p(td00), _p(td90), _p(te), td00.shape[0],
+       _p(ml), float(np.float32(En)), float(e2or), float(subcut), _p(out))
+    return _sky_tuple(out)
+
+
+def mra_from_td(n_ifo, n_pix, n_sky, FP, FX, rms, td00, td90, ml, En, e2or, xtalks, lookup,
+                l_max, components=False):
+    """mra_statistics_from_td: (submra, rHo, Eo, Lo, Ls, m); with components=True
+    also (amp, AMP, n_extracted)."""
+    lib = load()
+    fn = lib.subnet_oracle_mra_from_td
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int] * 3 + [vp] * 5 + [C.c_int, vp, C.c_float, C.c_double, vp, vp,
+                                              C.c_int, vp, vp, vp]
+    FP, FX, rms, td00, td90, ml = _subnet_common(FP, FX, rms, td00, td90, ml)
+    xt = _c(xtalks, np.float32).reshape(-1, 8)
+    lk = _c(lookup, np.int32).reshape(-1, 2)
+    out = np.zeros(6)
+    amp = np.zeros((n_ifo, n_pix), np.float32)
+    AMP = np.zeros((n_ifo, n_pix), np.float32)
+    k = fn(n_ifo, n_pix, n_sky, _p(FP), _p(FX), _p(rms), _p(td00), _p(td90), td00.shape[0],
+           _p(ml), float(np.float32(En)), float(e2or), _p(xt), _p(lk), int(l_max), _p(out),
+           _p(amp), _p(AMP))
+    res = (float(out[0]), float(out[1]), float(out[2]), float(out[3]), float(out[4]),
+           int(out[5]))
+    return (res, amp, AMP, k) if components else res
+
+
+def reduced_likelihood(Fp0, Fx0, rms, v00, v90):
+    """Lo = sum over pixels of sse_like_ps at the DPF of (Fp0, Fx0, rms); (m, n) inputs."""
+    lib = load()
+    fn = lib.subnet_oracle_reduced_likelihood
+    fn.restype = C.c_float
+    fn.argtypes = [vp] * 5 + [C.c_int, C.c_int]
+    Fp0, Fx0 = _c(Fp0, np.float32), _c(Fx0, np.float32)
+    rms, v00, v90 = _c(rms, np.float32), _c(v00, np.float32), _c(v90, np.float32)
+    m, n = rms.shape
+    return float(fn(_p(Fp0), _p(Fx0), _p(rms), _p(v00), _p(v90), m, n))

@@ -95,6 +95,15 @@ class GradDesc(C.Structure):
                 ("grad_td", vp), ("grad_scale", vp)]
 
 
+class SubnetDesc(C.Structure):
+    _fields_ = [("n_which", C.c_int), ("which", vp), ("n_ifo", C.c_int), ("n_sky", C.c_int64),
+                ("n_pix", vp), ("net", vp), ("K", vp), ("rms_off", vp), ("td_off", vp),
+                ("rms", vp), ("td", vp), ("net_off", vp), ("net_En", vp), ("net_Es", vp),
+                ("FP", vp), ("FX", vp), ("ml", vp), ("subcut", C.c_double), ("tiles", vp),
+                ("sky", vp), ("lk_off", vp), ("lookup", vp), ("xt_off", vp), ("xt", vp),
+                ("scratch", vp), ("scr_off", vp), ("mra", vp)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -116,6 +125,10 @@ SIGNATURES = {
     "pycwb_sky_grad": (i32, [C.POINTER(GradDesc), vp]),
     "pycwb_td_vectors": (i32, [vp, i64, vp, vp, i64, i32, vp, vp, vp, i32, i32, i32, vp, i64,
                                i32, vp]),
+    "pycwb_subnet_tiles": (sz, [i64]),
+    "pycwb_subnet_mra_scratch": (sz, [i32, i32]),
+    "pycwb_subnet_sky": (i32, [C.POINTER(SubnetDesc), vp]),
+    "pycwb_subnet_mra": (i32, [C.POINTER(SubnetDesc), vp]),
 }
 
 
