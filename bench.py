@@ -13,8 +13,9 @@ stage) from HIP events on its stream, and the CPU baseline: the oracle's
 OpenMP restatement of PycWB's Numba scan on a bounded sample.  Side legs
 report under their own keys: "refinement" (512 three-detector clusters, grid
 argmax then 20 projected-gradient steps of the continuous sky statistic) and
-"wdm" (forward WDM transform of 5 x 600 s at 4096 Hz at M = 8 .. 512) and
-"tdamp" (TD-amplitude extraction of 200,000 pixels x 129 delays).
+"wdm" (forward WDM transform of 5 x 600 s at 4096 Hz at M = 8 .. 512),
+"tdamp" (TD-amplitude extraction of 200,000 pixels x 129 delays) and "subnet"
+(sub-network cut of 4,096 superclusters' 300 loudest pixels at order 4).
 
 Prints one JSON line.
 """
@@ -284,6 +285,101 @@ def tdamp_leg(stream, log, M=64, n_pixels=200000, K=64, nc=12):
     return res
 
 
+def subnet_leg(stream, log, n_clusters=4096, loud=300, order=4, K=64, cpu=True):
+    """Sub-network cut of the superclusters of a lag: the LOUD = 300 loudest
+    pixels of 4,096 clusters (sizes log-uniform[16, 4096], capped at 300), half
+    H1L1V1 and half H1L1V1K1, K = 64, HEALPix order 4 (3,072 directions),
+    Acore = sqrt(2), subcut 0.33, subnet 0.7.  Times the sky loop (one launch
+    per detector count) with HIP events, then the MRA statistics of the
+    clusters with suball > subnet (self-only cross-talk, CSR upload included).
+    The CPU baseline is the oracle's single-threaded sky loop (the reference's
+    loop is serial per cluster) on the first 32 clusters, whose GPU results must
+    match it bit for bit."""
+    import torch
+
+    from pycwb_amd import synth
+    from pycwb_amd.batch import synth_batch
+    from pycwb_amd.skystats import self_xtalk
+    from pycwb_amd.super_cluster_native.sub_net_cut import (SubnetBatch,
+                                                            calculate_e2or_from_acore, needs_mra,
+                                                            sky_tuple, subnet_sky_tables)
+
+    dev = torch.device("cuda", 0)
+    nets = [synth.make_network(synth.NETWORK_3, order=order, K=K),
+            synth.make_network(synth.NETWORK_4, order=order, K=K)]
+    sizes = np.minimum(synth.cluster_sizes(n_clusters, seed=2024), loud)
+    ids = np.arange(n_clusters) % 2
+    batch = synth_batch(nets, sizes, ids, seed=2024 * 1000, device=dev)
+    acore, subcut, subnet = math.sqrt(2.0), 0.33, 0.7
+    En = [np.float32(2 * acore * acore * nt.n_ifo) for nt in nets]
+    e2or = [calculate_e2or_from_acore(acore, nt.n_ifo) for nt in nets]
+    Es = [np.float32(2 * e) for e in e2or]
+    tables = subnet_sky_tables([(nt.FP_t, nt.FX_t, nt.ml) for nt in nets], En, Es, dev)
+    sb = SubnetBatch(batch, tables, subcut, dev)
+    S = nets[0].n_sky
+    evals = float(np.sum(sizes.astype(np.float64))) * S
+    keep = sb.launch_sky(stream)
+    torch.cuda.synchronize()
+    reps = 3
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record(stream)
+    for _ in range(reps):
+        keep = sb.launch_sky(stream)
+    e[1].record(stream)
+    torch.cuda.synchronize()
+    del keep
+    sky_ms = e[0].elapsed_time(e[1]) / reps
+    sky = sb.sky_d.view(-1, 8)[:n_clusters].cpu().numpy()
+    need = [c for c in range(n_clusters) if needs_mra(sky_tuple(sky[c]), subnet)]
+    csrs = [self_xtalk(int(sizes[c])) for c in need]
+    t0 = time.perf_counter()
+    h = sb.prepare_mra(need, csrs)
+    torch.cuda.synchronize()
+    prep_s = time.perf_counter() - t0
+    e[2].record(stream)
+    sb.launch_mra(h, stream)
+    e[3].record(stream)
+    torch.cuda.synchronize()
+    del h
+    mra_ms = e[2].elapsed_time(e[3])
+    nd = 2 * K + 1
+    td_bytes = float(np.sum(sizes.astype(np.float64) * batch.nifo * nd * 8))
+    res = {"clusters": n_clusters, "loud": loud, "healpix_order": order, "n_sky": S,
+           "pixels": int(sizes.sum()), "evals": evals, "sky_ms": round(sky_ms, 3),
+           "evals_per_s": evals / (sky_ms * 1e-3), "mra_clusters": len(need),
+           "mra_ms": round(mra_ms, 3), "mra_host_prep_ms": round(prep_s * 1e3, 3),
+           "td_bytes": td_bytes, "td_gbs": td_bytes / (sky_ms * 1e-3) / 1e9,
+           "frac_hbm": td_bytes / (sky_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+    log(f"subnet: sky loop {sky_ms:.2f} ms ({evals / (sky_ms * 1e-3):.3e} evals/s), "
+        f"MRA of {len(need)} clusters {mra_ms:.2f} ms (host CSR preparation "
+        f"{prep_s * 1e3:.1f} ms)")
+    if cpu:
+        from oracle import oracle as ora
+
+        sample = list(range(32))
+        cpu_s = 0.0
+        outs = []
+        for c in sample:
+            k, P = int(ids[c]), int(sizes[c])
+            td = batch.cluster_td(c).cpu().numpy()
+            rms = batch.cluster_rms(c).cpu().numpy()
+            td00 = np.ascontiguousarray(td[..., 0].transpose(2, 1, 0))
+            td90 = np.ascontiguousarray(td[..., 1].transpose(2, 1, 0))
+            t1 = time.perf_counter()
+            outs.append(ora.sky_loc_from_td(nets[k].n_ifo, P, S, nets[k].FP_t, nets[k].FX_t, rms,
+                                            td00, td90, nets[k].ml, En[k], e2or[k], subcut))
+            cpu_s += time.perf_counter() - t1
+        ev = float(np.sum(sizes[sample].astype(np.float64))) * S
+        res["cpu_baseline"] = {"value": ev / cpu_s, "unit": "evals/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"sky loop of clusters 0-31 ({int(ev):d} evals), "
+                                         "oracle, one thread"}
+        res["parity_sample_bit_exact"] = all(sky_tuple(sky[c]) == o for c, o in zip(sample, outs))
+        log(f"subnet cpu baseline {ev / cpu_s:.3e} evals/s (1 thread); parity "
+            f"{res['parity_sample_bit_exact']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,7 +399,8 @@ def main():
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-wdm", action="store_true")
     ap.add_argument("--no-tdamp", action="store_true")
-    ap.add_argument("--only", choices=("refine", "wdm", "tdamp"), default=None,
+    ap.add_argument("--no-subnet", action="store_true")
+    ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet"), default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first copy this many GiB device-to-device (counter calibration)")
@@ -341,6 +438,10 @@ def main():
         return 0
     if args.only == "tdamp":
         print(json.dumps({"tdamp": tdamp_leg(torch.cuda.current_stream(), log)}), flush=True)
+        return 0
+    if args.only == "subnet":
+        print(json.dumps({"subnet": subnet_leg(torch.cuda.current_stream(), log,
+                                               cpu=not args.no_cpu)}), flush=True)
         return 0
     t0 = time.perf_counter()
     nets = [synth.make_network(synth.NETWORK_3, order=args.order, K=args.K),
@@ -451,6 +552,7 @@ def main():
                                     stream, log)
     wdm = None if args.no_wdm else wdm_leg(stream, log)
     tdamp = None if args.no_tdamp else tdamp_leg(stream, log)
+    subnet = None if args.no_subnet else subnet_leg(stream, log, cpu=not args.no_cpu)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
@@ -490,6 +592,7 @@ def main():
         "refinement": refinement,
         "wdm": wdm,
         "tdamp": tdamp,
+        "subnet": subnet,
     }
     print(json.dumps(out), flush=True)
     bs.close()

@@ -330,6 +330,10 @@ typedef struct pycwb_subnet_desc {
   float* scratch;            /* pycwb_subnet_mra_scratch(n, n_pix[c]) floats at scr_off[c] */
   const int64_t* scr_off;
   double* mra;               /* 8 doubles per cluster (indexed by cluster) */
+  /* NULL, or per network a permutation of the n_sky directions ((n_networks,
+   * n_sky) int32 at n_sky * net): lanes take directions in that order (group
+   * directions with similar delays; results do not depend on it) */
+  const int32_t* sky_order;
 } pycwb_subnet_desc;
 
 size_t pycwb_subnet_tiles(int64_t n_sky);

@@ -101,7 +101,7 @@ class SubnetDesc(C.Structure):
                 ("rms", vp), ("td", vp), ("net_off", vp), ("net_En", vp), ("net_Es", vp),
                 ("FP", vp), ("FX", vp), ("ml", vp), ("subcut", C.c_double), ("tiles", vp),
                 ("sky", vp), ("lk_off", vp), ("lookup", vp), ("xt_off", vp), ("xt", vp),
-                ("scratch", vp), ("scr_off", vp), ("mra", vp)]
+                ("scratch", vp), ("scr_off", vp), ("mra", vp), ("sky_order", vp)]
 
 
 # name -> (restype, argtypes)

@@ -62,6 +62,7 @@ struct SubnetArgs {
   float* scratch;
   const int64_t* scr_off;
   double* mra;
+  const int32_t* order;
 };
 
 // sse_like_ps (sub_net_cut.py:728-742): each np.dot a forward float32 sum.
@@ -101,85 +102,136 @@ __device__ __forceinline__ bool rec_better(double a, int64_t la, double b, int64
   return a > b || (a == b && la < lb);
 }
 
+// Per-cluster constants and the direction's delays and antenna factors.
 template <int N>
-__device__ void sky_direction(const SubnetArgs& a, int c, int64_t l, SkyRec& r) {
-  const int P = a.npix[c];
-  const int net = a.net[c];
-  const int K = a.K[c];
-  const int nd = 2 * K + 1;
-  const float En = a.net_En[net], Es = a.net_Es[net];
-  const float2* td = a.td + a.td_off[c];
-  const float* rms = a.rms + a.rms_off[c];
-  const int64_t no = a.net_off[net];
+struct DirCtx {
+  int P, nd;
+  float En, Es;
+  const float2* td;
+  const float* rms;
   int d[N];
   float fp0[N], fx0[N];
+};
+
+template <int N>
+__device__ __forceinline__ void dir_ctx(const SubnetArgs& a, int c, int64_t l, DirCtx<N>& x) {
+  x.P = a.npix[c];
+  const int net = a.net[c];
+  const int K = a.K[c];
+  x.nd = 2 * K + 1;
+  x.En = a.net_En[net];
+  x.Es = a.net_Es[net];
+  x.td = a.td + a.td_off[c];
+  x.rms = a.rms + a.rms_off[c];
+  const int64_t no = a.net_off[net];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    d[i] = a.ml[no + (int64_t)i * a.S + l] + K;
-    fp0[i] = a.FP[no + l * N + i];
-    fx0[i] = a.FX[no + l * N + i];
+    x.d[i] = a.ml[no + (int64_t)i * a.S + l] + K;
+    x.fp0[i] = a.FP[no + l * N + i];
+    x.fx0[i] = a.FX[no + l * N + i];
   }
-  // pass 1 (sub_net_cut.py:431-464)
-  float m_gt = 0.f, Eo_gt = 0.f, Ls_gt = 0.f, Ln_gt = 0.f;
-  float Eo_ge = 0.f, Ls_ge = 0.f, Ln_ge = 0.f;
-  for (int j = 0; j < P; ++j) {
-    const float2* row = td + (int64_t)j * N * nd;
-    float rE = 0.f, em = 0.f;
+}
+
+// What pass 2 needs from pass 1 (sub_net_cut.py:466-486).
+struct Pass1 {
+  float aa, Eo, Ls, Ln;
+};
+
+// Pass-1 sums of one direction (sub_net_cut.py:432-438).
+struct Acc1 {
+  float m_gt, Eo_gt, Ls_gt, Ln_gt, Eo_ge, Ls_ge, Ln_ge;
+};
+
+// One pixel of pass 1 (sub_net_cut.py:439-464) from its per-detector
+// energies de[i] = v00^2 + v90^2 at the direction's delays.
+template <int N>
+__device__ __forceinline__ void acc1_pixel(Acc1& s, const float (&de)[N], float En, float Es) {
+  float rE = 0.f, em = 0.f;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const float2 v = row[i * nd + d[i]];
-      const float de = v.x * v.x + v.y * v.y;
-      rE += de;
-      if (de > em) em = de;
-    }
-    const float sub = rE - em;
-    if (rE >= En) {
-      Eo_ge += rE;
-      Ls_ge += sub;
-      if (sub > Es) Ln_ge += rE;
-    }
-    if (rE > En) {
-      m_gt += 1.f;
-      Eo_gt += rE;
-      Ls_gt += sub;
-      if (sub > Es) Ln_gt += rE;
-    }
+  for (int i = 0; i < N; ++i) {
+    rE += de[i];
+    if (de[i] > em) em = de[i];
   }
-  const float Eo_cut = Eo_gt + 0.01f;
-  const int64_t m_cut = (int64_t)(2.0 * (double)m_gt + 0.01);
-  const float aa = Ls_gt * Ln_gt / (Eo_cut - Ls_gt);
+  const float sub = rE - em;
+  if (rE >= En) {
+    s.Eo_ge += rE;
+    s.Ls_ge += sub;
+    if (sub > Es) s.Ln_ge += rE;
+  }
+  if (rE > En) {
+    s.m_gt += 1.f;
+    s.Eo_gt += rE;
+    s.Ls_gt += sub;
+    if (sub > Es) s.Ln_gt += rE;
+  }
+}
+
+// The end of pass 1 (sub_net_cut.py:466-490): the subcut pre-cut and the
+// Eo <= 0 skip.  True when the direction goes on to pass 2.
+__device__ __forceinline__ bool acc1_finish(const SubnetArgs& a, const Acc1& s, Pass1& q) {
+  const float Eo_cut = s.Eo_gt + 0.01f;
+  const int64_t m_cut = (int64_t)(2.0 * (double)s.m_gt + 0.01);
+  const float aa = s.Ls_gt * s.Ln_gt / (Eo_cut - s.Ls_gt);
   if (a.subcut >= 0.0 &&
       ((double)aa - (double)m_cut) / ((double)aa + (double)m_cut + (double)1e-16f) < a.subcut)
-    return;
-  const float Eo = Eo_ge, Ls = Ls_ge, Ln = Ln_ge;
-  if (Eo <= 0.f) return;
-  // pass 2 (sub_net_cut.py:470-500): reduced pixels are those with E >= En
+    return false;
+  if (s.Eo_ge <= 0.f) return false;
+  q.aa = aa;
+  q.Eo = s.Eo_ge;
+  q.Ls = s.Ls_ge;
+  q.Ln = s.Ln_ge;
+  return true;
+}
+
+// Pass 1 with the lane's own gathers from the amplitude table.  (Staging a
+// block's pixel rows in LDS instead is slower: the 256 delay-ordered
+// directions of a block read a narrow delay range, and staging every delay
+// costs more than the gathers it saves.)
+template <int N>
+__device__ void pass1_global(const DirCtx<N>& x, Acc1& s) {
+  for (int j = 0; j < x.P; ++j) {
+    const float2* row = x.td + (int64_t)j * N * x.nd;
+    float de[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const float2 v = row[i * x.nd + x.d[i]];
+      de[i] = v.x * v.x + v.y * v.y;
+    }
+    acc1_pixel<N>(s, de, x.En, x.Es);
+  }
+}
+
+// Pass 2 (sub_net_cut.py:470-515): DPF likelihood over the pixels with
+// E >= En, then the direction's statistics.  r stays invalid unless AA > 0.
+template <int N>
+__device__ void sky_pass2(const DirCtx<N>& x, int64_t l, const Pass1& q, SkyRec& r) {
   int64_t m = 0;
   float Lo = 0.f;
-  for (int j = 0; j < P; ++j) {
-    const float2* row = td + (int64_t)j * N * nd;
+  for (int j = 0; j < x.P; ++j) {
+    const float2* row = x.td + (int64_t)j * N * x.nd;
     float v00[N], v90[N];
     float rE = 0.f;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      const float2 v = row[i * nd + d[i]];
+      const float2 v = row[i * x.nd + x.d[i]];
       v00[i] = v.x;
       v90[i] = v.y;
       rE += v.x * v.x + v.y * v.y;
     }
-    if (rE < En) continue;
+    if (rE < x.En) continue;
     m += 1;
     float rr[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) rr[i] = rms[(int64_t)j * N + i];
+    for (int i = 0; i < N; ++i) rr[i] = x.rms[(int64_t)j * N + i];
     Dpf<N> dp;
-    dpf_pixel<N>(rr, fp0, fx0, dp);
+    dpf_pixel<N>(rr, x.fp0, x.fx0, dp);
     Lo += sse_like<N>(dp.f, dp.F, v00, v90);
   }
-  // sub_net_cut.py:502-515
+  const float aa = q.aa, Eo = q.Eo, Ls = q.Ls, Ln = q.Ln;
   const double AA = (double)aa / ((double)(fabsf(aa) + fabsf(Eo - Lo)) +
                                   (double)(2 * m) * (double)(Eo - Ln) / (double)Eo);
   if (!(AA > 0.0)) return;
+  if (!rec_better(AA, l, r.AA, r.l)) return;
   const float ee = Ls * Eo / (Eo - Ls);
   const double emm = (double)fabsf(Eo - Lo) + (double)(2 * m);
   r.AA = AA;
@@ -214,19 +266,49 @@ __device__ __forceinline__ void store_rec(double* o, const SkyRec& r) {
   o[7] = 0.0;
 }
 
-// Grid (n_tiles, n_which): block b.y evaluates directions
-// [b.x * 256, b.x * 256 + 256) of cluster which[b.y] and writes its best record.
+// Grid (n_tiles, n_which): block b.y evaluates the directions at positions
+// [b.x * 256, b.x * 256 + 256) of its network's direction order (identity
+// without one) for cluster which[b.y] and writes its best record.
+// Pass 1 runs one direction per lane; the directions that survive the pre-cut
+// (typically a few per cent) are compacted into an LDS queue so that pass 2,
+// the costly DPF sweep, runs on dense lanes instead of a few lanes per wave.
 template <int N>
 __global__ __launch_bounds__(kSkyBlock) void k_subnet_sky(SubnetArgs a) {
   __shared__ double s_rec[kSkyBlock / 64][kRec];
+  __shared__ int s_ql[kSkyBlock];
+  __shared__ Pass1 s_qp[kSkyBlock];
+  __shared__ int s_count;
   const int slot = blockIdx.y;
   const int c = a.which[slot];
-  const int64_t l = (int64_t)blockIdx.x * kSkyBlock + threadIdx.x;
+  const int64_t pos = (int64_t)blockIdx.x * kSkyBlock + threadIdx.x;
+  const int32_t* order = a.order ? a.order + (int64_t)a.net[c] * a.S : nullptr;
+  if (threadIdx.x == 0) s_count = 0;
+  __syncthreads();
+  if (pos < a.S) {
+    const int64_t l = order ? (int64_t)order[pos] : pos;
+    DirCtx<N> x;
+    dir_ctx<N>(a, c, l, x);
+    Acc1 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    pass1_global<N>(x, acc);
+    Pass1 q;
+    if (acc1_finish(a, acc, q)) {
+      const int k = atomicAdd(&s_count, 1);
+      s_ql[k] = (int)l;
+      s_qp[k] = q;
+    }
+  }
+  __syncthreads();
+  const int count = s_count;
   SkyRec r;
   r.AA = -1.0;
   r.l = INT64_MAX;
   r.Em = 0.f; r.Am = 0.0; r.Vm = 0; r.suball = 0.0; r.EE = 0.0;
-  if (l < a.S) sky_direction<N>(a, c, l, r);
+  for (int k = threadIdx.x; k < count; k += kSkyBlock) {
+    const int64_t lk = s_ql[k];
+    DirCtx<N> x;
+    dir_ctx<N>(a, c, lk, x);
+    sky_pass2<N>(x, lk, s_qp[k], r);
+  }
   double bA = r.AA;
   int64_t bl = r.l;
   wave_best(bA, bl);
@@ -236,10 +318,13 @@ __global__ __launch_bounds__(kSkyBlock) void k_subnet_sky(SubnetArgs a) {
   if (bA > 0.0 && r.l == bl) store_rec(s_rec[wave], r);
   __syncthreads();
   if (threadIdx.x == 0) {
-    // waves cover increasing directions: strict > keeps the first on ties
     int best = -1;
-    for (int w = 0; w < kSkyBlock / 64; ++w)
-      if (s_rec[w][0] > 0.0 && (best < 0 || s_rec[w][0] > s_rec[best][0])) best = w;
+    for (int w = 0; w < kSkyBlock / 64; ++w) {
+      if (!(s_rec[w][0] > 0.0)) continue;
+      if (best < 0 || rec_better(s_rec[w][0], (int64_t)s_rec[w][1], s_rec[best][0],
+                                 (int64_t)s_rec[best][1]))
+        best = w;
+    }
     double* o = a.tiles + ((int64_t)slot * a.n_tiles + blockIdx.x) * kRec;
     if (best < 0) {
       o[0] = -1.0;
@@ -518,6 +603,7 @@ SubnetArgs args_of(const pycwb_subnet_desc* d) {
   a.tiles = d->tiles; a.sky = d->sky; a.lk_off = d->lk_off; a.lookup = d->lookup;
   a.xt_off = d->xt_off; a.xt = d->xt; a.scratch = d->scratch; a.scr_off = d->scr_off;
   a.mra = d->mra;
+  a.order = d->sky_order;
   return a;
 }
 
@@ -542,8 +628,7 @@ size_t pycwb_subnet_mra_scratch(int n_ifo, int n_pix) {
 int pycwb_subnet_sky(const pycwb_subnet_desc* d, void* stream) {
   if (!d || d->n_which < 0) return PYCWB_ERR_ARG;
   if (d->n_which == 0) return PYCWB_OK;
-  if (!common_ok(d) || !d->tiles || d->n_which > 65535 ||
-      pycwb_subnet_tiles(d->n_sky) > 0x7fffffffu)
+  if (!common_ok(d) || !d->tiles || d->n_which > 65535 || d->n_sky > 0x7fffffff)
     return PYCWB_ERR_ARG;
   (void)hipGetLastError();
   return dispatch<LaunchSky>(d->n_ifo, args_of(d), d->n_which, (hipStream_t)stream);

@@ -32,6 +32,7 @@ import numpy as np
 import torch
 
 from .. import _lib
+from .. import sky as sky_grid
 from ..batch import ClusterBatch
 from ..device import lib, ptr, require_gpu, stream_ptr
 
@@ -102,11 +103,14 @@ class SubnetSky:
     ml_host: list
     En: torch.Tensor      # float32 (n_net,): float32(2 Acore^2 n)
     Es: torch.Tensor      # float32 (n_net,): float32(2 e2or)
+    order: torch.Tensor = None   # int32 (n_net, S): directions sorted by delay tuple
 
 
-def subnet_sky_tables(tables, En, Es, device=None) -> SubnetSky:
+def subnet_sky_tables(tables, En, Es, device=None, ordered: bool = True) -> SubnetSky:
     """tables: list of (FP (S, n), FX (S, n), ml (n, S)) per network; En, Es:
-    per-network float32 thresholds."""
+    per-network float32 thresholds.  ordered: lanes visit directions sorted by
+    delay tuple (neighbouring lanes gather neighbouring taps); the results do
+    not depend on it."""
     device = device or require_gpu()
     S = int(np.asarray(tables[0][0]).shape[0])
     offs, fp, fx, ml, nifo, mlh = [], [], [], [], [], []
@@ -131,7 +135,10 @@ def subnet_sky_tables(tables, En, Es, device=None) -> SubnetSky:
                      FP=t(fp, torch.float32), FX=t(fx, torch.float32), ml=t(ml, torch.int32),
                      ml_host=mlh,
                      En=torch.tensor(np.asarray(En, np.float32), device=device),
-                     Es=torch.tensor(np.asarray(Es, np.float32), device=device))
+                     Es=torch.tensor(np.asarray(Es, np.float32), device=device),
+                     order=None if not ordered else torch.from_numpy(np.ascontiguousarray(
+                         np.stack([sky_grid.delay_order(m, np.arange(S)) for m in mlh]),
+                         np.int32)).to(device))
 
 
 def cluster_batch_from_host(rms_list, td_list, net_ids, device=None) -> ClusterBatch:
@@ -172,10 +179,15 @@ def _check_csr(lookup, xt, P):
     q = xt[:, 0]
     if xt.shape[0] and (q.min() < 0 or q.max() >= P or (q != np.floor(q)).any()):
         raise ValueError("cross-talk neighbour index out of range")
-    rows = np.repeat(np.arange(P), (e - s).astype(np.int64))
-    cols = np.concatenate([q[a:b] for a, b in zip(s, e)]).astype(np.int64) if P else q
-    if rows.size and np.unique(rows * P + cols).size != rows.size:
-        raise ValueError("a cross-talk row lists a neighbour twice")
+    counts = (e - s).astype(np.int64)
+    rows = np.repeat(np.arange(P, dtype=np.int64), counts)
+    if rows.size:
+        first = np.cumsum(counts) - counts
+        entry = np.arange(rows.size, dtype=np.int64) - np.repeat(first, counts) + \
+            np.repeat(s.astype(np.int64), counts)
+        cols = q[entry].astype(np.int64)
+        if np.unique(rows * P + cols).size != rows.size:
+            raise ValueError("a cross-talk row lists a neighbour twice")
     return lookup, xt
 
 
@@ -216,6 +228,7 @@ class SubnetBatch:
         desc.FP, desc.FX, desc.ml = ptr(s.FP), ptr(s.FX), ptr(s.ml)
         desc.subcut = self.subcut
         desc.sky, desc.mra = ptr(self.sky_d), ptr(self.mra_d)
+        desc.sky_order = ptr(s.order)
         return desc
 
     def launch_sky(self, stream=None) -> list:
@@ -244,16 +257,15 @@ class SubnetBatch:
         del keep
         return self.sky_d.view(-1, 8)[:self.batch.n_clusters].cpu().numpy()
 
-    def mra_results(self, clusters, csrs, stream=None) -> np.ndarray:
-        """Multi-resolution statistics of the given clusters at their sky-loop
-        l_max (run sky_results first); csrs[k] = (lookup, xtalk) of clusters[k].
-        Returns (len(clusters), 8) float64."""
+    def prepare_mra(self, clusters, csrs) -> dict:
+        """Validate and upload the cross-talk CSRs of the given clusters
+        (csrs[k] = (lookup, xtalk) of clusters[k]); the returned handle feeds
+        launch_mra."""
         clusters = np.asarray(clusters, np.int64).reshape(-1)
-        if clusters.size == 0:
-            return np.zeros((0, 8))
         b, d = self.batch, self.device
-        lk_parts, xt_parts, lk_off, xt_off = [], [], np.zeros(b.n_clusters, np.int64), \
-            np.zeros(b.n_clusters, np.int64)
+        lk_parts, xt_parts = [], []
+        lk_off = np.zeros(b.n_clusters, np.int64)
+        xt_off = np.zeros(b.n_clusters, np.int64)
         scr_off = np.zeros(b.n_clusters, np.int64)
         lo = xo = so = 0
         for c, (lookup, xt) in zip(clusters, csrs):
@@ -265,29 +277,40 @@ class SubnetBatch:
             lo += P
             xo += xt.shape[0]
             so += int(lib().pycwb_subnet_mra_scratch(n, P))
-        lk_d = torch.from_numpy(np.concatenate(lk_parts + [np.zeros(2, np.int32)])).to(d)
-        xt_d = torch.from_numpy(np.concatenate(xt_parts + [np.zeros(8, np.float32)])).to(d)
-        lk_off_d = torch.from_numpy(lk_off).to(d)
-        xt_off_d = torch.from_numpy(xt_off).to(d)
-        scr_off_d = torch.from_numpy(scr_off).to(d)
-        scratch = torch.empty(max(so, 1), dtype=torch.float32, device=d)
-        stream = stream or torch.cuda.current_stream()
-        st = stream_ptr(stream)
-        keep = []
+        t = lambda a: torch.from_numpy(a).to(d)
         nifo = b.nifo[clusters]
-        for n in sorted(set(int(x) for x in nifo)):
-            part = clusters[nifo == n].astype(np.int32)
-            which = torch.from_numpy(part).to(d)
-            desc = self._desc(n, which, part.size)
-            desc.lk_off, desc.lookup, desc.xt_off, desc.xt = (ptr(lk_off_d), ptr(lk_d),
-                                                              ptr(xt_off_d), ptr(xt_d))
-            desc.scratch, desc.scr_off = ptr(scratch), ptr(scr_off_d)
+        groups = {n: t(clusters[nifo == n].astype(np.int32))
+                  for n in sorted(set(int(x) for x in nifo))}
+        return {"clusters": clusters, "groups": groups,
+                "lk": t(np.concatenate(lk_parts + [np.zeros(2, np.int32)])),
+                "xt": t(np.concatenate(xt_parts + [np.zeros(8, np.float32)])),
+                "lk_off": t(lk_off), "xt_off": t(xt_off), "scr_off": t(scr_off),
+                "scratch": torch.empty(max(so, 1), dtype=torch.float32, device=d)}
+
+    def launch_mra(self, h: dict, stream=None) -> None:
+        """Enqueue the MRA statistics of a prepared handle (keep the handle alive
+        until the stream has run it)."""
+        st = stream_ptr(stream)
+        for n, which in h["groups"].items():
+            desc = self._desc(n, which, int(which.numel()))
+            desc.lk_off, desc.lookup = ptr(h["lk_off"]), ptr(h["lk"])
+            desc.xt_off, desc.xt = ptr(h["xt_off"]), ptr(h["xt"])
+            desc.scratch, desc.scr_off = ptr(h["scratch"]), ptr(h["scr_off"])
             _lib.check(lib().pycwb_subnet_mra(C.byref(desc), st), "pycwb_subnet_mra")
-            keep.append(which)
+
+    def mra_results(self, clusters, csrs, stream=None) -> np.ndarray:
+        """Multi-resolution statistics of the given clusters at their sky-loop
+        l_max (run sky_results first); csrs[k] = (lookup, xtalk) of clusters[k].
+        Returns (len(clusters), 8) float64."""
+        clusters = np.asarray(clusters, np.int64).reshape(-1)
+        if clusters.size == 0:
+            return np.zeros((0, 8))
+        h = self.prepare_mra(clusters, csrs)
+        stream = stream or torch.cuda.current_stream()
+        self.launch_mra(h, stream)
         stream.synchronize()
-        del keep
-        out = self.mra_d.view(-1, 8).cpu().numpy()
-        return out[clusters]
+        del h
+        return self.mra_d.view(-1, 8).cpu().numpy()[clusters]
 
 
 def sky_tuple(row) -> tuple:
