@@ -351,6 +351,54 @@ int pycwb_subnet_sky(const pycwb_subnet_desc* desc, void* stream);
  * (submra, rHo, Eo, Lo, Ls, m, extracted components, pixels above En). */
 int pycwb_subnet_mra(const pycwb_subnet_desc* desc, void* stream);
 
+/* Cross-talk CSR of a batch of clusters from the WDM cross-talk catalog.
+ * Replaces getXTalk_pixels_fast (pycwb/modules/xtalk/monster.py:240-440) as
+ * called through XTalk.get_xtalk_pixels / get_xtalk_pixels_from_arrays
+ * (pycwb/modules/xtalk/type.py:34-63).  Cluster c owns pixels
+ * [pix_off[c], pix_off[c+1]) with nlayer = layers + 1 and time = WDM map index
+ * (time bin * nlayer + frequency).  Catalog (load_catalog, monster.py:63-153):
+ * layers (n_res) int32 > 0, table (n_res, n_res, max_layer+1, 2, 2) int32
+ * [start, end) ranges into coeff (n_coeff, 5) float32.
+ * Workspace: hash_keys (int64) and hash_vals (int32), n_slots entries, cluster
+ * c owning [hash_off[c], hash_off[c+1]) -- a power of two >= 2 * its pixels;
+ * counts (n_pixels int32), row_off (n_pixels + 1 int64), block_sums
+ * (pycwb_xtalk_scan_blocks(n_pixels) int64), sort_keys (xt_capacity int64).
+ * pycwb_xtalk_count fills row_off and xt_off (n_clusters + 1 int64: each
+ * cluster's first entry; xt_off[n_clusters] = total entries); the host reads
+ * the total, sizes xt, and pycwb_xtalk_fill writes xt (total, 8) float32 rows
+ * in the reference's order and lookup (n_pixels, 2) int32 [start, end)
+ * relative to the cluster's first entry -- each cluster's slices equal the
+ * reference's (lookup, xtalk) pair. */
+typedef struct pycwb_xtalk_desc {
+  int n_clusters;
+  int64_t n_pixels;
+  int64_t n_slots;
+  const int64_t* pix_off;
+  const int32_t* nlayer;
+  const int64_t* time;
+  const int64_t* hash_off;
+  int n_res;
+  int max_layer;
+  const int32_t* layers;
+  const int32_t* table;
+  const float* coeff;
+  int64_t n_coeff;
+  int64_t* hash_keys;
+  int32_t* hash_vals;
+  int32_t* counts;
+  int64_t* row_off;
+  int64_t* sort_keys;
+  int64_t* block_sums;
+  int32_t* lookup;
+  int64_t* xt_off;
+  float* xt;
+  int64_t xt_capacity;
+} pycwb_xtalk_desc;
+
+size_t pycwb_xtalk_scan_blocks(int64_t n_pixels);
+int pycwb_xtalk_count(const pycwb_xtalk_desc* desc, void* stream);
+int pycwb_xtalk_fill(const pycwb_xtalk_desc* desc, void* stream);
+
 /* WDM forward transform t2w of n_series float64 series (row stride ldx), all
  * with the same filter.  Replaces WDM<T>::t2w (cwb-core/WDM.cc:1385-1534) as
  * called through wdm_wavelet's t2w at

@@ -405,3 +405,63 @@ def reduced_likelihood(Fp0, Fx0, rms, v00, v90):
     rms, v00, v90 = _c(rms, np.float32), _c(v00, np.float32), _c(v90, np.float32)
     m, n = rms.shape
     return float(fn(_p(Fp0), _p(Fx0), _p(rms), _p(v00), _p(v90), m, n))
+
+
+# ---- cross-talk lookups (xtalk_oracle.c) ----
+
+def _catalog(layers, table, coeff):
+    layers = _c(layers, np.int32)
+    table = _c(table, np.int32)
+    coeff = _c(coeff, np.float32).reshape(-1, 5)
+    return layers, table, coeff, int(layers.size), int(table.shape[2] - 1)
+
+
+def xtalk_get(nlayer1, indx1, nlayer2, indx2, layers, table, coeff):
+    """getXTalk: (c1, c2, c3, c4) float32, 3s when the pair has no entry."""
+    lib = load()
+    fn = lib.xtalk_oracle_get
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int64, vp, C.c_int, C.c_int, vp, vp, vp]
+    layers, table, coeff, nres, maxl = _catalog(layers, table, coeff)
+    out = np.zeros(4, np.float32)
+    if fn(int(nlayer1), int(indx1), int(nlayer2), int(indx2), _p(layers), nres, maxl,
+          _p(table), _p(coeff), _p(out)) != 0:
+        raise ValueError("resolution not found in the catalog layers")
+    return out
+
+
+def _pixels_call(name, nlayer, time, layers, table, coeff, extra=()):
+    lib = load()
+    fn = getattr(lib, name)
+    fn.restype = C.c_int64
+    layers, table, coeff, nres, maxl = _catalog(layers, table, coeff)
+    nl = _c(nlayer, np.int32)
+    tm = _c(time, np.int64)
+    P = nl.size
+    lookup = np.zeros((P, 2), np.int32)
+    cap = max(1, 64 * P)
+    while True:
+        xt = np.zeros((cap, 8), np.float32)
+        args = [P, _p(nl), _p(tm), _p(layers), nres, maxl, _p(table), _p(coeff)]
+        types = [C.c_int, vp, vp, vp, C.c_int, C.c_int, vp, vp]
+        if extra:
+            args.append(coeff.shape[0])
+            types.append(C.c_int64)
+        fn.argtypes = types + [vp, vp, C.c_int64]
+        n = fn(*args, _p(lookup), _p(xt), cap)
+        if n >= 0:
+            return lookup, xt[:n].copy()
+        if cap > 4 * P * P + 64:
+            raise ValueError("resolution not found in the catalog layers")
+        cap *= 4
+
+
+def xtalk_pixels_pairwise(nlayer, time, layers, table, coeff):
+    """getXTalk_pixels_numba: (lookup (P, 2) int32, xt (m, 8) float32)."""
+    return _pixels_call("xtalk_oracle_pixels_pairwise", nlayer, time, layers, table, coeff)
+
+
+def xtalk_pixels_fast(nlayer, time, layers, table, coeff):
+    """getXTalk_pixels_fast: (lookup (P, 2) int32, xt (m, 8) float32)."""
+    return _pixels_call("xtalk_oracle_pixels_fast", nlayer, time, layers, table, coeff,
+                        extra=(True,))

@@ -104,6 +104,15 @@ class SubnetDesc(C.Structure):
                 ("scratch", vp), ("scr_off", vp), ("mra", vp), ("sky_order", vp)]
 
 
+class XtalkDesc(C.Structure):
+    _fields_ = [("n_clusters", C.c_int), ("n_pixels", C.c_int64), ("n_slots", C.c_int64),
+                ("pix_off", vp), ("nlayer", vp), ("time", vp), ("hash_off", vp),
+                ("n_res", C.c_int), ("max_layer", C.c_int), ("layers", vp), ("table", vp),
+                ("coeff", vp), ("n_coeff", C.c_int64), ("hash_keys", vp), ("hash_vals", vp),
+                ("counts", vp), ("row_off", vp), ("sort_keys", vp), ("block_sums", vp),
+                ("lookup", vp), ("xt_off", vp), ("xt", vp), ("xt_capacity", C.c_int64)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -129,6 +138,9 @@ SIGNATURES = {
     "pycwb_subnet_mra_scratch": (sz, [i32, i32]),
     "pycwb_subnet_sky": (i32, [C.POINTER(SubnetDesc), vp]),
     "pycwb_subnet_mra": (i32, [C.POINTER(SubnetDesc), vp]),
+    "pycwb_xtalk_scan_blocks": (sz, [i64]),
+    "pycwb_xtalk_count": (i32, [C.POINTER(XtalkDesc), vp]),
+    "pycwb_xtalk_fill": (i32, [C.POINTER(XtalkDesc), vp]),
 }
 
 

@@ -12,10 +12,11 @@ clusters of all lags as one batch: one hipGraph launch for the regulator, scan
 and argmax of every cluster, one launch for every cluster's best-sky
 statistics, and plain-kernel sky maps only for accepted clusters.
 
-Cross-talk: ``xtalk`` is any object with ``get_xtalk_pixels(pixel_arrays,
-check)`` returning the CSR ``(lookup, xtalk)`` of pycwb/modules/xtalk
-(``SelfXTalk`` gives the self-only CSR).  Loading a binary catalog from
-``MRAcatalog`` is not part of this package.
+Cross-talk: ``xtalk`` is a catalog (``pycwb_amd.xtalk.XTalk``, loaded from
+``MRAcatalog`` when ``xtalk`` is None; the batch builds every cluster's CSR in
+one GPU pass) or any object with ``get_xtalk_pixels(pixel_arrays, check)``
+returning the CSR ``(lookup, xtalk)`` of pycwb/modules/xtalk (``SelfXTalk``
+gives the self-only CSR).
 
 Not computed here (waveform-domain quantities of populate_detection_statistics
 that need the MRA waveform reconstruction): net_cc, sky_cc, sub_net, like_net,
@@ -36,6 +37,7 @@ from ..batch import BatchedSkyScan, ClusterBatch, NetTablesDev
 from ..device import require_gpu
 from ..skystats import SkyStatistics as _GpuSkyStats
 from ..skystats import SCALARS, self_xtalk
+from ..xtalk import XTalk
 from .detection_statistics import get_likelihood_rejection_reason
 from .sky_mask import compute_sky_valid_indices, sky_valid_indices_for_cluster
 from .sky_scan import MAP_NAMES, PlainClusterScan, scan_flags
@@ -276,8 +278,7 @@ def evaluate_cluster_likelihood(nIFO, cluster, config, MRAcatalog=None, strains=
             raise ValueError("likelihood(): xtalk or MRAcatalog must be provided. For "
                              "multi-cluster / multi-lag use, call evaluate_fragment_clusters()"
                              " instead.")
-        raise NotImplementedError("loading an MRA cross-talk catalog is not part of this "
-                                  "package; pass xtalk= (e.g. SelfXTalk())")
+        xtalk = XTalk.load(MRAcatalog, dump=True)
     if setup is None:
         sc_ml = sc_fp = sc_fx = None
         if supercluster_setup is not None:
@@ -367,8 +368,7 @@ def evaluate_fragment_clusters(config, fragment_clusters, strains, MRAcatalog, n
     accepted in that lag, in cluster order.
     """
     if xtalk is None:
-        raise NotImplementedError("loading an MRA cross-talk catalog is not part of this "
-                                  "package; pass xtalk= (e.g. SelfXTalk())")
+        xtalk = XTalk.load(MRAcatalog, dump=True)
     t0 = time.perf_counter()
     nIFO = int(getattr(config, "nIFO"))
     setup = prepare_likelihood_inputs(config, strains, nIFO)
@@ -416,7 +416,11 @@ def _evaluate_batch(config, setup, items, xtalk, nIFO, dev) -> dict:
         td_parts.append(packed.reshape(-1))
         n_pix.append(len(pa))
         Ks.append(nd // 2)
-        xts.append(xtalk.get_xtalk_pixels(pa, True))
+        if not hasattr(xtalk, "build_batch"):
+            xts.append(xtalk.get_xtalk_pixels(pa, True))
+    if hasattr(xtalk, "build_batch"):   # a catalog: every cluster's CSR in one GPU pass
+        xts = xtalk.build_batch([(cl.pixel_arrays.layers, cl.pixel_arrays.time)
+                                 for _, _, cl in items], device=dev)
     if int(np.abs(ml_h).max()) > min(Ks):
         raise ValueError("|ml| exceeds the TD half-width of a cluster")
     n_pix = np.asarray(n_pix, np.int64)

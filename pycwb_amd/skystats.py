@@ -57,8 +57,18 @@ class SkyStatistics:
         self.s64_off = np.concatenate([[0], np.cumsum(s64)[:-1]]).astype(np.int64)
         self.s32 = torch.zeros(int(s32.sum()), dtype=torch.float32, device=dev)
         self.s64 = torch.zeros(int(s64.sum()), dtype=torch.float64, device=dev)
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=dt)
+        if hasattr(xtalk, "pix_off") and hasattr(xtalk, "xt_off"):
+            # device-resident CSRs of the whole batch (pycwb_amd.xtalk.XTalkCSR)
+            if xtalk.pix_off.size != Cn + 1 or np.any(np.diff(xtalk.pix_off) != P):
+                raise ValueError("cross-talk CSR does not match the batch's clusters")
+            self.lk_off = xtalk.pix_off[:-1].astype(np.int64)
+            self.xt_off = xtalk.xt_off[:-1].astype(np.int64)
+            self.d_lookup, self.d_xt = xtalk.lookup, xtalk.xt
         if xtalk is None:
             lks, xts = zip(*(self_xtalk(int(p)) for p in P))
+        elif hasattr(xtalk, "pix_off"):
+            lks = xts = None
         else:
             lks, xts = zip(*xtalk)
             for c, (lk, xt) in enumerate(xtalk):
@@ -70,15 +80,15 @@ class SkyStatistics:
                     raise ValueError(f"cluster {c}: xtalk lookup rows out of range")
                 if len(xt) and (xt[:, 0].min() < 0 or xt[:, 0].max() >= P[c]):
                     raise ValueError(f"cluster {c}: xtalk neighbour index out of range")
-        lk_len = np.array([len(a) for a in lks], dtype=np.int64)
-        xt_len = np.array([len(a) for a in xts], dtype=np.int64)
-        self.lk_off = np.concatenate([[0], np.cumsum(lk_len)[:-1]]).astype(np.int64)
-        self.xt_off = np.concatenate([[0], np.cumsum(xt_len)[:-1]]).astype(np.int64)
-        lk_all = np.concatenate([np.asarray(a, np.int32).reshape(-1, 2) for a in lks])
-        xt_all = np.concatenate([np.asarray(a, np.float32).reshape(-1, 8) for a in xts]) \
-            if xt_len.sum() else np.zeros((1, 8), np.float32)
-        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=dt)
-        self.d_lookup, self.d_xt = t(lk_all, torch.int32), t(xt_all, torch.float32)
+        if lks is not None:
+            lk_len = np.array([len(a) for a in lks], dtype=np.int64)
+            xt_len = np.array([len(a) for a in xts], dtype=np.int64)
+            self.lk_off = np.concatenate([[0], np.cumsum(lk_len)[:-1]]).astype(np.int64)
+            self.xt_off = np.concatenate([[0], np.cumsum(xt_len)[:-1]]).astype(np.int64)
+            lk_all = np.concatenate([np.asarray(a, np.int32).reshape(-1, 2) for a in lks])
+            xt_all = np.concatenate([np.asarray(a, np.float32).reshape(-1, 8) for a in xts]) \
+                if xt_len.sum() else np.zeros((1, 8), np.float32)
+            self.d_lookup, self.d_xt = t(lk_all, torch.int32), t(xt_all, torch.float32)
         self.d_lk_off, self.d_xt_off = t(self.lk_off, torch.int64), t(self.xt_off, torch.int64)
         self.d_s32_off, self.d_s64_off = t(self.s32_off, torch.int64), t(self.s64_off, torch.int64)
         self.d_npix, self.d_nifo = t(batch.n_pix, torch.int32), t(batch.nifo, torch.int32)
