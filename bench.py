@@ -14,8 +14,9 @@ OpenMP restatement of PycWB's Numba scan on a bounded sample.  Side legs
 report under their own keys: "refinement" (512 three-detector clusters, grid
 argmax then 20 projected-gradient steps of the continuous sky statistic) and
 "wdm" (forward WDM transform of 5 x 600 s at 4096 Hz at M = 8 .. 512),
-"tdamp" (TD-amplitude extraction of 200,000 pixels x 129 delays) and "subnet"
-(sub-network cut of 4,096 superclusters' 300 loudest pixels at order 4).
+"tdamp" (TD-amplitude extraction of 200,000 pixels x 129 delays), "subnet"
+(sub-network cut of 4,096 superclusters' 300 loudest pixels at order 4) and
+"xtalk" (cross-talk CSRs of 4,096 clusters from a 7-resolution catalog).
 
 Prints one JSON line.
 """
@@ -380,6 +381,62 @@ def subnet_leg(stream, log, n_clusters=4096, loud=300, order=4, K=64, cpu=True):
     return res
 
 
+def xtalk_leg(stream, log, n_clusters=4096, cpu=True):
+    """Cross-talk CSRs of 4,096 clusters (sizes log-uniform[16, 4096], pixels in
+    compact time-frequency blocks over 7 resolutions, 8..512 layers) from a
+    synthetic catalog with the real one's bucket structure.  Times
+    XTalk.build_batch end to end (uploads, hash, catalog walk, scan, host read
+    of the total, ordered fill); the CPU baseline is the oracle's restatement
+    of getXTalk_pixels_fast (one thread) on the first 64 clusters, whose CSRs
+    must match the GPU's bit for bit."""
+    import torch
+
+    from pycwb_amd import synth
+    from pycwb_amd.xtalk import synthetic_catalog
+
+    layers = [8, 16, 32, 64, 128, 256, 512]
+    cat = synthetic_catalog(layers, seed=7)
+    rng = np.random.default_rng(7)
+    sizes = synth.cluster_sizes(n_clusters, seed=7)
+    clusters = []
+    for P in sizes:
+        r = rng.integers(0, len(layers), size=P)
+        L = np.asarray(layers)[r]
+        t0 = int(rng.integers(100, 10000))
+        band = float(rng.uniform(0.1, 0.8))
+        ts = t0 * (512 // L) + rng.integers(0, 4 * (512 // L) + 1)
+        fs = np.minimum(L, (band * L).astype(np.int64) + rng.integers(0, L // 8 + 2))
+        clusters.append(((L + 1).astype(np.int32), (ts * (L + 1) + fs).astype(np.int64)))
+    cat.build_batch(clusters[:8], stream)             # warm (catalog upload)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    csr = cat.build_batch(clusters, stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3
+    n_pix = int(sizes.sum())
+    res = {"clusters": n_clusters, "pixels": n_pix, "entries": int(csr.xt_off[-1]),
+           "ms": round(ms, 3), "pixels_per_s": n_pix / (ms * 1e-3)}
+    log(f"xtalk: CSR of {n_pix} pixels ({res['entries']} entries) in {ms:.1f} ms")
+    if cpu:
+        from oracle import oracle as ora
+
+        sample = list(range(64))
+        t = time.perf_counter()
+        outs = [ora.xtalk_pixels_fast(clusters[c][0], clusters[c][1], cat.layers,
+                                      cat.lookup_table, cat.coeff) for c in sample]
+        cpu_s = time.perf_counter() - t
+        px = int(sizes[sample].sum())
+        ok = True
+        for c, (lk, xt) in zip(sample, outs):
+            g_lk, g_xt = csr.cluster(c)
+            ok &= bool(np.array_equal(g_lk, lk) and np.array_equal(g_xt, xt))
+        res["cpu_baseline"] = {"value": px / cpu_s, "unit": "pixels/s", "cores": 1,
+                               "kind": "port", "sample": f"clusters 0-63 ({px} pixels), oracle"}
+        res["parity_sample_bit_exact"] = ok
+        log(f"xtalk cpu baseline {px / cpu_s:.3e} pixels/s (1 thread); parity {ok}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -400,7 +457,9 @@ def main():
     ap.add_argument("--no-wdm", action="store_true")
     ap.add_argument("--no-tdamp", action="store_true")
     ap.add_argument("--no-subnet", action="store_true")
-    ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet"), default=None,
+    ap.add_argument("--no-xtalk", action="store_true")
+    ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk"),
+                    default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first copy this many GiB device-to-device (counter calibration)")
@@ -438,6 +497,10 @@ def main():
         return 0
     if args.only == "tdamp":
         print(json.dumps({"tdamp": tdamp_leg(torch.cuda.current_stream(), log)}), flush=True)
+        return 0
+    if args.only == "xtalk":
+        print(json.dumps({"xtalk": xtalk_leg(torch.cuda.current_stream(), log,
+                                             cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.only == "subnet":
         print(json.dumps({"subnet": subnet_leg(torch.cuda.current_stream(), log,
@@ -553,6 +616,7 @@ def main():
     wdm = None if args.no_wdm else wdm_leg(stream, log)
     tdamp = None if args.no_tdamp else tdamp_leg(stream, log)
     subnet = None if args.no_subnet else subnet_leg(stream, log, cpu=not args.no_cpu)
+    xtalk = None if args.no_xtalk else xtalk_leg(stream, log, cpu=not args.no_cpu)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
@@ -593,6 +657,7 @@ def main():
         "wdm": wdm,
         "tdamp": tdamp,
         "subnet": subnet,
+        "xtalk": xtalk,
     }
     print(json.dumps(out), flush=True)
     bs.close()
