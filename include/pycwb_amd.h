@@ -414,6 +414,19 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
                   int stride, const double* filter, int m_H, const double* twiddle,
                   int quadrature, double* out, void* stream);
 
+/* WDM inverse transforms w2t (quad = 0: from the 00 coefficients) and w2tQ
+ * (quad = 1: from the 90 coefficients) of n_series maps.  Replaces
+ * WDM<T>::w2t / w2tQ (cwb-core/WDM.cc:1536-1710) as called through
+ * wdm_wavelet's w2t / w2tQ (pycwb/modules/data_conditioning/whitening.py:110-111,
+ * PSD_correction.py:130-131, regression.py:801-806).
+ * map: (n_series, N, M+1) complex pairs (00, 90) -- pycwb_wdm_t2w's output
+ * layout; M a power of two, 2 <= M <= 1024; filter/twiddle as for t2w;
+ * work: (n_series, N, 2M) float64 scratch; out: (n_series, n_out) float64,
+ * the first n_out samples of the reconstruction. */
+int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const double* filter,
+                  int m_H, const double* twiddle, int quad, double* work, double* out,
+                  int64_t n_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -465,3 +465,19 @@ def xtalk_pixels_fast(nlayer, time, layers, table, coeff):
     """getXTalk_pixels_fast: (lookup (P, 2) int32, xt (m, 8) float32)."""
     return _pixels_call("xtalk_oracle_pixels_fast", nlayer, time, layers, table, coeff,
                         extra=(True,))
+
+
+def wdm_w2t(map_complex, M, filt, n_out, quad=False):
+    """WDM<T>::w2t (quad=False: the 00 coefficients, the real parts) or w2tQ
+    (quad=True: the 90 coefficients, the imaginary parts) of a complex
+    (M+1, N) map; returns n_out float64 samples."""
+    lib = load()
+    fn = lib.wdm_oracle_w2t
+    fn.restype = C.c_int
+    fn.argtypes = [vp, C.c_int64, C.c_int, vp, C.c_int, C.c_int, C.c_int64, vp]
+    mp = np.asarray(map_complex)
+    col = np.ascontiguousarray((mp.imag if quad else mp.real).T, dtype=np.float64)
+    filt = _c(filt, np.float64)
+    out = np.zeros(int(n_out))
+    fn(_p(col), col.shape[0], int(M), _p(filt), filt.size, 1 if quad else 0, int(n_out), _p(out))
+    return out

@@ -178,3 +178,70 @@ int64_t wdm_oracle_t2w(const double *x, int64_t L, int M, const double *filt, in
     free(ts); free(re); free(cs); free(sn);
     return N;
 }
+
+/* WDM<T>::w2t / w2tQ (cwb-core/WDM.cc:1536-1710): inverse transform of an
+ * (N, M+1) map (map: 00 coefficients for quad = 0, 90 coefficients for
+ * quad = 1), n_out output samples.  Per time bin the half spectrum is built
+ * with the reference's phase rules, the complex-to-real inverse transform of
+ * length 2M is a direct sum (textbook routine in place of FFTW's c2r,
+ * unnormalised: r[k] = X0 + (-1)^k XM + 2 sum_j Re(X_j e^{+i pi j k / M})),
+ * and the filter-weighted overlap-add runs over the bins in order. */
+int wdm_oracle_w2t(const double *map, int64_t N, int M, const double *filt, int m_H, int quad,
+                   int64_t n_out, double *out)
+{
+    int M2 = 2 * M;
+    int64_t nTS = (int64_t)M * N + 2 * (int64_t)m_H;
+    double *ts = calloc((size_t)nTS, sizeof(double));
+    double *tmp = ts + m_H;
+    double *reX = malloc(sizeof(double) * M2), *imX = malloc(sizeof(double) * M2);
+    double *r = malloc(sizeof(double) * M2);
+    double *cs = malloc(sizeof(double) * M2), *sn = malloc(sizeof(double) * M2);
+    for (int k = 0; k < M2; ++k) {
+        cs[k] = cos(M_PI * k / M);
+        sn[k] = sin(M_PI * k / M);
+    }
+    const double sqrt2 = sqrt(2.0);
+    for (int64_t n = 0; n < N; ++n) {
+        const double *col = map + n * (M + 1);
+        for (int j = 0; j < M2; ++j) reX[j] = imX[j] = 0;
+        if (!quad) {
+            for (int j = 1; j < M; ++j) {
+                if ((n + j) & 1) imX[j] = col[j] / sqrt2;
+                else if (j & 1) reX[j] = -col[j] / sqrt2;
+                else reX[j] = col[j] / sqrt2;
+            }
+            if ((n & 1) == 0) reX[0] = col[0];
+            if (((n + M) & 1) == 0) reX[M] = (M & 1) ? -col[M] : col[M];
+        } else {
+            for (int j = 1; j < M; ++j) {
+                if ((n + j) & 1) reX[j] = col[j] / sqrt2;
+                else if (j & 1) imX[j] = col[j] / sqrt2;
+                else imX[j] = -col[j] / sqrt2;
+            }
+            if ((n + M) & 1) reX[M] = col[M];
+            if (n & 1) reX[0] = col[0];
+        }
+        for (int k = 0; k < M2; ++k) {
+            double s = reX[0] + ((k & 1) ? -reX[M] : reX[M]);
+            for (int j = 1; j < M; ++j) {
+                int q = (int)(((int64_t)j * k) % M2);
+                s += 2.0 * (reX[j] * cs[q] - imX[j] * sn[q]);
+            }
+            r[k] = s;
+        }
+        double *x = tmp + n * M;
+        int m = M * (int)(n & 1), mm = m;
+        x[0] += filt[0] * r[m];
+        for (int j = 1; j < m_H; ++j) {
+            ++m;
+            if (m == M2) m = 0;
+            x[j] += filt[j] * r[m];
+            if (mm == 0) mm = M2 - 1;
+            else --mm;
+            x[-j] += filt[j] * r[mm];
+        }
+    }
+    for (int64_t n = 0; n < n_out; ++n) out[n] = n < nTS - m_H ? tmp[n] : 0.0;
+    free(ts); free(reX); free(imX); free(r); free(cs); free(sn);
+    return 0;
+}

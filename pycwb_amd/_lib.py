@@ -141,6 +141,7 @@ SIGNATURES = {
     "pycwb_xtalk_scan_blocks": (sz, [i64]),
     "pycwb_xtalk_count": (i32, [C.POINTER(XtalkDesc), vp]),
     "pycwb_xtalk_fill": (i32, [C.POINTER(XtalkDesc), vp]),
+    "pycwb_wdm_w2t": (i32, [vp, i64, i32, i32, vp, i32, vp, i32, vp, vp, i64, vp]),
 }
 
 

@@ -217,6 +217,108 @@ void launch_poly(dim3 grid, hipStream_t st, const double* x, int64_t L, int64_t 
                      x, L, ldx, n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
 }
 
+// Inverse transform w2t / w2tQ (cwb-core/WDM.cc:1536-1710), two kernels.
+// k_w2t_bins: per time bin n, the half spectrum from column n (00 or 90
+// coefficients, the reference's phase rules) and its complex-to-real inverse
+// of length 2M, r[k] = sum_j X_j e^{+i pi j k / M} (Hermitian extension),
+// computed as the forward FFT of the conjugate extension; two bins share one
+// complex FFT (bin 2p real, 2p+1 imaginary: both results are real).
+// k_w2t_gather: output sample t sums, over the bins n in ascending order
+// (the reference's overlap-add order), h[|t - nM|] r_n[(M (n & 1) + t - nM)
+// mod 2M] for |t - nM| < m_H.
+__global__ __launch_bounds__(kThreads) void k_w2t_bins(
+    const double2* __restrict__ map, int64_t N, int n_series, int M,
+    const double2* __restrict__ tw, int log2_2M, int B, int quad, double* __restrict__ work) {
+  extern __shared__ double2 buf[];  // (B / 2) * 2M complex
+  const int M2 = 2 * M;
+  const int64_t tiles = (N + B - 1) / B;
+  const int s = (int)(blockIdx.x / tiles);
+  const int64_t n0 = (blockIdx.x % tiles) * B;
+  if (s >= n_series) return;
+  const double r2 = 1.4142135623730951;
+  const int pairs = B / 2;
+  for (int e = threadIdx.x; e < pairs * M2; e += kThreads) {
+    const int p = e / M2, k = e % M2;
+    double wr[2], wi[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t n = n0 + 2 * p + h;
+      wr[h] = wi[h] = 0.0;
+      if (n >= N) continue;
+      const int j = k <= M ? k : M2 - k;          // spectrum index of this slot
+      const double2 cc = map[((int64_t)s * N + n) * (M + 1) + j];
+      const double c = quad ? cc.y : cc.x;
+      double re = 0.0, im = 0.0;
+      if (j == 0) {
+        if (quad ? (n & 1) : !(n & 1)) re = c;
+      } else if (j == M) {
+        if (quad ? ((n + M) & 1) : !((n + M) & 1)) re = (!quad && (M & 1)) ? -c : c;
+      } else if (!quad) {
+        if ((n + j) & 1) im = c / r2;
+        else if (j & 1) re = -c / r2;
+        else re = c / r2;
+      } else {
+        if ((n + j) & 1) re = c / r2;
+        else if (j & 1) im = c / r2;
+        else im = -c / r2;
+      }
+      // conjugate extension: W_j = conj X_j (j <= M), W_{2M-j} = X_j
+      wr[h] = re;
+      wi[h] = (k <= M) ? -im : im;
+      if (k == 0 || k == M) wi[h] = 0.0;
+    }
+    // pair: Z = W_a + i W_b
+    buf[p * M2 + bitrev((unsigned)k, log2_2M)] = make_double2(wr[0] - wi[1], wi[0] + wr[1]);
+  }
+  __syncthreads();
+  for (int h = 1; h < M2; h <<= 1) {
+    const int tstride = M2 / (2 * h);
+    for (int e = threadIdx.x; e < pairs * M; e += kThreads) {
+      const int p = e / M, r = e % M;
+      const int g = r / h, j = r % h;
+      const int i0 = p * M2 + g * 2 * h + j, i1 = i0 + h;
+      const double2 w = tw[j * tstride];
+      const double2 u = buf[i0], v = buf[i1];
+      const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
+      buf[i0] = make_double2(u.x + vr, u.y + vi);
+      buf[i1] = make_double2(u.x - vr, u.y - vi);
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < B * M2; e += kThreads) {
+    const int b = e / M2, k = e % M2;
+    const int64_t n = n0 + b;
+    if (n >= N) continue;
+    const double2 z = buf[(b >> 1) * M2 + k];
+    work[((int64_t)s * N + n) * M2 + k] = (b & 1) ? z.y : z.x;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_w2t_gather(
+    const double* __restrict__ work, int64_t N, int n_series, int M,
+    const double* __restrict__ filt, int m_H, int64_t n_out, double* __restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (g >= n_out * n_series) return;
+  const int s = (int)(g / n_out);
+  const int64_t t = g % n_out;
+  const int M2 = 2 * M;
+  int64_t lo = t - (m_H - 1);
+  int64_t n_lo = lo <= 0 ? 0 : (lo + M - 1) / M;
+  int64_t n_hi = (t + m_H - 1) / M;
+  if (n_hi > N - 1) n_hi = N - 1;
+  double acc = 0.0;
+  const double* w = work + (int64_t)s * N * M2;
+  for (int64_t n = n_lo; n <= n_hi; ++n) {
+    const int64_t j = t - n * M;
+    const int64_t aj = j < 0 ? -j : j;
+    if (aj >= m_H) continue;
+    int64_t idx = (M * (n & 1) + j) % M2;
+    if (idx < 0) idx += M2;
+    acc += filt[aj] * w[n * M2 + idx];
+  }
+  out[(int64_t)s * n_out + t] = acc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -273,6 +375,42 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     pycwb::record_error("k_t2w", hipGetErrorString(e));
+    return PYCWB_ERR_HIP;
+  }
+  return PYCWB_OK;
+}
+
+int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const double* filter,
+                  int m_H, const double* twiddle, int quad, double* work, double* out,
+                  int64_t n_out, void* stream) {
+  if (!map || !filter || !twiddle || !work || !out) return PYCWB_ERR_ARG;
+  if (n_series < 1 || M < 2 || (M & (M - 1)) || M > kPolyLdsMax / 2 || N < 1 || m_H < 1 ||
+      n_out < 1)
+    return PYCWB_ERR_ARG;
+  (void)hipGetLastError();
+  int log2_2M = 0;
+  while ((1 << log2_2M) < 2 * M) ++log2_2M;
+  int B = (kPolyLdsMax / M) & ~1;   // bins per block: (B / 2) * 2M complex = 32 KiB
+  if (B > 64) B = 64;
+  if (B < 2) B = 2;
+  const int64_t tiles = (N + B - 1) / B;
+  const int64_t grid = tiles * n_series;
+  if (grid > 0x7fffffff || n_out * n_series / kThreads > 0x7fffffff) return PYCWB_ERR_ARG;
+  const hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_w2t_bins, dim3((unsigned)grid), dim3(kThreads),
+                     (size_t)(B / 2) * 2 * M * sizeof(double2), st, (const double2*)map, N,
+                     n_series, M, (const double2*)twiddle, log2_2M, B, quad, work);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    pycwb::record_error("k_w2t_bins", hipGetErrorString(e));
+    return PYCWB_ERR_HIP;
+  }
+  const int64_t total = n_out * n_series;
+  hipLaunchKernelGGL(k_w2t_gather, dim3((unsigned)((total + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, work, N, n_series, M, filter, m_H, n_out, out);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    pycwb::record_error("k_w2t_gather", hipGetErrorString(e));
     return PYCWB_ERR_HIP;
   }
   return PYCWB_OK;

@@ -301,6 +301,53 @@ class WDM:
         _lib.check(rc, "pycwb_wdm_t2w")
         return torch.view_as_complex(out).transpose(1, 2)
 
+    def w2t_device(self, tf: torch.Tensor, n_out: int, quad: bool = False,
+                   stream=None) -> torch.Tensor:
+        """Inverse transform on the GPU (WDM.cc:1536-1710): tf complex128
+        (n_series, M+1, N) or (M+1, N) -- t2w_device's output -- to
+        (n_series, n_out) float64 from the 00 (quad=False, w2t) or the 90
+        (quad=True, w2tQ) coefficients."""
+        T = tf if tf.dim() == 3 else tf.unsqueeze(0)
+        S_, M1, N = T.shape
+        if M1 != self.M + 1:
+            raise ValueError(f"map has {M1} layers, the transform {self.M + 1}")
+        if self.M < 2:
+            raise ValueError("the inverse needs M >= 2")
+        n_out = int(n_out)
+        if not (n_out <= self.M * N < 2 * n_out):
+            raise ValueError("inverse is not defined for the up-sampled map "
+                             f"(M N = {self.M * N}, {n_out} samples)")
+        dev = T.device
+        bm = torch.view_as_real(T.to(torch.complex128).transpose(1, 2).contiguous())
+        filt, tw = self._device(dev)
+        work = torch.empty((S_, N, 2 * self.M), dtype=torch.float64, device=dev)
+        out = torch.empty((S_, n_out), dtype=torch.float64, device=dev)
+        rc = lib().pycwb_wdm_w2t(ptr(bm), N, S_, self.M, ptr(filt), self.m_H, ptr(tw),
+                                 1 if quad else 0, ptr(work), ptr(out), n_out,
+                                 stream_ptr(stream))
+        _lib.check(rc, "pycwb_wdm_w2t")
+        return out
+
+    def _inverse(self, tf_map, quad):
+        data = tf_map.data if hasattr(tf_map, "data") else tf_map
+        n_out = int(getattr(tf_map, "len_timeseries", 0) or 0)
+        dev = require_gpu()
+        T = data if isinstance(data, torch.Tensor) else \
+            torch.from_numpy(np.ascontiguousarray(np.asarray(data, np.complex128)))
+        T = T.to(dev)
+        if not n_out:
+            n_out = self.M * T.shape[-1]
+        return self.w2t_device(T, n_out, quad)[0].cpu().numpy()
+
+    def w2t(self, tf_map) -> np.ndarray:
+        """Time series from the 00 coefficients of a TimeFrequencyMap (or a
+        complex (M+1, N) array); len_timeseries samples (WDM.cc:1536-1630)."""
+        return self._inverse(tf_map, False)
+
+    def w2tQ(self, tf_map) -> np.ndarray:
+        """Time series from the 90 coefficients (WDM.cc:1635-1710)."""
+        return self._inverse(tf_map, True)
+
     def t2w(self, x, sample_rate: float = 1.0, t0: float = 0.0, MM: int = -1):
         """One series -> TimeFrequencyMap with complex (M+1, N) data."""
         dev = require_gpu()
