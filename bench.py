@@ -212,7 +212,9 @@ def refinement_leg(n_clusters, order, K, steps, stream, log):
 def wdm_leg(stream, log, seconds=600, rate=4096, n_det=5):
     """WDM t2w of n_det series of seconds*rate samples at 7 resolutions
     (M = 8 .. 512), HIP events per level; algorithmic bytes = input read once
-    plus the complex128 map written once."""
+    plus the complex128 map written once.  Then the inverse w2t of the same
+    maps (the same bytes, read and written the other way) and its round-trip
+    error away from the edges."""
     import torch
 
     from pycwb_amd.wdm import WDM
@@ -223,7 +225,7 @@ def wdm_leg(stream, log, seconds=600, rate=4096, n_det=5):
     g.manual_seed(7)
     x = torch.randn(n_det, L, generator=g, device=dev, dtype=torch.float64)
     levels = []
-    tot_ms, tot_bytes = 0.0, 0
+    tot_ms, tot_ims, tot_bytes = 0.0, 0.0, 0
     for M in (8, 16, 32, 64, 128, 256, 512):
         w = WDM(M, M, 6, 10)
         w.t2w_device(x)  # warm (filter upload)
@@ -238,14 +240,28 @@ def wdm_leg(stream, log, seconds=600, rate=4096, n_det=5):
         ms = e0.elapsed_time(e1) / reps
         N = y.shape[-1]
         nbytes = n_det * (L * 8 + N * (M + 1) * 16)
+        # inverse (00 phase) of the same maps
+        z = w.w2t_device(y, L, stream=stream)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            z = w.w2t_device(y, L, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ims = e0.elapsed_time(e1) / reps
+        err = float((z - x)[:, w.m_H:-w.m_H].abs().max() / x.abs().max())
         levels.append({"M": M, "m_H": int(w.m_H), "ms": round(ms, 4),
-                       "gbs": nbytes / (ms * 1e-3) / 1e9})
+                       "gbs": nbytes / (ms * 1e-3) / 1e9, "w2t_ms": round(ims, 4),
+                       "w2t_gbs": nbytes / (ims * 1e-3) / 1e9, "roundtrip_rel_err": err})
         tot_ms += ms
+        tot_ims += ims
         tot_bytes += nbytes
-        del y
+        del y, z
     res = {"series": n_det, "samples": L, "levels": levels, "ms_total": round(tot_ms, 3),
            "gbs": tot_bytes / (tot_ms * 1e-3) / 1e9,
-           "frac_hbm": tot_bytes / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+           "frac_hbm": tot_bytes / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+           "w2t_ms_total": round(tot_ims, 3),
+           "w2t_gbs": tot_bytes / (tot_ims * 1e-3) / 1e9}
     log(f"wdm: {res}")
     return res
 

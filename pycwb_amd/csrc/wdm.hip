@@ -295,25 +295,23 @@ __global__ __launch_bounds__(kThreads) void k_w2t_bins(
 }
 
 __global__ __launch_bounds__(kThreads) void k_w2t_gather(
-    const double* __restrict__ work, int64_t N, int n_series, int M,
+    const double* __restrict__ work, int64_t N, int M, int log2_M,
     const double* __restrict__ filt, int m_H, int64_t n_out, double* __restrict__ out) {
-  const int64_t g = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (g >= n_out * n_series) return;
-  const int s = (int)(g / n_out);
-  const int64_t t = g % n_out;
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= n_out) return;
+  const int s = blockIdx.y;
   const int M2 = 2 * M;
-  int64_t lo = t - (m_H - 1);
-  int64_t n_lo = lo <= 0 ? 0 : (lo + M - 1) / M;
-  int64_t n_hi = (t + m_H - 1) / M;
+  const int64_t lo = t - (m_H - 1);
+  const int64_t n_lo = lo <= 0 ? 0 : (lo + M - 1) >> log2_M;
+  int64_t n_hi = (t + m_H - 1) >> log2_M;
   if (n_hi > N - 1) n_hi = N - 1;
   double acc = 0.0;
   const double* w = work + (int64_t)s * N * M2;
   for (int64_t n = n_lo; n <= n_hi; ++n) {
-    const int64_t j = t - n * M;
+    const int64_t j = t - (n << log2_M);
     const int64_t aj = j < 0 ? -j : j;
     if (aj >= m_H) continue;
-    int64_t idx = (M * (n & 1) + j) % M2;
-    if (idx < 0) idx += M2;
+    const int idx = (int)((M * (n & 1) + j) & (M2 - 1));   // mod 2M, also for j < 0
     acc += filt[aj] * w[n * M2 + idx];
   }
   out[(int64_t)s * n_out + t] = acc;
@@ -395,7 +393,8 @@ int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const doubl
   if (B < 2) B = 2;
   const int64_t tiles = (N + B - 1) / B;
   const int64_t grid = tiles * n_series;
-  if (grid > 0x7fffffff || n_out * n_series / kThreads > 0x7fffffff) return PYCWB_ERR_ARG;
+  if (grid > 0x7fffffff || n_out / kThreads > 0x7fffffff || n_series > 65535)
+    return PYCWB_ERR_ARG;
   const hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_w2t_bins, dim3((unsigned)grid), dim3(kThreads),
                      (size_t)(B / 2) * 2 * M * sizeof(double2), st, (const double2*)map, N,
@@ -405,9 +404,9 @@ int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const doubl
     pycwb::record_error("k_w2t_bins", hipGetErrorString(e));
     return PYCWB_ERR_HIP;
   }
-  const int64_t total = n_out * n_series;
-  hipLaunchKernelGGL(k_w2t_gather, dim3((unsigned)((total + kThreads - 1) / kThreads)),
-                     dim3(kThreads), 0, st, work, N, n_series, M, filter, m_H, n_out, out);
+  hipLaunchKernelGGL(k_w2t_gather,
+                     dim3((unsigned)((n_out + kThreads - 1) / kThreads), (unsigned)n_series),
+                     dim3(kThreads), 0, st, work, N, M, log2_2M - 1, filter, m_H, n_out, out);
   e = hipGetLastError();
   if (e != hipSuccess) {
     pycwb::record_error("k_w2t_gather", hipGetErrorString(e));
