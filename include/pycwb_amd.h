@@ -427,6 +427,51 @@ int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const doubl
                   int m_H, const double* twiddle, int quad, double* work, double* out,
                   int64_t n_out, void* stream);
 
+/* Network-pixel selection of n_lag lags at once.  Replaces
+ * select_network_pixels (pycwb/modules/coherence_native/selection.py:144-243)
+ * with _align_threshold_map_numba, _candidate_passes_support_numba and
+ * _select_candidates_numba (coherence_native/kernels.py:107-137,231-368),
+ * called once per lag by coherence_single_lag (pipeline.py:81-172).
+ * energy: (n_ifo, n_freq, n_time) float64 per-detector pixel energies;
+ * shift_bins: (n_lag, n_ifo) circular time shifts (>= 0) within the valid
+ * range [edge_bins, n_time - edge_bins); veto: (n_time) int16 (1 keep) or
+ * NULL; ib >= 1 / ie: first and end frequency rows; eo: the energy threshold.
+ * Workspace: combined (n_lag, n_freq, n_time) float64, flags (same) uint8,
+ * live (n_lag, n_time) uint8 (the shifted-veto live mask, an output too),
+ * row_off (n_lag * n_freq + 1) int64, lag_off (n_lag + 1) int64.
+ * pycwb_select_count fills them (lag_off[n_lag] = total pixels); the host
+ * reads the total, sizes the outputs (capacity pixels) and pycwb_select_emit
+ * writes, per lag from lag_off[l], the pixels in (frequency, time) order:
+ * freq, time (int64), value (summed raw energy), det_energy (n_ifo each,
+ * clipped at 0) and det_index (n_ifo each, time * n_freq + frequency). */
+typedef struct pycwb_select_desc {
+  int n_ifo;
+  int n_freq;
+  int64_t n_time;
+  const double* energy;
+  int n_lag;
+  const int64_t* shift_bins;
+  const int16_t* veto;
+  int edge_bins;
+  int ib;
+  int ie;
+  double eo;
+  double* combined;
+  uint8_t* flags;
+  uint8_t* live;
+  int64_t* row_off;
+  int64_t* lag_off;
+  int64_t capacity;
+  int64_t* freq;
+  int64_t* time;
+  double* value;
+  double* det_energy;
+  int64_t* det_index;
+} pycwb_select_desc;
+
+int pycwb_select_count(const pycwb_select_desc* desc, void* stream);
+int pycwb_select_emit(const pycwb_select_desc* desc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -481,3 +481,33 @@ def wdm_w2t(map_complex, M, filt, n_out, quad=False):
     out = np.zeros(int(n_out))
     fn(_p(col), col.shape[0], int(M), _p(filt), filt.size, 1 if quad else 0, int(n_out), _p(out))
     return out
+
+
+# ---- network-pixel selection (select_oracle.c) ----
+
+def select_pixels(energy, shift_bins, veto, edge_bins, ib, ie, eo):
+    """One lag of select_network_pixels: dict with live_mask, mask, frequency,
+    time, energy, pix_det_energy, pix_det_index (energy (n_ifo, n_freq, n_time))."""
+    lib = load()
+    fn = lib.select_oracle
+    fn.restype = C.c_int64
+    fn.argtypes = [C.c_int, C.c_int, C.c_int64, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                   C.c_double, vp, vp, vp, vp, vp, vp, vp, C.c_int64]
+    E = _c(energy, np.float64)
+    n_ifo, n_freq, n_time = E.shape
+    sh = _c(shift_bins, np.int64)
+    vt = None if veto is None else _c(veto, np.int16)
+    cap = n_freq * n_time
+    live = np.zeros(n_time, np.uint8)
+    mask = np.zeros((n_freq, n_time), np.uint8)
+    fr = np.zeros(cap, np.int64)
+    tm = np.zeros(cap, np.int64)
+    val = np.zeros(cap)
+    de = np.zeros((cap, n_ifo))
+    di = np.zeros((cap, n_ifo), np.int64)
+    n = fn(n_ifo, n_freq, n_time, _p(E), _p(sh), None if vt is None else _p(vt), int(edge_bins),
+           int(ib), int(ie), float(eo), _p(live), _p(mask), _p(fr), _p(tm), _p(val), _p(de),
+           _p(di), cap)
+    return {"live_mask": live.astype(bool), "mask": mask.astype(bool), "frequency": fr[:n],
+            "time": tm[:n], "energy": val[:n], "pix_det_energy": de[:n],
+            "pix_det_index": di[:n]}

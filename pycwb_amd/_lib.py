@@ -113,6 +113,15 @@ class XtalkDesc(C.Structure):
                 ("lookup", vp), ("xt_off", vp), ("xt", vp), ("xt_capacity", C.c_int64)]
 
 
+class SelectDesc(C.Structure):
+    _fields_ = [("n_ifo", C.c_int), ("n_freq", C.c_int), ("n_time", C.c_int64),
+                ("energy", vp), ("n_lag", C.c_int), ("shift_bins", vp), ("veto", vp),
+                ("edge_bins", C.c_int), ("ib", C.c_int), ("ie", C.c_int), ("eo", C.c_double),
+                ("combined", vp), ("flags", vp), ("live", vp), ("row_off", vp),
+                ("lag_off", vp), ("capacity", C.c_int64), ("freq", vp), ("time", vp),
+                ("value", vp), ("det_energy", vp), ("det_index", vp)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -142,6 +151,8 @@ SIGNATURES = {
     "pycwb_xtalk_count": (i32, [C.POINTER(XtalkDesc), vp]),
     "pycwb_xtalk_fill": (i32, [C.POINTER(XtalkDesc), vp]),
     "pycwb_wdm_w2t": (i32, [vp, i64, i32, i32, vp, i32, vp, i32, vp, vp, i64, vp]),
+    "pycwb_select_count": (i32, [C.POINTER(SelectDesc), vp]),
+    "pycwb_select_emit": (i32, [C.POINTER(SelectDesc), vp]),
 }
 
 
