@@ -15,8 +15,9 @@ report under their own keys: "refinement" (512 three-detector clusters, grid
 argmax then 20 projected-gradient steps of the continuous sky statistic) and
 "wdm" (forward WDM transform of 5 x 600 s at 4096 Hz at M = 8 .. 512),
 "tdamp" (TD-amplitude extraction of 200,000 pixels x 129 delays), "subnet"
-(sub-network cut of 4,096 superclusters' 300 loudest pixels at order 4) and
-"xtalk" (cross-talk CSRs of 4,096 clusters from a 7-resolution catalog).
+(sub-network cut of 4,096 superclusters' 300 loudest pixels at order 4),
+"xtalk" (cross-talk CSRs of 4,096 clusters from a 7-resolution catalog) and
+"select" (network-pixel selection of 128 lags of 5 detectors' 600 s maps).
 
 Prints one JSON line.
 """
@@ -453,6 +454,68 @@ def xtalk_leg(stream, log, n_clusters=4096, cpu=True):
     return res
 
 
+def select_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cpu=True):
+    """Network-pixel selection of one resolution (M = 64: 65 x 38,400 pixels) of
+    5 detectors' 600 s energy maps for 128 lags in one pass (count, host read
+    of the total, emit; the sparse payload comes back, not the dense masks).  Energy maps: unit-mean exponential noise plus 200
+    coherent blobs; threshold Eo = 2.3 n (about 1 % of the summed pixels above
+    it).  CPU baseline: the oracle's single-lag loop (one thread) on 4 lags,
+    whose payloads must match the GPU's bit for bit."""
+    import torch
+
+    from pycwb_amd.coherence_native.selection import select_network_pixels_lags
+
+    dev = torch.device("cuda", 0)
+    n_freq, n_time = M + 1, seconds * rate // M
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    E = -torch.log(torch.rand((n_det, n_freq, n_time), generator=g, device=dev,
+                              dtype=torch.float64))
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        f0, t0 = int(rng.integers(1, n_freq - 4)), int(rng.integers(0, n_time - 8))
+        E[:, f0:f0 + 3, t0:t0 + 6] += float(rng.uniform(2, 10))
+    edge = 2 * rate // M
+    cache = {"arrays_stack": E.contiguous(), "n_ifo": n_det, "n_freq": n_freq,
+             "n_time": n_time, "dt": M / rate, "rate": rate / M, "edge_bins": edge,
+             "valid_start": edge, "valid_stop": n_time - edge, "nn_valid": n_time - 2 * edge,
+             "ib": 1, "ie": n_freq - 1, "start": 0.0, "stop": float(seconds), "f_low": 0.0,
+             "f_high": rate / 2.0, "shift_bins_by_lag": None}
+    shifts = np.zeros((n_lag, n_det), np.int64)
+    shifts[:, 1:] = (np.arange(n_lag)[:, None] * np.arange(1, n_det)[None, :] * 7) % \
+        (n_time - 2 * edge)
+    eo = 2.3 * n_det
+    select_network_pixels_lags(cache, eo, shifts[:2], stream=stream)     # warm
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = select_network_pixels_lags(cache, eo, shifts, stream=stream, with_mask=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3
+    pix = n_freq * n_time * n_lag
+    sel = int(sum(o["time"].size for o in out))
+    res = {"M": M, "detectors": n_det, "lags": n_lag, "pixels_per_lag": n_freq * n_time,
+           "selected": sel, "ms": round(ms, 3), "lag_pixels_per_s": pix / (ms * 1e-3)}
+    log(f"select: {n_lag} lags x {n_freq * n_time} pixels in {ms:.1f} ms, {sel} selected")
+    if cpu:
+        from oracle import oracle as ora
+
+        Eh = E.cpu().numpy()
+        t = time.perf_counter()
+        outs = [ora.select_pixels(Eh, shifts[l], None, edge, 1, n_freq - 1, eo)
+                for l in range(4)]
+        cpu_s = time.perf_counter() - t
+        ok = all(np.array_equal(outs[l]["time"], out[l]["time"]) and
+                 np.array_equal(outs[l]["frequency"], out[l]["frequency"]) and
+                 np.array_equal(outs[l]["energy"], out[l]["energy"]) and
+                 np.array_equal(outs[l]["pix_det_index"], out[l]["pix_det_index"])
+                 for l in range(4))
+        res["cpu_baseline"] = {"value": 4 * n_freq * n_time / cpu_s, "unit": "lag-pixels/s",
+                               "cores": 1, "kind": "port", "sample": "lags 0-3, oracle"}
+        res["parity_sample_bit_exact"] = bool(ok)
+        log(f"select cpu baseline {4 * n_freq * n_time / cpu_s:.3e} lag-pixels/s; parity {ok}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -474,7 +537,8 @@ def main():
     ap.add_argument("--no-tdamp", action="store_true")
     ap.add_argument("--no-subnet", action="store_true")
     ap.add_argument("--no-xtalk", action="store_true")
-    ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk"),
+    ap.add_argument("--no-select", action="store_true")
+    ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk", "select"),
                     default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
@@ -513,6 +577,10 @@ def main():
         return 0
     if args.only == "tdamp":
         print(json.dumps({"tdamp": tdamp_leg(torch.cuda.current_stream(), log)}), flush=True)
+        return 0
+    if args.only == "select":
+        print(json.dumps({"select": select_leg(torch.cuda.current_stream(), log,
+                                               cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.only == "xtalk":
         print(json.dumps({"xtalk": xtalk_leg(torch.cuda.current_stream(), log,
@@ -633,6 +701,7 @@ def main():
     tdamp = None if args.no_tdamp else tdamp_leg(stream, log)
     subnet = None if args.no_subnet else subnet_leg(stream, log, cpu=not args.no_cpu)
     xtalk = None if args.no_xtalk else xtalk_leg(stream, log, cpu=not args.no_cpu)
+    select = None if args.no_select else select_leg(stream, log, cpu=not args.no_cpu)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
@@ -674,6 +743,7 @@ def main():
         "tdamp": tdamp,
         "subnet": subnet,
         "xtalk": xtalk,
+        "select": select,
     }
     print(json.dumps(out), flush=True)
     bs.close()

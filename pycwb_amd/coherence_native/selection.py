@@ -90,9 +90,11 @@ def _build_selection_cache(tf_maps, edge: float = 0.0, lag_shifts_by_lag=None,
 
 
 def select_network_pixels_lags(cache: dict, energy_threshold: float, shift_bins: np.ndarray,
-                               veto=None, stream=None) -> list[dict]:
+                               veto=None, stream=None, with_mask: bool = True) -> list[dict]:
     """Every lag of ``shift_bins`` ((n_lag, n_ifo) bins) in one GPU pass; one
-    payload per lag (the keys of select_network_pixels)."""
+    payload per lag (the keys of select_network_pixels).  ``with_mask=False``
+    leaves out the dense (n_freq, n_time) selection masks (``mask`` is None),
+    which are most of the bytes brought back to the host."""
     n_ifo, n_freq, n_time = cache["n_ifo"], cache["n_freq"], cache["n_time"]
     sh = np.ascontiguousarray(np.asarray(shift_bins, dtype=np.int64).reshape(-1, n_ifo))
     n_lag = sh.shape[0]
@@ -138,13 +140,15 @@ def select_network_pixels_lags(cache: dict, energy_threshold: float, shift_bins:
     stream.synchronize()
     fr_h, tm_h, val_h = fr.cpu().numpy(), tm.cpu().numpy(), val.cpu().numpy()
     de_h, di_h = de.cpu().numpy(), di.cpu().numpy()
-    flags_h, live_h = flags.cpu().numpy(), live.cpu().numpy()
+    flags_h = flags.cpu().numpy() if with_mask else None
+    live_h = live.cpu().numpy()
     out = []
     for l in range(n_lag):
         a, b = int(lo[l]), int(lo[l + 1])
         lm = live_h[l].astype(bool)
         out.append({
-            "mask": flags_h[l].astype(bool), "time": tm_h[a:b], "frequency": fr_h[a:b],
+            "mask": flags_h[l].astype(bool) if with_mask else None,
+            "time": tm_h[a:b], "frequency": fr_h[a:b],
             "energy": val_h[a:b], "pix_det_energy": de_h[a:b], "pix_det_index": di_h[a:b],
             "rate": cache["rate"], "layers": n_freq, "start": cache["start"],
             "stop": cache["stop"], "f_low": cache["f_low"], "f_high": cache["f_high"],
