@@ -472,6 +472,39 @@ typedef struct pycwb_select_desc {
 int pycwb_select_count(const pycwb_select_desc* desc, void* stream);
 int pycwb_select_emit(const pycwb_select_desc* desc, void* stream);
 
+/* Time-delay max-energy maps (pattern path).  Replaces the delay loop of
+ * time_delay_max_energy_numba (pycwb/modules/coherence_native/
+ * time_delay_numba.py:183-288, 578-679; called through max_energy,
+ * projection.py:63-124, by the coherence setup, setup.py:171-195).
+ * pycwb_max_energy_shift: n_copies shifted copies (n_copies, L) of a series,
+ * shift k > 0: left (x[i] = ts[i + k] for i < L - k), k < 0: right
+ * (x[i] = ts[i + k] for i >= -k), the other end keeping the original samples.
+ * pycwb_max_energy_packet: packet energy (_wdm_packet_energy_nb, :49-104) of
+ * n_maps quadrature maps (n_maps, T, M1) (00, 90) pairs -- pycwb_wdm_t2w's
+ * layout -- folded into the running pixel maximum cur (T, M1) float64
+ * (init = 1: the first map initialises it).  jb/je/mL/mH/mean/p are
+ * _compute_packet_energy_params (time_delay_packet.py:8-64).
+ * pycwb_max_energy_finish zeroes layer 0 (and 1 when zero_two_layers). */
+typedef struct pycwb_packet_desc {
+  const double* maps;
+  int n_maps;
+  int64_t T;
+  int M1;
+  int64_t jb;
+  int64_t je;
+  int mL;
+  int mH;
+  double mean;
+  int64_t p[9];
+  int init;
+  double* cur;
+} pycwb_packet_desc;
+
+int pycwb_max_energy_shift(const double* ts, int64_t L, const int64_t* shifts, int n_copies,
+                           double* out, void* stream);
+int pycwb_max_energy_packet(const pycwb_packet_desc* desc, void* stream);
+int pycwb_max_energy_finish(double* cur, int64_t T, int M1, int zero_two_layers, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

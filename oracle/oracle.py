@@ -511,3 +511,40 @@ def select_pixels(energy, shift_bins, veto, edge_bins, ib, ie, eo):
     return {"live_mask": live.astype(bool), "mask": mask.astype(bool), "frequency": fr[:n],
             "time": tm[:n], "energy": val[:n], "pix_det_energy": de[:n],
             "pix_det_index": di[:n]}
+
+
+# ---- time-delay max-energy maps (maxenergy_oracle.c) ----
+
+def max_energy_params(M1, T, pattern, edge, wavelet_rate, f_low, f_high, df):
+    """_compute_packet_energy_params: (jb, je, mL, mH, mean, p[9])."""
+    lib = load()
+    fn = lib.maxenergy_oracle_params
+    fn.restype = None
+    fn.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                   C.c_double, vp, vp, vp, vp, vp, vp]
+    jb, je = C.c_int64(), C.c_int64()
+    mL, mH = C.c_int(), C.c_int()
+    mean = C.c_double()
+    p = np.zeros(9, np.int64)
+    fn(int(M1), int(T), int(pattern), float(edge), int(wavelet_rate), float(f_low),
+       float(f_high), float(df), C.byref(jb), C.byref(je), C.byref(mL), C.byref(mH),
+       C.byref(mean), _p(p))
+    return jb.value, je.value, mL.value, mH.value, mean.value, p
+
+
+def max_energy(ts, M, filt, max_delay, downsample, pattern, edge, wavelet_rate, f_low, f_high,
+               df):
+    """(M+1, T) maximum packet energy over the time-delayed copies of ts."""
+    lib = load()
+    fn = lib.maxenergy_oracle
+    fn.restype = C.c_int64
+    fn.argtypes = [vp, C.c_int64, C.c_int, vp, C.c_int, C.c_int64, C.c_int, C.c_int,
+                   C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, vp]
+    x = _c(ts, np.float64)
+    filt = _c(filt, np.float64)
+    L = x.size
+    T = (L + (M - L % M if L % M else 0)) // M
+    out = np.zeros((T, M + 1))
+    fn(_p(x), L, int(M), _p(filt), filt.size, int(max_delay), int(downsample), int(pattern),
+       float(edge), int(wavelet_rate), float(f_low), float(f_high), float(df), _p(out))
+    return out.T.copy()

@@ -122,6 +122,12 @@ class SelectDesc(C.Structure):
                 ("value", vp), ("det_energy", vp), ("det_index", vp)]
 
 
+class PacketDesc(C.Structure):
+    _fields_ = [("maps", vp), ("n_maps", C.c_int), ("T", C.c_int64), ("M1", C.c_int),
+                ("jb", C.c_int64), ("je", C.c_int64), ("mL", C.c_int), ("mH", C.c_int),
+                ("mean", C.c_double), ("p", C.c_int64 * 9), ("init", C.c_int), ("cur", vp)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -153,6 +159,9 @@ SIGNATURES = {
     "pycwb_wdm_w2t": (i32, [vp, i64, i32, i32, vp, i32, vp, i32, vp, vp, i64, vp]),
     "pycwb_select_count": (i32, [C.POINTER(SelectDesc), vp]),
     "pycwb_select_emit": (i32, [C.POINTER(SelectDesc), vp]),
+    "pycwb_max_energy_shift": (i32, [vp, i64, vp, i32, vp, vp]),
+    "pycwb_max_energy_packet": (i32, [C.POINTER(PacketDesc), vp]),
+    "pycwb_max_energy_finish": (i32, [vp, i64, i32, i32, vp]),
 }
 
 
