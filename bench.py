@@ -16,8 +16,9 @@ argmax then 20 projected-gradient steps of the continuous sky statistic) and
 "wdm" (forward WDM transform of 5 x 600 s at 4096 Hz at M = 8 .. 512),
 "tdamp" (TD-amplitude extraction of 200,000 pixels x 129 delays), "subnet"
 (sub-network cut of 4,096 superclusters' 300 loudest pixels at order 4),
-"xtalk" (cross-talk CSRs of 4,096 clusters from a 7-resolution catalog) and
-"select" (network-pixel selection of 128 lags of 5 detectors' 600 s maps).
+"xtalk" (cross-talk CSRs of 4,096 clusters from a 7-resolution catalog),
+"select" (network-pixel selection of 128 lags of 5 detectors' 600 s maps) and
+"maxenergy" (time-delay max-energy map of a 600 s series, 87 shifted copies).
 
 Prints one JSON line.
 """
@@ -516,6 +517,60 @@ def select_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cp
     return res
 
 
+def maxenergy_leg(stream, log, M=64, seconds=600, rate=4096, max_delay_s=0.042, ds=4,
+                  pattern=3, cpu=True):
+    """Time-delay max-energy map of one detector's 600 s series at M = 64:
+    the series and its copies shifted by 4, 8, ..., 172 samples to both sides
+    (87 quadrature t2w, batched 16 at a time), packet energy (pattern 3) and
+    the pixel maximum.  CPU baseline: the oracle's loop (direct-DFT t2w, one
+    thread) on the first 30 s of the same series, max delay 12 samples, whose
+    map the GPU reproduces to 1e-10 of its maximum."""
+    import torch
+
+    from pycwb_amd.coherence_native.projection import delay_shifts, max_energy_device
+    from pycwb_amd.wdm import WDM
+
+    dev = torch.device("cuda", 0)
+    L = seconds * rate
+    g = torch.Generator(device=dev)
+    g.manual_seed(9)
+    x = torch.randn(L, generator=g, device=dev, dtype=torch.float64)
+    w = WDM(M, M, 6, 10)
+    md = int(rate * max_delay_s)
+    n_maps = int(delay_shifts(md, ds, L).size)
+    args = (w, md, ds, pattern, 1.0, rate // M, 32.0, 1024.0, rate / (2.0 * M))
+    max_energy_device(x[:8 * w.m_H], *args)      # warm (filter upload)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    cur = max_energy_device(x, *args, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    res = {"M": M, "samples": L, "maps": n_maps, "ms": round(ms, 3),
+           "samples_per_s": L / (ms * 1e-3), "map_samples_per_s": L * n_maps / (ms * 1e-3)}
+    log(f"max-energy: {n_maps} maps of {L} samples in {ms:.1f} ms")
+    if cpu:
+        from oracle import oracle as ora
+
+        Ls = 30 * rate
+        xs = x[:Ls].cpu().numpy()
+        sargs = (w, 12, ds, pattern, 1.0, rate // M, 32.0, 1024.0, rate / (2.0 * M))
+        g_small = max_energy_device(x[:Ls], *sargs).cpu().numpy()
+        t = time.perf_counter()
+        o = ora.max_energy(xs, M, w.filter, 12, ds, pattern, 1.0, rate // M, 32.0, 1024.0,
+                           rate / (2.0 * M))
+        cpu_s = time.perf_counter() - t
+        n_small = int(delay_shifts(12, ds, Ls).size)
+        ok = bool(np.abs(g_small - o.T).max() <= 1e-10 * np.abs(o).max())
+        res["cpu_baseline"] = {"value": Ls * n_small / cpu_s, "unit": "map-samples/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"first 30 s, {n_small} maps, oracle (direct-DFT t2w)"}
+        res["parity_sample"] = ok
+        log(f"max-energy cpu baseline {Ls * n_small / cpu_s:.3e} map-samples/s; parity {ok}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -538,8 +593,9 @@ def main():
     ap.add_argument("--no-subnet", action="store_true")
     ap.add_argument("--no-xtalk", action="store_true")
     ap.add_argument("--no-select", action="store_true")
-    ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk", "select"),
-                    default=None,
+    ap.add_argument("--no-maxenergy", action="store_true")
+    ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk", "select",
+                                       "maxenergy"), default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first copy this many GiB device-to-device (counter calibration)")
@@ -577,6 +633,10 @@ def main():
         return 0
     if args.only == "tdamp":
         print(json.dumps({"tdamp": tdamp_leg(torch.cuda.current_stream(), log)}), flush=True)
+        return 0
+    if args.only == "maxenergy":
+        print(json.dumps({"maxenergy": maxenergy_leg(torch.cuda.current_stream(), log,
+                                                     cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.only == "select":
         print(json.dumps({"select": select_leg(torch.cuda.current_stream(), log,
@@ -702,6 +762,7 @@ def main():
     subnet = None if args.no_subnet else subnet_leg(stream, log, cpu=not args.no_cpu)
     xtalk = None if args.no_xtalk else xtalk_leg(stream, log, cpu=not args.no_cpu)
     select = None if args.no_select else select_leg(stream, log, cpu=not args.no_cpu)
+    maxen = None if args.no_maxenergy else maxenergy_leg(stream, log, cpu=not args.no_cpu)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
@@ -744,6 +805,7 @@ def main():
         "subnet": subnet,
         "xtalk": xtalk,
         "select": select,
+        "maxenergy": maxen,
     }
     print(json.dumps(out), flush=True)
     bs.close()
