@@ -505,6 +505,57 @@ int pycwb_max_energy_shift(const double* ts, int64_t L, const int64_t* shifts, i
 int pycwb_max_energy_packet(const pycwb_packet_desc* desc, void* stream);
 int pycwb_max_energy_finish(double* cur, int64_t T, int M1, int zero_two_layers, void* stream);
 
+/* ---- Connected-component clustering of the selected pixels ----
+ * Replaces cluster_pixels (pycwb/modules/coherence_native/clustering.py:
+ * 18-157) for every lag of a call: _label_components_grid (kernels.py:24-105)
+ * and _subnet_subrho_batch_numba (kernels.py:194-229) with noise_rms = 1.
+ * Pixels: lag_off (n_lag + 1) int64 ranges of lag-major pixel lists (the
+ * layout pycwb_select_emit writes); freq, time int64 TF bins; energy (n_pix)
+ * and det_energy (n_pix, n_ifo) float64; noise_rms (n_pix, n_ifo) or NULL
+ * (all 1).  kf, kt >= 1 connectivity radii; dt = 1 / rate (1 when rate <= 0),
+ * half_rate = rate / 2, n_sub = 2 iGamma^-1(n_ifo - 1, 0.314).
+ * Outputs: label (n_pix) 1-based within the lag; lag_cluster_off (n_lag + 1)
+ * cluster ranges per lag; cluster_off (n_pix + 1 capacity) and members
+ * (n_pix): the pixels of each cluster in index order; per cluster (n_pix
+ * capacity) c_energy, c_time, c_freq (numpy's summation order), c_subnet,
+ * c_subrho.  work / work_bytes: device workspace of at least
+ * pycwb_cluster_work_bytes (an int32 index map of n_lag * n_freq * n_time
+ * cells, ten int32 arrays of n_pix and hipCUB's temporary storage).  One call
+ * is stream-ordered and needs no host synchronisation; the cluster count is
+ * lag_cluster_off[n_lag].  Limits: n_lag <= 65535, n_pix < 2^31 - 1. */
+typedef struct pycwb_cluster_desc {
+  int n_lag;
+  const int64_t* lag_off;
+  int64_t n_pix;
+  const int64_t* freq;
+  const int64_t* time;
+  const double* energy;
+  const double* det_energy;
+  const double* noise_rms;
+  int n_ifo;
+  int n_freq;
+  int n_time;
+  int kf;
+  int kt;
+  double dt;
+  double half_rate;
+  double n_sub;
+  void* work;
+  size_t work_bytes;
+  int64_t* label;
+  int64_t* lag_cluster_off;
+  int64_t* cluster_off;
+  int64_t* members;
+  double* c_energy;
+  double* c_time;
+  double* c_freq;
+  double* c_subnet;
+  double* c_subrho;
+} pycwb_cluster_desc;
+
+int pycwb_cluster_work_bytes(const pycwb_cluster_desc* desc, size_t* bytes);
+int pycwb_cluster_pixels(const pycwb_cluster_desc* desc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

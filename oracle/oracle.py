@@ -548,3 +548,63 @@ def max_energy(ts, M, filt, max_delay, downsample, pattern, edge, wavelet_rate, 
     fn(_p(x), L, int(M), _p(filt), filt.size, int(max_delay), int(downsample), int(pattern),
        float(edge), int(wavelet_rate), float(f_low), float(f_high), float(df), _p(out))
     return out.T.copy()
+
+
+# ---- connected-component clustering (cluster_oracle.c) ----
+
+def cluster_labels(freq, time, n_freq, n_time, kf=1, kt=1):
+    """_label_components_grid: 1-based labels in pixel order of the roots."""
+    lib = load()
+    fn = lib.cluster_oracle_labels
+    fn.restype = None
+    fn.argtypes = [vp, vp, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+    f, t = _c(freq, np.int64), _c(time, np.int64)
+    out = np.zeros(f.size, np.int64)
+    fn(_p(f), _p(t), f.size, int(n_freq), int(n_time), int(kf), int(kt), _p(out))
+    return out
+
+
+def np_sum(x):
+    """numpy's add.reduce of a contiguous float64 array, restated."""
+    lib = load()
+    fn = lib.cluster_oracle_np_sum
+    fn.restype = C.c_double
+    fn.argtypes = [vp, C.c_int64]
+    a = _c(x, np.float64)
+    return fn(_p(a), a.size)
+
+
+def subnet_subrho(asnr, noise_rms, n_sub):
+    """_subnet_subrho_numba on (n_pix, n_ifo) arrays: (subnet, subrho)."""
+    lib = load()
+    fn = lib.cluster_oracle_subnet
+    fn.restype = None
+    fn.argtypes = [vp, vp, C.c_int64, C.c_int, C.c_double, vp, vp]
+    a, v = _c(asnr, np.float64), _c(noise_rms, np.float64)
+    sn, sr = C.c_double(), C.c_double()
+    fn(_p(a), _p(v), a.shape[0], a.shape[1], float(n_sub), C.byref(sn), C.byref(sr))
+    return sn.value, sr.value
+
+
+def cluster_run(freq, time, energy, det_energy, n_freq, n_time, kf, kt, rate, n_sub):
+    """cluster_pixels for one payload: dict with labels, members, cluster_off,
+    energy, c_time, c_freq, subnet, subrho (per cluster)."""
+    lib = load()
+    fn = lib.cluster_oracle_run
+    fn.restype = C.c_int64
+    fn.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                   C.c_double, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp]
+    f, t = _c(freq, np.int64), _c(time, np.int64)
+    e = _c(energy, np.float64)
+    de = _c(det_energy, np.float64)
+    n = f.size
+    n_ifo = de.shape[1] if de.ndim == 2 else 0
+    lab, mem = np.zeros(n, np.int64), np.zeros(n, np.int64)
+    off = np.zeros(n + 1, np.int64)
+    cap = max(n, 1)
+    ce, ct, cf, cs, cr = (np.zeros(cap) for _ in range(5))
+    nc = fn(_p(f), _p(t), _p(e), _p(de), n, int(n_ifo), int(n_freq), int(n_time), int(kf),
+            int(kt), float(rate), float(n_sub), _p(lab), _p(mem), _p(off), _p(ce), _p(ct),
+            _p(cf), _p(cs), _p(cr))
+    return {"labels": lab, "members": mem, "cluster_off": off[:nc + 1], "energy": ce[:nc],
+            "c_time": ct[:nc], "c_freq": cf[:nc], "subnet": cs[:nc], "subrho": cr[:nc]}

@@ -128,6 +128,17 @@ class PacketDesc(C.Structure):
                 ("mean", C.c_double), ("p", C.c_int64 * 9), ("init", C.c_int), ("cur", vp)]
 
 
+class ClusterDesc(C.Structure):
+    _fields_ = [("n_lag", C.c_int), ("lag_off", vp), ("n_pix", C.c_int64), ("freq", vp),
+                ("time", vp), ("energy", vp), ("det_energy", vp), ("noise_rms", vp),
+                ("n_ifo", C.c_int), ("n_freq", C.c_int), ("n_time", C.c_int), ("kf", C.c_int),
+                ("kt", C.c_int), ("dt", C.c_double), ("half_rate", C.c_double),
+                ("n_sub", C.c_double), ("work", vp), ("work_bytes", C.c_size_t),
+                ("label", vp), ("lag_cluster_off", vp), ("cluster_off", vp), ("members", vp),
+                ("c_energy", vp), ("c_time", vp), ("c_freq", vp), ("c_subnet", vp),
+                ("c_subrho", vp)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -162,6 +173,8 @@ SIGNATURES = {
     "pycwb_max_energy_shift": (i32, [vp, i64, vp, i32, vp, vp]),
     "pycwb_max_energy_packet": (i32, [C.POINTER(PacketDesc), vp]),
     "pycwb_max_energy_finish": (i32, [vp, i64, i32, i32, vp]),
+    "pycwb_cluster_work_bytes": (i32, [C.POINTER(ClusterDesc), C.POINTER(C.c_size_t)]),
+    "pycwb_cluster_pixels": (i32, [C.POINTER(ClusterDesc), vp]),
 }
 
 

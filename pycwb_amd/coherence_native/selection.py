@@ -7,6 +7,8 @@ coherence_native.selection (selection.py:14-243).
   select_network_pixels        (:125-243) one lag, the reference's payload
   select_network_pixels_lags              every lag of a segment in one pass
                                           (pycwb_select_count / _emit)
+  select_network_pixels_device            the same, the pixels left on the
+                                          device (clustering's input)
 
 The energy maps are the real parts of the detectors' TF maps (the
 max-energy maps the coherence setup produces; veto_threshold.py:75-95).
@@ -89,12 +91,14 @@ def _build_selection_cache(tf_maps, edge: float = 0.0, lag_shifts_by_lag=None,
     }
 
 
-def select_network_pixels_lags(cache: dict, energy_threshold: float, shift_bins: np.ndarray,
-                               veto=None, stream=None, with_mask: bool = True) -> list[dict]:
-    """Every lag of ``shift_bins`` ((n_lag, n_ifo) bins) in one GPU pass; one
-    payload per lag (the keys of select_network_pixels).  ``with_mask=False``
-    leaves out the dense (n_freq, n_time) selection masks (``mask`` is None),
-    which are most of the bytes brought back to the host."""
+def select_network_pixels_device(cache: dict, energy_threshold: float, shift_bins: np.ndarray,
+                                 veto=None, stream=None) -> dict:
+    """Every lag of ``shift_bins`` ((n_lag, n_ifo) bins) in one GPU pass; the
+    selected pixels stay on the device, lag-major: ``lag_off`` (n_lag + 1)
+    host int64 ranges (and ``lag_off_dev``), ``frequency``, ``time``,
+    ``energy``, ``pix_det_energy`` (n, n_ifo), ``pix_det_index`` (n, n_ifo),
+    ``flags`` (n_lag, n_freq, n_time) uint8, ``live`` (n_lag, n_time) and the
+    support maps ``combined``."""
     n_ifo, n_freq, n_time = cache["n_ifo"], cache["n_freq"], cache["n_time"]
     sh = np.ascontiguousarray(np.asarray(shift_bins, dtype=np.int64).reshape(-1, n_ifo))
     n_lag = sh.shape[0]
@@ -137,11 +141,28 @@ def select_network_pixels_lags(cache: dict, energy_threshold: float, shift_bins:
     d.capacity, d.freq, d.time, d.value, d.det_energy, d.det_index = total, ptr(fr), ptr(tm), \
         ptr(val), ptr(de), ptr(di)
     _lib.check(lib().pycwb_select_emit(C.byref(d), st), "pycwb_select_emit")
+    return {"lag_off": lo, "lag_off_dev": lag_off, "frequency": fr, "time": tm, "energy": val,
+            "pix_det_energy": de, "pix_det_index": di, "flags": flags, "live": live,
+            "combined": comb,
+            "n_pix": total, "n_lag": n_lag, "n_ifo": n_ifo, "n_freq": n_freq,
+            "n_time": n_time}
+
+
+def select_network_pixels_lags(cache: dict, energy_threshold: float, shift_bins: np.ndarray,
+                               veto=None, stream=None, with_mask: bool = True) -> list[dict]:
+    """Every lag of ``shift_bins`` ((n_lag, n_ifo) bins) in one GPU pass; one
+    payload per lag (the keys of select_network_pixels, and ``shape``, the
+    (n_freq, n_time) of the maps).  ``with_mask=False`` leaves out the dense
+    (n_freq, n_time) selection masks (``mask`` is None), which are most of
+    the bytes brought back to the host."""
+    stream = stream or torch.cuda.current_stream()
+    sel = select_network_pixels_device(cache, energy_threshold, shift_bins, veto, stream)
     stream.synchronize()
-    fr_h, tm_h, val_h = fr.cpu().numpy(), tm.cpu().numpy(), val.cpu().numpy()
-    de_h, di_h = de.cpu().numpy(), di.cpu().numpy()
-    flags_h = flags.cpu().numpy() if with_mask else None
-    live_h = live.cpu().numpy()
+    n_lag, lo = sel["n_lag"], sel["lag_off"]
+    fr_h, tm_h, val_h = (sel[k].cpu().numpy() for k in ("frequency", "time", "energy"))
+    de_h, di_h = sel["pix_det_energy"].cpu().numpy(), sel["pix_det_index"].cpu().numpy()
+    flags_h = sel["flags"].cpu().numpy() if with_mask else None
+    live_h = sel["live"].cpu().numpy()
     out = []
     for l in range(n_lag):
         a, b = int(lo[l]), int(lo[l + 1])
@@ -150,9 +171,10 @@ def select_network_pixels_lags(cache: dict, energy_threshold: float, shift_bins:
             "mask": flags_h[l].astype(bool) if with_mask else None,
             "time": tm_h[a:b], "frequency": fr_h[a:b],
             "energy": val_h[a:b], "pix_det_energy": de_h[a:b], "pix_det_index": di_h[a:b],
-            "rate": cache["rate"], "layers": n_freq, "start": cache["start"],
+            "rate": cache["rate"], "layers": sel["n_freq"], "start": cache["start"],
             "stop": cache["stop"], "f_low": cache["f_low"], "f_high": cache["f_high"],
             "live_mask": lm, "live_samples": int(np.sum(lm)),
+            "shape": (sel["n_freq"], sel["n_time"]),
         })
     return out
 

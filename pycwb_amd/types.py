@@ -5,7 +5,8 @@ The GPU facade duck-types its inputs: a PycWB ``Cluster`` (with its
 standalone use and tests:
 
 * ``PixelArrays`` -- pycwb/types/pixel_arrays.py:48-104 (SoA pixels; TD amplitudes
-  in CSR rows ``pixel * n_ifo + ifo`` of length 2*(2K+1): phase 00 then 90);
+  in CSR rows ``pixel * n_ifo + ifo`` of length 2*(2K+1): phase 00 then 90;
+  ``from_arrays``, :159-216);
 * ``ClusterMeta`` / ``Cluster`` / ``FragmentCluster`` -- pycwb/types/network_cluster.py;
 * ``Config`` -- the parameters the likelihood reads, with the schema defaults of
   pycwb/constants/user_parameters_schema.py.
@@ -36,6 +37,28 @@ class PixelArrays:
     td_amp_flat: np.ndarray     # float32
     td_amp_offsets: np.ndarray  # int32 (n_pix * n_ifo + 1,)
     _n_ifo: int = 0
+    pixel_index: np.ndarray = None  # (n_ifo, n_pix) int32
+
+    @classmethod
+    def from_arrays(cls, *, time, frequency, layers, rate, noise_rms, pixel_index, n_ifo,
+                    core=None, likelihood=None, null=None, wave=None, w_90=None, asnr=None,
+                    a_90=None) -> "PixelArrays":
+        """pixel_arrays.py:159-216 (without TD amplitudes)."""
+        n = len(time)
+        zp = lambda: np.zeros(n, dtype=np.float32)
+        zi = lambda: np.zeros((n_ifo, n), dtype=np.float32)
+        f32 = lambda a, z: np.asarray(a, dtype=np.float32) if a is not None else z()
+        return cls(time=np.asarray(time, dtype=np.int32),
+                   frequency=np.asarray(frequency, dtype=np.int32),
+                   layers=np.asarray(layers, dtype=np.int32),
+                   rate=np.asarray(rate, dtype=np.float32),
+                   core=np.asarray(core, dtype=bool) if core is not None else np.zeros(n, bool),
+                   likelihood=f32(likelihood, zp), null=f32(null, zp),
+                   noise_rms=np.asarray(noise_rms, dtype=np.float32),
+                   wave=f32(wave, zi), w_90=f32(w_90, zi), asnr=f32(asnr, zi),
+                   a_90=f32(a_90, zi), td_amp_flat=np.zeros(0, dtype=np.float32),
+                   td_amp_offsets=np.zeros(n * n_ifo + 1, dtype=np.int32), _n_ifo=n_ifo,
+                   pixel_index=np.asarray(pixel_index, dtype=np.int32))
 
     @classmethod
     def from_td(cls, td00: np.ndarray, td90: np.ndarray, noise_rms: np.ndarray,
@@ -138,6 +161,38 @@ class Cluster:
 @dataclass
 class FragmentCluster:
     clusters: list = field(default_factory=list)
+    rate: float = 0.0
+    start: float = 0.0
+    stop: float = 0.0
+    bpp: float = 0.0
+    shift: float = 0.0
+    f_low: float = 0.0
+    f_high: float = 0.0
+    n_pix: int = 0
+    run: int = 0
+    pair: bool = False
+    subnet_threshold: float = 0.0
+
+    @staticmethod
+    def _status_ok(event_status):
+        if event_status is not None and (not isinstance(event_status, int) or event_status > 1
+                                         or event_status < -2):
+            raise ValueError('event_status must be -2, -1, 0, 1 or None')
+
+    def event_count(self, event_status=None) -> int:
+        """network_cluster.py:396-410: clusters with status < 1, or == event_status."""
+        self._status_ok(event_status)
+        if event_status is None:
+            return sum(1 for c in self.clusters if c.cluster_status < 1)
+        return sum(1 for c in self.clusters if c.cluster_status == event_status)
+
+    def pixel_count(self, event_status=None) -> int:
+        """network_cluster.py:412-426."""
+        self._status_ok(event_status)
+        if event_status is None:
+            return sum(len(c.pixel_arrays) for c in self.clusters if c.cluster_status < 1)
+        return sum(len(c.pixel_arrays) for c in self.clusters
+                   if c.cluster_status == event_status)
 
 
 @dataclass

@@ -1,0 +1,319 @@
+"""Connected-component clustering (coherence_native/clustering.py:18-157).
+
+CPU pins of the oracle (oracle/cluster_oracle.c):
+  * numpy's summation order: the restated add.reduce equals np.sum bit for
+    bit on every size from 0 to 300 and across the 8192-element chunking;
+  * labels: the same partition as scipy's connected components of the
+    brute-force pixel graph (|df| <= kf and |dt| <= kt, one TF cell always
+    joined, off-grid pixels alone), numbered by each component's smallest
+    pixel; with kf = kt = 1 the same partition as scipy.ndimage.label with
+    8-connectivity;
+  * subnet/subrho: closed forms (one detector per pixel: both 0; equal
+    amplitudes over n detectors: a_fac = 1/n, so rho = (1 - 1/n)(n a^2 -
+    2 n_sub)), the strong-two-detector limit subnet -> 1, and 0 <= subnet < 1;
+  * per-cluster energy, c_time, c_freq equal np.sum / np.mean of the member
+    arrays exactly; n_sub solves Q(n_ifo - 1, n_sub / 2) = 0.314;
+  * the reference's own known answers (coherence_native/tests/
+    test_refactor_import_contract.py:274-288).
+GPU: labels, members, offsets and every per-cluster statistic bit-exact
+against the oracle, one payload and many lags per call.
+"""
+
+import numpy as np
+import pytest
+
+
+def _payload(points, shape=(6, 10), n_ifo=1, energy=None, det=None):
+    freq = np.array([p[0] for p in points], dtype=np.int64)
+    time = np.array([p[1] for p in points], dtype=np.int64)
+    mask = np.zeros(shape, dtype=bool)
+    ok = (freq >= 0) & (freq < shape[0]) & (time >= 0) & (time < shape[1])
+    mask[freq[ok], time[ok]] = True
+    n = len(points)
+    return {"mask": mask, "time": time, "frequency": freq,
+            "energy": np.full(n, 4.0) if energy is None else energy,
+            "pix_det_energy": np.full((n, n_ifo), 4.0) if det is None else det,
+            "pix_det_index": np.arange(n * n_ifo, dtype=np.int64).reshape(n, n_ifo),
+            "rate": 2.0, "layers": shape[0], "start": 0.0, "stop": 5.0, "f_low": 0.0,
+            "f_high": 10.0}
+
+
+def _random_pixels(rng, n_freq, n_time, n, dup=0, off=0):
+    cells = rng.choice(n_freq * n_time, size=n, replace=False)
+    f, t = cells // n_time, cells % n_time
+    order = np.lexsort((t, f))
+    f, t = f[order], t[order]
+    if dup:
+        k = rng.integers(0, n, size=dup)
+        f, t = np.concatenate([f, f[k]]), np.concatenate([t, t[k]])
+    if off:
+        f = np.concatenate([f, rng.integers(-3, 0, size=off)])
+        t = np.concatenate([t, rng.integers(0, n_time, size=off)])
+    perm = rng.permutation(f.size) if (dup or off) else np.arange(f.size)
+    return f[perm].astype(np.int64), t[perm].astype(np.int64)
+
+
+def _brute_labels(f, t, n_freq, n_time, kf, kt):
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    n = f.size
+    on = (f >= 0) & (f < n_freq) & (t >= 0) & (t < n_time)
+    adj = ((np.abs(f[:, None] - f[None, :]) <= kf) & (np.abs(t[:, None] - t[None, :]) <= kt)
+           & on[:, None] & on[None, :])
+    i, j = np.nonzero(adj)
+    _, comp = connected_components(coo_matrix((np.ones(i.size), (i, j)), shape=(n, n)),
+                                   directed=False)
+    first = {}
+    lab = np.zeros(n, np.int64)
+    for k in range(n):
+        if comp[k] not in first:
+            first[comp[k]] = len(first) + 1
+        lab[k] = first[comp[k]]
+    return lab
+
+
+def test_oracle_np_sum_is_numpy_sum(oracle):
+    rng = np.random.default_rng(3)
+    for n in list(range(0, 300)) + [8191, 8192, 8193, 16385, 20000, 70001]:
+        x = rng.normal(size=n) * np.exp(rng.normal(size=n) * 4)
+        assert oracle.np_sum(x) == np.sum(x), n
+
+
+@pytest.mark.parametrize("kf,kt,dup,off", [(1, 1, 0, 0), (2, 1, 0, 0), (1, 3, 5, 0),
+                                           (1, 1, 7, 6), (3, 2, 3, 4)])
+def test_oracle_labels_are_connected_components(oracle, kf, kt, dup, off):
+    rng = np.random.default_rng(kf * 10 + kt + dup)
+    n_freq, n_time = 17, 60
+    f, t = _random_pixels(rng, n_freq, n_time, 140, dup, off)
+    lab = oracle.cluster_labels(f, t, n_freq, n_time, kf, kt)
+    np.testing.assert_array_equal(lab, _brute_labels(f, t, n_freq, n_time, kf, kt))
+
+
+def test_oracle_labels_match_ndimage_eight_connectivity(oracle):
+    from scipy import ndimage
+
+    rng = np.random.default_rng(5)
+    mask = rng.random((40, 90)) < 0.3
+    f, t = np.nonzero(mask)
+    lab = oracle.cluster_labels(f, t, 40, 90, 1, 1)
+    img, n = ndimage.label(mask, structure=np.ones((3, 3)))
+    ref = img[f, t]
+    assert lab.max() == n
+    # the same partition: a bijection between the two labelings
+    pairs = set(zip(lab.tolist(), ref.tolist()))
+    assert len(pairs) == n
+
+
+def test_oracle_subnet_closed_forms(oracle):
+    from scipy.special import gammainccinv
+
+    n_sub = 2.0 * gammainccinv(2.0, 0.314)
+    # one detector carries each pixel's energy: no subnetwork energy
+    a = np.zeros((5, 3))
+    a[np.arange(5), [0, 1, 2, 0, 1]] = [3.0, 4.0, 5.0, 6.0, 7.0]
+    assert oracle.subnet_subrho(a, np.ones_like(a), n_sub) == (0.0, 0.0)
+    # equal amplitudes over n detectors: a_fac = 1/n
+    for n, amp in ((2, 5.0), (3, 4.0), (4, 2.5)):
+        s, r = oracle.subnet_subrho(np.full((1, n), amp), np.ones((1, n)), n_sub)
+        np.testing.assert_allclose(r, np.sqrt((1 - 1 / n) * (n * amp * amp - 2 * n_sub)),
+                                   rtol=1e-14)
+    s, _ = oracle.subnet_subrho(np.full((1, 2), 1e4), np.ones((1, 2)), n_sub)
+    assert 1 - 1e-6 < s < 1
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        a = rng.exponential(3.0, size=(rng.integers(1, 40), 3))
+        s, r = oracle.subnet_subrho(a, np.ones_like(a), n_sub)
+        assert 0.0 <= s < 1.0
+    s, r = oracle.subnet_subrho(np.full((1, 2), 0.1), np.ones((1, 2)), n_sub)
+    assert np.isnan(r)
+
+
+def test_oracle_cluster_statistics_are_numpy(oracle):
+    from scipy.special import gammaincc
+
+    from pycwb_amd.coherence_native.clustering import _n_sub
+
+    rng = np.random.default_rng(9)
+    n_freq, n_time, rate = 33, 12000, 64.0
+    mask = np.zeros((n_freq, n_time), bool)
+    mask[3:30, 100:500] = True                      # one cluster of 10800 pixels
+    mask[rng.integers(0, n_freq, 400), rng.integers(0, n_time, 400)] = True
+    f, t = np.nonzero(mask)
+    n = f.size
+    e = rng.exponential(5.0, n)
+    de = rng.exponential(2.0, (n, 3))
+    ns = _n_sub(3)
+    assert abs(gammaincc(2.0, ns / 2) - 0.314) < 1e-12
+    o = oracle.cluster_run(f, t, e, de, n_freq, n_time, 1, 1, rate, ns)
+    off, mem = o["cluster_off"], o["members"]
+    assert np.diff(off).max() > 8192
+    dt = 1.0 / rate
+    for c in range(off.size - 1):
+        idx = mem[off[c]:off[c + 1]]
+        assert np.all(np.diff(idx) > 0) and np.all(o["labels"][idx] == c + 1)
+        assert o["energy"][c] == np.sum(e[idx])
+        assert o["c_time"][c] == np.mean(t[idx].astype(float) * dt - dt / 2)
+        assert o["c_freq"][c] == np.mean(f[idx].astype(float) * (rate / 2))
+    np.testing.assert_array_equal(np.sort(mem), np.arange(n))
+
+
+def test_oracle_reference_known_answers(oracle):
+    for pts, n_cl in (([], 0), ([(2, 3), (2, 4)], 1), ([(1, 2), (4, 7)], 2)):
+        p = _payload(pts)
+        o = oracle.cluster_run(p["frequency"], p["time"], p["energy"], p["pix_det_energy"], 6,
+                               10, 1, 1, 2.0, 0.0)
+        assert o["cluster_off"].size - 1 == n_cl
+
+
+# ---------------------------------------------------------------- GPU
+
+
+def _assert_same_as_oracle(g, o, lag_slice=None):
+    nc = o["cluster_off"].size - 1
+    np.testing.assert_array_equal(g["labels"], o["labels"])
+    np.testing.assert_array_equal(g["members"], o["members"])
+    np.testing.assert_array_equal(g["cluster_off"], o["cluster_off"])
+    for k in ("energy", "c_time", "c_freq", "subnet", "subrho"):
+        np.testing.assert_array_equal(g[k][:nc], o[k][:nc], err_msg=k)
+
+
+def _gpu_run(sel_host, n_freq, n_time, kf, kt, rate):
+    import torch
+
+    from pycwb_amd.coherence_native import cluster_pixels_device
+
+    h2d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    sel = {"lag_off": sel_host["lag_off"], "frequency": h2d(sel_host["frequency"]),
+           "time": h2d(sel_host["time"]), "energy": h2d(sel_host["energy"]),
+           "pix_det_energy": h2d(sel_host["pix_det_energy"]), "n_freq": n_freq,
+           "n_time": n_time}
+    out = cluster_pixels_device(sel, kt=kt, kf=kf, rate=rate)
+    n, nc = int(sel_host["lag_off"][-1]), out["n_clusters"]
+    return out, {"labels": out["label"][:n].cpu().numpy(),
+                 "members": out["members"][:n].cpu().numpy(),
+                 "cluster_off": out["cluster_off"][:nc + 1].cpu().numpy(),
+                 **{k: out[k][:nc].cpu().numpy() for k in ("energy", "c_time", "c_freq",
+                                                            "subnet", "subrho")}}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,n_ifo,kf,kt,dup,off,n_pix",
+                         [(0, 2, 1, 1, 0, 0, 300), (1, 3, 2, 1, 0, 0, 900),
+                          (2, 3, 1, 3, 9, 0, 700), (3, 1, 1, 1, 5, 7, 200),
+                          (4, 5, 3, 2, 11, 3, 1500), (5, 2, 1, 1, 0, 0, 1)])
+def test_gpu_cluster_one_payload_bit_exact(oracle, seed, n_ifo, kf, kt, dup, off, n_pix):
+    from pycwb_amd.coherence_native.clustering import _n_sub
+
+    print(f"case cluster seed={seed} n_ifo={n_ifo} kf={kf} kt={kt}", flush=True)
+    rng = np.random.default_rng(seed)
+    n_freq, n_time, rate = 33, 400, 32.0
+    f, t = _random_pixels(rng, n_freq, n_time, n_pix, dup, off)
+    n = f.size
+    e = rng.exponential(5.0, n)
+    de = rng.exponential(2.0, (n, n_ifo))
+    de[rng.random((n, n_ifo)) < 0.1] = 0.0
+    sel = {"lag_off": np.array([0, n]), "frequency": f, "time": t, "energy": e,
+           "pix_det_energy": de}
+    _, g = _gpu_run(sel, n_freq, n_time, kf, kt, rate)
+    o = oracle.cluster_run(f, t, e, de, n_freq, n_time, kf, kt, rate, _n_sub(n_ifo))
+    _assert_same_as_oracle(g, o)
+
+
+@pytest.mark.gpu
+def test_gpu_cluster_large_component_and_empty(oracle):
+    from pycwb_amd.coherence_native.clustering import _n_sub
+
+    print("case cluster large component", flush=True)
+    rng = np.random.default_rng(9)
+    n_freq, n_time, rate = 33, 12000, 64.0
+    mask = np.zeros((n_freq, n_time), bool)
+    mask[3:30, 100:500] = True
+    mask[rng.integers(0, n_freq, 400), rng.integers(0, n_time, 400)] = True
+    f, t = np.nonzero(mask)
+    n = f.size
+    e = rng.exponential(5.0, n)
+    de = rng.exponential(2.0, (n, 3))
+    sel = {"lag_off": np.array([0, n]), "frequency": f.astype(np.int64),
+           "time": t.astype(np.int64), "energy": e, "pix_det_energy": de}
+    _, g = _gpu_run(sel, n_freq, n_time, 1, 1, rate)
+    o = oracle.cluster_run(f, t, e, de, n_freq, n_time, 1, 1, rate, _n_sub(3))
+    _assert_same_as_oracle(g, o)
+    print("case cluster empty", flush=True)
+    empty = {"lag_off": np.array([0, 0, 0]), "frequency": np.zeros(0, np.int64),
+             "time": np.zeros(0, np.int64), "energy": np.zeros(0),
+             "pix_det_energy": np.zeros((0, 2))}
+    out, _ = _gpu_run(empty, n_freq, n_time, 1, 1, rate)
+    assert out["n_clusters"] == 0 and list(out["lag_cluster_off"]) == [0, 0, 0]
+
+
+@pytest.mark.gpu
+def test_gpu_cluster_many_lags_from_selection(oracle):
+    import torch
+
+    from pycwb_amd.coherence_native import cluster_pixels_device, select_network_pixels_device
+    from pycwb_amd.coherence_native.clustering import _n_sub
+
+    print("case cluster many lags", flush=True)
+    rng = np.random.default_rng(11)
+    n_ifo, n_freq, n_time = 3, 33, 600
+    E = rng.exponential(1.0, size=(n_ifo, n_freq, n_time))
+    for _ in range(12):
+        f0, t0 = rng.integers(1, n_freq - 4), rng.integers(0, n_time - 20)
+        E[:, f0:f0 + 4, t0:t0 + 15] += rng.uniform(2, 12)
+    shifts = np.concatenate([np.zeros((1, n_ifo), np.int64),
+                             rng.integers(0, n_time, size=(23, n_ifo))])
+    cache = {"arrays_stack": torch.from_numpy(E).to("cuda:0"), "n_ifo": n_ifo,
+             "n_freq": n_freq, "n_time": n_time, "dt": 1 / 16.0, "rate": 16.0,
+             "edge_bins": 0, "valid_start": 0, "valid_stop": n_time, "nn_valid": n_time,
+             "ib": 1, "ie": n_freq - 2, "start": 0.0, "stop": 0.0, "f_low": 0.0,
+             "f_high": 0.0, "shift_bins_by_lag": None}
+    sel = select_network_pixels_device(cache, 2.0 * n_ifo, shifts)
+    out = cluster_pixels_device(sel, kt=2, kf=1, rate=16.0)
+    lo, lco = out["lag_off"], out["lag_cluster_off"]
+    assert lo[-1] > 1000 and out["n_clusters"] > 24
+    host = {k: sel[k].cpu().numpy() for k in ("frequency", "time", "energy", "pix_det_energy")}
+    lab = out["label"].cpu().numpy()
+    mem = out["members"].cpu().numpy()
+    off = out["cluster_off"].cpu().numpy()
+    stats = {k: out[k].cpu().numpy() for k in ("energy", "c_time", "c_freq", "subnet", "subrho")}
+    for l in range(lo.size - 1):
+        a, b, ca, cb = lo[l], lo[l + 1], lco[l], lco[l + 1]
+        o = oracle.cluster_run(host["frequency"][a:b], host["time"][a:b], host["energy"][a:b],
+                               host["pix_det_energy"][a:b], n_freq, n_time, 1, 2, 16.0,
+                               _n_sub(n_ifo))
+        g = {"labels": lab[a:b], "members": mem[off[ca]:off[cb]] - a,
+             "cluster_off": off[ca:cb + 1] - off[ca],
+             **{k: v[ca:cb] for k, v in stats.items()}}
+        _assert_same_as_oracle(g, o)
+
+
+@pytest.mark.gpu
+def test_gpu_cluster_pixels_facade(oracle):
+    from pycwb_amd.coherence_native import cluster_pixels
+    from pycwb_amd.coherence_native.clustering import _n_sub
+
+    print("case cluster facade", flush=True)
+    for pts, n_cl, n_px in (([], 0, 0), ([(2, 3), (2, 4)], 1, 2), ([(1, 2), (4, 7)], 2, 2)):
+        fc = cluster_pixels(_payload(pts), kt=1, kf=1)
+        assert fc.event_count() == n_cl and fc.pixel_count() == n_px
+    rng = np.random.default_rng(2)
+    f, t = _random_pixels(rng, 20, 80, 300)
+    n = f.size
+    p = _payload(list(zip(f, t)), shape=(20, 80), n_ifo=2, energy=rng.exponential(3.0, n),
+                 det=rng.exponential(2.0, (n, 2)))
+    fc = cluster_pixels(p, kt=1, kf=1)
+    o = oracle.cluster_run(f, t, p["energy"], p["pix_det_energy"], 20, 80, 1, 1, 2.0, _n_sub(2))
+    assert fc.event_count() == o["cluster_off"].size - 1 and fc.n_pix == n
+    for c, cl in enumerate(fc.clusters):
+        idx = o["members"][o["cluster_off"][c]:o["cluster_off"][c + 1]]
+        pa, m = cl.pixel_arrays, cl.cluster_meta
+        np.testing.assert_array_equal(pa.time, t[idx] * 20 + f[idx])
+        np.testing.assert_array_equal(pa.frequency, f[idx])
+        np.testing.assert_array_equal(pa.pixel_index, p["pix_det_index"][idx].T)
+        np.testing.assert_array_equal(pa.asnr, np.sqrt(p["pix_det_energy"][idx]).T
+                                      .astype(np.float32))
+        np.testing.assert_array_equal(
+            [m.energy, m.like_net, m.sub_net, m.net_rho, m.c_time, m.c_freq],
+            [o["energy"][c], o["energy"][c], o["subnet"][c], o["subrho"][c], o["c_time"][c],
+             o["c_freq"][c]])
