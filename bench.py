@@ -17,8 +17,10 @@ argmax then 20 projected-gradient steps of the continuous sky statistic) and
 "tdamp" (TD-amplitude extraction of 200,000 pixels x 129 delays), "subnet"
 (sub-network cut of 4,096 superclusters' 300 loudest pixels at order 4),
 "xtalk" (cross-talk CSRs of 4,096 clusters from a 7-resolution catalog),
-"select" (network-pixel selection of 128 lags of 5 detectors' 600 s maps) and
-"maxenergy" (time-delay max-energy map of a 600 s series, 87 shifted copies).
+"select" (network-pixel selection of 128 lags of 5 detectors' 600 s maps),
+"maxenergy" (time-delay max-energy map of a 600 s series, 87 shifted copies)
+and "cluster" (connected-component clustering of the 128 lags' selected
+pixels).
 
 Prints one JSON line.
 """
@@ -455,16 +457,10 @@ def xtalk_leg(stream, log, n_clusters=4096, cpu=True):
     return res
 
 
-def select_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cpu=True):
-    """Network-pixel selection of one resolution (M = 64: 65 x 38,400 pixels) of
-    5 detectors' 600 s energy maps for 128 lags in one pass (count, host read
-    of the total, emit; the sparse payload comes back, not the dense masks).  Energy maps: unit-mean exponential noise plus 200
-    coherent blobs; threshold Eo = 2.3 n (about 1 % of the summed pixels above
-    it).  CPU baseline: the oracle's single-lag loop (one thread) on 4 lags,
-    whose payloads must match the GPU's bit for bit."""
+def _select_workload(M=64, seconds=600, rate=4096, n_det=5, n_lag=128):
+    """5 detectors' 600 s energy maps at M = 64 (65 x 38,400 pixels): unit-mean
+    exponential noise plus 200 coherent blobs, 128 lags, Eo = 2.3 n."""
     import torch
-
-    from pycwb_amd.coherence_native.selection import select_network_pixels_lags
 
     dev = torch.device("cuda", 0)
     n_freq, n_time = M + 1, seconds * rate // M
@@ -485,7 +481,23 @@ def select_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cp
     shifts = np.zeros((n_lag, n_det), np.int64)
     shifts[:, 1:] = (np.arange(n_lag)[:, None] * np.arange(1, n_det)[None, :] * 7) % \
         (n_time - 2 * edge)
-    eo = 2.3 * n_det
+    return cache, shifts, 2.3 * n_det, edge
+
+
+def select_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cpu=True):
+    """Network-pixel selection of one resolution (M = 64: 65 x 38,400 pixels) of
+    5 detectors' 600 s energy maps for 128 lags in one pass (count, host read
+    of the total, emit; the sparse payload comes back, not the dense masks).
+    Energy maps: unit-mean exponential noise plus 200 coherent blobs; threshold
+    Eo = 2.3 n (about 1 % of the summed pixels above it).  CPU baseline: the
+    oracle's single-lag loop (one thread) on 4 lags, whose payloads must match
+    the GPU's bit for bit."""
+    import torch
+
+    from pycwb_amd.coherence_native.selection import select_network_pixels_lags
+
+    cache, shifts, eo, edge = _select_workload(M, seconds, rate, n_det, n_lag)
+    n_freq, n_time = cache["n_freq"], cache["n_time"]
     select_network_pixels_lags(cache, eo, shifts[:2], stream=stream)     # warm
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -500,7 +512,7 @@ def select_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cp
     if cpu:
         from oracle import oracle as ora
 
-        Eh = E.cpu().numpy()
+        Eh = cache["arrays_stack"].cpu().numpy()
         t = time.perf_counter()
         outs = [ora.select_pixels(Eh, shifts[l], None, edge, 1, n_freq - 1, eo)
                 for l in range(4)]
@@ -514,6 +526,93 @@ def select_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cp
                                "cores": 1, "kind": "port", "sample": "lags 0-3, oracle"}
         res["parity_sample_bit_exact"] = bool(ok)
         log(f"select cpu baseline {4 * n_freq * n_time / cpu_s:.3e} lag-pixels/s; parity {ok}")
+    return res
+
+
+def cluster_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cpu=True,
+                reps=5):
+    """Connected-component clustering (kf = kt = 1) of the pixels the selection
+    leg's workload selects, all 128 lags in one stream-ordered call (index map
+    of 128 x 65 x 38,400 cells, union-find, root scan, member sort, per-cluster
+    sums and subnet/subrho), timed with HIP events over ``reps`` calls on a
+    selection already resident on the device.  CPU baseline: the oracle's
+    cluster_pixels (one thread) on lags 0-3, whose clusters must match the
+    GPU's bit for bit."""
+    import ctypes as C
+
+    import torch
+
+    from pycwb_amd import _lib
+    from pycwb_amd.coherence_native.clustering import cluster_pixels_device, _n_sub
+    from pycwb_amd.coherence_native.selection import select_network_pixels_device
+    from pycwb_amd.device import lib, ptr, stream_ptr
+
+    cache, shifts, eo, edge = _select_workload(M, seconds, rate, n_det, n_lag)
+    n_freq, n_time = cache["n_freq"], cache["n_time"]
+    sel = select_network_pixels_device(cache, eo, shifts, stream=stream)
+    stream.synchronize()
+    out = cluster_pixels_device(sel, kt=1, kf=1, rate=cache["rate"], stream=stream)  # warm
+    n, nc = int(sel["n_pix"]), out["n_clusters"]
+    # the same call through the C ABI, timed on its stream
+    d = _lib.ClusterDesc()
+    d.n_lag, d.lag_off, d.n_pix = n_lag, ptr(sel["lag_off_dev"]), n
+    d.freq, d.time, d.energy = ptr(sel["frequency"]), ptr(sel["time"]), ptr(sel["energy"])
+    d.det_energy, d.noise_rms, d.n_ifo = ptr(sel["pix_det_energy"]), None, n_det
+    d.n_freq, d.n_time, d.kf, d.kt = n_freq, n_time, 1, 1
+    d.dt, d.half_rate, d.n_sub = 1.0 / cache["rate"], cache["rate"] / 2, _n_sub(n_det)
+    d.work, d.work_bytes = ptr(out["work"]), out["work"].numel()
+    d.label, d.lag_cluster_off = ptr(out["label"]), ptr(out["lag_cluster_off_dev"])
+    d.cluster_off, d.members = ptr(out["cluster_off"]), ptr(out["members"])
+    d.c_energy, d.c_time, d.c_freq = ptr(out["energy"]), ptr(out["c_time"]), ptr(out["c_freq"])
+    d.c_subnet, d.c_subrho = ptr(out["subnet"]), ptr(out["subrho"])
+    st = stream_ptr(stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        _lib.check(lib().pycwb_cluster_pixels(C.byref(d), st), "pycwb_cluster_pixels")
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    t = time.perf_counter()
+    cluster_pixels_device(sel, kt=1, kf=1, rate=cache["rate"], stream=stream)
+    wall = (time.perf_counter() - t) * 1e3
+    sizes = np.diff(out["cluster_off"][:nc + 1].cpu().numpy())
+    res = {"M": M, "detectors": n_det, "lags": n_lag, "pixels": n, "clusters": nc,
+           "largest_cluster": int(sizes.max()) if nc else 0, "ms": round(ms, 3),
+           "wall_ms": round(wall, 3), "pixels_per_s": n / (ms * 1e-3),
+           "map_cells": n_lag * n_freq * n_time}
+    log(f"cluster: {n} pixels of {n_lag} lags -> {nc} clusters in {ms:.2f} ms "
+        f"(call with allocation and sync {wall:.1f} ms)")
+    if cpu:
+        from oracle import oracle as ora
+
+        lo, lco = out["lag_off"], out["lag_cluster_off"]
+        host = {k: sel[k].cpu().numpy() for k in ("frequency", "time", "energy",
+                                                  "pix_det_energy")}
+        lab = out["label"].cpu().numpy()
+        mem = out["members"].cpu().numpy()
+        off = out["cluster_off"].cpu().numpy()
+        stats = {k: out[k].cpu().numpy() for k in ("energy", "c_time", "c_freq", "subnet",
+                                                   "subrho")}
+        ok, n_cpu = True, 0
+        t = time.perf_counter()
+        runs = [ora.cluster_run(host["frequency"][lo[l]:lo[l + 1]],
+                                host["time"][lo[l]:lo[l + 1]], host["energy"][lo[l]:lo[l + 1]],
+                                host["pix_det_energy"][lo[l]:lo[l + 1]], n_freq, n_time, 1, 1,
+                                cache["rate"], _n_sub(n_det)) for l in range(4)]
+        cpu_s = time.perf_counter() - t
+        for l, o in enumerate(runs):
+            a, b, ca, cb = lo[l], lo[l + 1], lco[l], lco[l + 1]
+            n_cpu += b - a
+            ok = ok and np.array_equal(lab[a:b], o["labels"]) and \
+                np.array_equal(mem[off[ca]:off[cb]] - a, o["members"]) and \
+                np.array_equal(off[ca:cb + 1] - off[ca], o["cluster_off"]) and \
+                all(np.array_equal(stats[k][ca:cb], o[k], equal_nan=True)
+                    for k in ("energy", "c_time", "c_freq", "subnet", "subrho"))
+        res["cpu_baseline"] = {"value": n_cpu / cpu_s, "unit": "pixels/s", "cores": 1,
+                               "kind": "port", "sample": "lags 0-3, oracle"}
+        res["parity_sample_bit_exact"] = bool(ok)
+        log(f"cluster cpu baseline {n_cpu / cpu_s:.3e} pixels/s; parity {ok}")
     return res
 
 
@@ -594,8 +693,9 @@ def main():
     ap.add_argument("--no-xtalk", action="store_true")
     ap.add_argument("--no-select", action="store_true")
     ap.add_argument("--no-maxenergy", action="store_true")
+    ap.add_argument("--no-cluster", action="store_true")
     ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk", "select",
-                                       "maxenergy"), default=None,
+                                       "maxenergy", "cluster"), default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first copy this many GiB device-to-device (counter calibration)")
@@ -641,6 +741,10 @@ def main():
     if args.only == "select":
         print(json.dumps({"select": select_leg(torch.cuda.current_stream(), log,
                                                cpu=not args.no_cpu)}), flush=True)
+        return 0
+    if args.only == "cluster":
+        print(json.dumps({"cluster": cluster_leg(torch.cuda.current_stream(), log,
+                                                 cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.only == "xtalk":
         print(json.dumps({"xtalk": xtalk_leg(torch.cuda.current_stream(), log,
@@ -763,6 +867,7 @@ def main():
     xtalk = None if args.no_xtalk else xtalk_leg(stream, log, cpu=not args.no_cpu)
     select = None if args.no_select else select_leg(stream, log, cpu=not args.no_cpu)
     maxen = None if args.no_maxenergy else maxenergy_leg(stream, log, cpu=not args.no_cpu)
+    clus = None if args.no_cluster else cluster_leg(stream, log, cpu=not args.no_cpu)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
@@ -806,6 +911,7 @@ def main():
         "xtalk": xtalk,
         "select": select,
         "maxenergy": maxen,
+        "cluster": clus,
     }
     print(json.dumps(out), flush=True)
     bs.close()

@@ -194,6 +194,26 @@ class FragmentCluster:
         return sum(len(c.pixel_arrays) for c in self.clusters
                    if c.cluster_status == event_status)
 
+    def remove_rejected(self) -> None:
+        """network_cluster.py:447-451."""
+        self.clusters = [c for c in self.clusters if c.cluster_status < 1]
+
+    def select(self, name: str, threshold: float) -> np.ndarray:
+        """network_cluster.py:453-487: rejects (status 1) the not-yet-rejected
+        clusters whose subrho (net_rho) or subnet (sub_net) is below threshold
+        (a NaN value is not below it); returns the values."""
+        key = str(name).strip().lower()
+        if key == "subrho":
+            values = np.array([c.cluster_meta.net_rho for c in self.clusters], dtype=float)
+        elif key == "subnet":
+            values = np.array([c.cluster_meta.sub_net for c in self.clusters], dtype=float)
+        else:
+            raise ValueError(f"Unsupported selection '{name}'. Supported: subrho, subnet")
+        for c, v in zip(self.clusters, values):
+            if c.cluster_status <= 0 and v < threshold:
+                c.cluster_status = 1
+        return values
+
 
 @dataclass
 class Config:

@@ -317,3 +317,21 @@ def test_gpu_cluster_pixels_facade(oracle):
             [m.energy, m.like_net, m.sub_net, m.net_rho, m.c_time, m.c_freq],
             [o["energy"][c], o["energy"][c], o["subnet"][c], o["subrho"][c], o["c_time"][c],
              o["c_freq"][c]])
+
+
+def test_fragment_select_and_remove_rejected():
+    from pycwb_amd.types import Cluster, ClusterMeta, FragmentCluster
+
+    metas = [ClusterMeta(net_rho=r, sub_net=s) for r, s in
+             ((5.0, 0.9), (2.0, 0.9), (float("nan"), 0.1), (7.0, 0.3))]
+    fc = FragmentCluster(clusters=[Cluster(cluster_meta=m) for m in metas])
+    fc.clusters[3].cluster_status = -1
+    np.testing.assert_array_equal(fc.select("subrho", 4.0), [5.0, 2.0, np.nan, 7.0])
+    assert [c.cluster_status for c in fc.clusters] == [0, 1, 0, -1]
+    fc.select("SubNet ", 0.5)
+    assert [c.cluster_status for c in fc.clusters] == [0, 1, 1, 1]
+    assert fc.event_count() == 1 and fc.event_count(1) == 3
+    fc.remove_rejected()
+    assert len(fc.clusters) == 1
+    with pytest.raises(ValueError):
+        fc.select("likelihood", 1.0)
