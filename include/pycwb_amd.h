@@ -556,6 +556,43 @@ typedef struct pycwb_cluster_desc {
 int pycwb_cluster_work_bytes(const pycwb_cluster_desc* desc, size_t* bytes);
 int pycwb_cluster_pixels(const pycwb_cluster_desc* desc, void* stream);
 
+/* ---- Pixel-energy threshold statistics ----
+ * Replaces the array work of compute_threshold
+ * (pycwb/modules/coherence_native/veto_threshold.py:98-222); the Gamma fits on
+ * the resulting scalars stay on the host.  maps (n_ifo, n_freq, n_time)
+ * float64 are summed over detectors in order and clipped to [0, hi]
+ * (hi = n_ifo * 100) into values (count), in the reference's order:
+ * time_major = 0: rows of frequency over time bins [begin, begin + count /
+ * n_freq) (the edge-cropped sum raveled, :173-176); time_major = 1: the
+ * time-major flat indices [begin, begin + count) (t * n_freq + f; the
+ * w[nL:nR] region, :136-157).  counts[0], counts[1]: values above 1e-4 and
+ * above 1e-3.  sorted (count) or NULL: the values ascending (np.sort,
+ * :186-201).  chunk_sums (2 * ceil(count / 8192)) or NULL: the values above
+ * 1e-3, in order, summed per 8192-value chunk in numpy's pairwise order, with
+ * the sum of their logs beside each (np.mean(positive), np.mean(np.log(
+ * positive)), :162-163, once the host adds the counts[1] / 8192 rounded up
+ * chunk sums to 0.0 in order).  work: at least pycwb_threshold_work_bytes.
+ * count < 2^31 - 1. */
+typedef struct pycwb_threshold_desc {
+  const double* maps;
+  int n_ifo;
+  int n_freq;
+  int64_t n_time;
+  int time_major;
+  int64_t begin;
+  int64_t count;
+  double hi;
+  void* work;
+  size_t work_bytes;
+  double* values;
+  uint64_t* counts;
+  double* sorted;
+  double* chunk_sums;
+} pycwb_threshold_desc;
+
+int pycwb_threshold_work_bytes(const pycwb_threshold_desc* desc, size_t* bytes);
+int pycwb_threshold_stats(const pycwb_threshold_desc* desc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

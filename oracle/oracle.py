@@ -608,3 +608,35 @@ def cluster_run(freq, time, energy, det_energy, n_freq, n_time, kf, kt, rate, n_
             _p(cf), _p(cs), _p(cr))
     return {"labels": lab, "members": mem, "cluster_off": off[:nc + 1], "energy": ce[:nc],
             "c_time": ct[:nc], "c_freq": cf[:nc], "subnet": cs[:nc], "subrho": cr[:nc]}
+
+
+# ---- threshold statistics (threshold_oracle.c) ----
+
+def threshold_stats(maps, time_major, begin, count, hi):
+    """compute_threshold's array work on (n_ifo, n_freq, n_time) maps: dict with
+    values, counts (> 1e-4, > 1e-3), sorted, sum and log_sum of the values
+    above 1e-3 (numpy order), n_pos."""
+    lib = load()
+    E = _c(maps, np.float64)
+    n_ifo, n_freq, n_time = E.shape
+    fv = lib.threshold_oracle_values
+    fv.restype = None
+    fv.argtypes = [vp, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int64, C.c_int64, C.c_double,
+                   vp, vp]
+    vals = np.zeros(max(count, 1))
+    counts = np.zeros(2, np.int64)
+    fv(_p(E), n_ifo, n_freq, n_time, int(time_major), int(begin), int(count), float(hi),
+       _p(vals), _p(counts))
+    vals = vals[:count]
+    fs = lib.threshold_oracle_sorted
+    fs.restype = None
+    fs.argtypes = [vp, C.c_int64, vp]
+    srt = np.zeros(max(count, 1))
+    fs(_p(np.ascontiguousarray(vals)), int(count), _p(srt))
+    fp = lib.threshold_oracle_positive_sums
+    fp.restype = C.c_int64
+    fp.argtypes = [vp, C.c_int64, vp, vp]
+    s, ls = C.c_double(), C.c_double()
+    m = fp(_p(np.ascontiguousarray(vals)), int(count), C.byref(s), C.byref(ls))
+    return {"values": vals, "counts": counts, "sorted": srt[:count], "sum": s.value,
+            "log_sum": ls.value, "n_pos": m}

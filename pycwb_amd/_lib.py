@@ -139,6 +139,13 @@ class ClusterDesc(C.Structure):
                 ("c_subrho", vp)]
 
 
+class ThresholdDesc(C.Structure):
+    _fields_ = [("maps", vp), ("n_ifo", C.c_int), ("n_freq", C.c_int), ("n_time", C.c_int64),
+                ("time_major", C.c_int), ("begin", C.c_int64), ("count", C.c_int64),
+                ("hi", C.c_double), ("work", vp), ("work_bytes", C.c_size_t), ("values", vp),
+                ("counts", vp), ("sorted", vp), ("chunk_sums", vp)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -175,6 +182,8 @@ SIGNATURES = {
     "pycwb_max_energy_finish": (i32, [vp, i64, i32, i32, vp]),
     "pycwb_cluster_work_bytes": (i32, [C.POINTER(ClusterDesc), C.POINTER(C.c_size_t)]),
     "pycwb_cluster_pixels": (i32, [C.POINTER(ClusterDesc), vp]),
+    "pycwb_threshold_work_bytes": (i32, [C.POINTER(ThresholdDesc), C.POINTER(C.c_size_t)]),
+    "pycwb_threshold_stats": (i32, [C.POINTER(ThresholdDesc), vp]),
 }
 
 

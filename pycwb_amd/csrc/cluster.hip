@@ -27,14 +27,15 @@
 //   (sort)                stable radix sort of (cluster id, pixel index): the
 //                         members of each cluster in pixel order
 //   k_cl_members  slot    members and cluster offsets
-//   k_cl_stats    cluster energy, c_time, c_freq (numpy's add.reduce order),
-//                         subnet and subrho (kernels.py:141-191)
+//   k_cl_stats    cluster energy, c_time, c_freq (numpy's add.reduce order,
+//                         np_sum.hh), subnet and subrho (kernels.py:141-191)
 // The scan and the sort are hipCUB's device-wide primitives.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <math.h>
 #include <stdint.h>
 
+#include "np_sum.hh"
 #include "skyscan_lane.hh"
 #include "pycwb_amd.h"
 
@@ -206,77 +207,6 @@ __global__ void k_cl_empty(ClArgs a) {
   const int i = threadIdx.x + blockIdx.x * kBlock;
   if (i <= a.n_lag) a.lag_cluster_off[i] = 0;
   if (i == 0) a.cluster_off[0] = 0;
-}
-
-// numpy's pairwise sum of g(s), ..., g(s + n - 1) (n <= 8192): below 8 values
-// a plain loop from 0.0, up to 128 values eight strided accumulators combined
-// as ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)) and the rest added in
-// order, else the sum of the halves split at n / 2 rounded down to a
-// multiple of 8.  The recursion runs on an explicit stack (depth <= 7).
-template <class G>
-__device__ double pw_leaf(const G& g, int64_t s, int64_t n) {
-  if (n < 8) {
-    double r = 0.0;
-    for (int64_t i = 0; i < n; ++i) r += g(s + i);
-    return r;
-  }
-  double r0 = g(s), r1 = g(s + 1), r2 = g(s + 2), r3 = g(s + 3);
-  double r4 = g(s + 4), r5 = g(s + 5), r6 = g(s + 6), r7 = g(s + 7);
-  int64_t i = 8;
-  for (; i < n - (n % 8); i += 8) {
-    r0 += g(s + i);
-    r1 += g(s + i + 1);
-    r2 += g(s + i + 2);
-    r3 += g(s + i + 3);
-    r4 += g(s + i + 4);
-    r5 += g(s + i + 5);
-    r6 += g(s + i + 6);
-    r7 += g(s + i + 7);
-  }
-  double res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
-  for (; i < n; ++i) res += g(s + i);
-  return res;
-}
-
-template <class G>
-__device__ double pw_sum(const G& g, int64_t s, int64_t n) {
-  constexpr int kDepth = 12;
-  int64_t fs[kDepth], fn[kDepth];
-  double fv[kDepth];
-  int ph[kDepth];
-  int sp = 0;
-  fs[0] = s;
-  fn[0] = n;
-  ph[0] = 0;
-  double ret = 0.0;
-  for (;;) {
-    if (ph[sp] == 0 && fn[sp] <= 128) {
-      ret = pw_leaf(g, fs[sp], fn[sp]);
-    } else if (ph[sp] < 2) {
-      int64_t n2 = fn[sp] / 2;
-      n2 -= n2 % 8;
-      if (ph[sp] == 1) fv[sp] = ret;
-      fs[sp + 1] = ph[sp] == 0 ? fs[sp] : fs[sp] + n2;
-      fn[sp + 1] = ph[sp] == 0 ? n2 : fn[sp] - n2;
-      ph[sp + 1] = 0;
-      ph[sp] += 1;
-      ++sp;
-      continue;
-    } else {
-      ret = fv[sp] + ret;
-    }
-    if (sp == 0) return ret;
-    --sp;
-  }
-}
-
-// numpy's add.reduce of a contiguous float64 array: 8192-element chunks, each
-// summed pairwise, added to 0.0 in order.
-template <class G>
-__device__ double np_reduce(const G& g, int64_t n) {
-  double r = 0.0;
-  for (int64_t s = 0; s < n; s += 8192) r += pw_sum(g, s, n - s < 8192 ? n - s : 8192);
-  return r;
 }
 
 struct EnergyOf {
