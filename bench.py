@@ -19,8 +19,9 @@ argmax then 20 projected-gradient steps of the continuous sky statistic) and
 "xtalk" (cross-talk CSRs of 4,096 clusters from a 7-resolution catalog),
 "select" (network-pixel selection of 128 lags of 5 detectors' 600 s maps),
 "maxenergy" (time-delay max-energy map of a 600 s series, 87 shifted copies)
-and "cluster" (connected-component clustering of the 128 lags' selected
-pixels).
+"cluster" (connected-component clustering of the 128 lags' selected
+pixels) and "threshold" (the pixel-energy threshold's array statistics of the
+same maps).
 
 Prints one JSON line.
 """
@@ -616,6 +617,86 @@ def cluster_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, c
     return res
 
 
+def threshold_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, cpu=True, reps=5):
+    """Array work of the pixel-energy threshold on the selection workload's
+    maps (5 detectors, 65 x 38,400 pixels, 2 s edges): the no-shape path
+    (edge-cropped detector sum, clip, counts, radix sort of 2.48e6 values) and
+    the shape path (time-major region, clip, counts, compaction of the values
+    above 1e-3, numpy-order sums of them and of their logs), each one
+    pycwb_threshold_stats call timed with HIP events over ``reps`` calls.
+    CPU baseline: the oracle's restatement of both (one thread) on the same
+    maps; counts, order statistics and sums must equal the GPU's."""
+    import ctypes as C
+
+    import torch
+
+    from pycwb_amd import _lib
+    from pycwb_amd.device import lib, ptr, stream_ptr
+
+    cache, _, _, edge = _select_workload(M, seconds, rate, n_det, 1)
+    E = cache["arrays_stack"]
+    n_freq, n_time = cache["n_freq"], cache["n_time"]
+    dev = E.device
+    st = stream_ptr(stream)
+    res = {"M": M, "detectors": n_det, "pixels": n_freq * n_time}
+    outs = {}
+    for name, tm, begin, count in (
+            ("no_shape", 0, edge, n_freq * (n_time - 2 * edge)),
+            ("shape", 1, edge * n_freq, n_freq * n_time - 2 * edge * n_freq - 1)):
+        d = _lib.ThresholdDesc()
+        d.maps, d.n_ifo, d.n_freq, d.n_time = ptr(E), n_det, n_freq, n_time
+        d.time_major, d.begin, d.count, d.hi = tm, begin, count, n_det * 100.0
+        vals = torch.empty(count, dtype=torch.float64, device=dev)
+        counts = torch.empty(2, dtype=torch.int64, device=dev)
+        srt = torch.empty(count, dtype=torch.float64, device=dev) if tm == 0 else None
+        sums = torch.empty(2 * ((count + 8191) // 8192), dtype=torch.float64, device=dev) \
+            if tm == 1 else None
+        d.values, d.counts, d.sorted, d.chunk_sums = ptr(vals), ptr(counts), ptr(srt), ptr(sums)
+        nb = C.c_size_t()
+        _lib.check(lib().pycwb_threshold_work_bytes(C.byref(d), C.byref(nb)), "work bytes")
+        work = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
+        d.work, d.work_bytes = ptr(work), int(nb.value)
+        _lib.check(lib().pycwb_threshold_stats(C.byref(d), st), "pycwb_threshold_stats")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            _lib.check(lib().pycwb_threshold_stats(C.byref(d), st), "pycwb_threshold_stats")
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        c = counts.cpu().numpy()
+        outs[name] = (tm, begin, count, c, srt, sums)
+        res[name] = {"values": count, "ms": round(ms, 4), "values_per_s": count / (ms * 1e-3),
+                     "above_1e-3": int(c[1]),
+                     "read_gbs": n_det * count * 8 / (ms * 1e-3) / 1e9}
+        log(f"threshold {name}: {count} values in {ms:.3f} ms")
+    if cpu:
+        from oracle import oracle as ora
+
+        Eh = E.cpu().numpy()
+        ok, cpu_s, n_cpu = True, 0.0, 0
+        for name, (tm, begin, count, c, srt, sums) in outs.items():
+            t = time.perf_counter()
+            o = ora.threshold_stats(Eh, tm, begin, count, n_det * 100.0)
+            cpu_s += time.perf_counter() - t
+            n_cpu += count
+            ok = ok and list(o["counts"]) == [int(c[0]), int(c[1])]
+            if srt is not None:
+                ok = ok and np.array_equal(srt.cpu().numpy(), o["sorted"])
+            if sums is not None:
+                ch = sums.cpu().numpy()
+                s = ls = 0.0
+                for k in range((int(c[1]) + 8191) // 8192):
+                    s += float(ch[2 * k])
+                    ls += float(ch[2 * k + 1])
+                ok = ok and s == o["sum"] and abs(ls - o["log_sum"]) <= 1e-13 * abs(o["log_sum"])
+        res["cpu_baseline"] = {"value": n_cpu / cpu_s, "unit": "values/s", "cores": 1,
+                               "kind": "port", "sample": "both paths, whole maps, oracle"}
+        res["parity_bit_exact"] = bool(ok)
+        log(f"threshold cpu baseline {n_cpu / cpu_s:.3e} values/s; parity {ok}")
+    return res
+
+
 def maxenergy_leg(stream, log, M=64, seconds=600, rate=4096, max_delay_s=0.042, ds=4,
                   pattern=3, cpu=True):
     """Time-delay max-energy map of one detector's 600 s series at M = 64:
@@ -694,8 +775,9 @@ def main():
     ap.add_argument("--no-select", action="store_true")
     ap.add_argument("--no-maxenergy", action="store_true")
     ap.add_argument("--no-cluster", action="store_true")
+    ap.add_argument("--no-threshold", action="store_true")
     ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk", "select",
-                                       "maxenergy", "cluster"), default=None,
+                                       "maxenergy", "cluster", "threshold"), default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first copy this many GiB device-to-device (counter calibration)")
@@ -741,6 +823,10 @@ def main():
     if args.only == "select":
         print(json.dumps({"select": select_leg(torch.cuda.current_stream(), log,
                                                cpu=not args.no_cpu)}), flush=True)
+        return 0
+    if args.only == "threshold":
+        print(json.dumps({"threshold": threshold_leg(torch.cuda.current_stream(), log,
+                                                     cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.only == "cluster":
         print(json.dumps({"cluster": cluster_leg(torch.cuda.current_stream(), log,
@@ -868,6 +954,7 @@ def main():
     select = None if args.no_select else select_leg(stream, log, cpu=not args.no_cpu)
     maxen = None if args.no_maxenergy else maxenergy_leg(stream, log, cpu=not args.no_cpu)
     clus = None if args.no_cluster else cluster_leg(stream, log, cpu=not args.no_cpu)
+    thr = None if args.no_threshold else threshold_leg(stream, log, cpu=not args.no_cpu)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
@@ -912,6 +999,7 @@ def main():
         "select": select,
         "maxenergy": maxen,
         "cluster": clus,
+        "threshold": thr,
     }
     print(json.dumps(out), flush=True)
     bs.close()

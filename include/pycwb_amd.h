@@ -572,7 +572,7 @@ int pycwb_cluster_pixels(const pycwb_cluster_desc* desc, void* stream);
  * the sum of their logs beside each (np.mean(positive), np.mean(np.log(
  * positive)), :162-163, once the host adds the counts[1] / 8192 rounded up
  * chunk sums to 0.0 in order).  work: at least pycwb_threshold_work_bytes.
- * count < 2^31 - 1. */
+ * begin + count < 2^31 - 1. */
 typedef struct pycwb_threshold_desc {
   const double* maps;
   int n_ifo;

@@ -43,38 +43,49 @@ __device__ double np_leaf(const G& g, int64_t s, int64_t n) {
   return res;
 }
 
-// The pairwise tree over [0, n) (n <= 8192): leaf(s, len) gives each leaf's
-// value, the tree adds them as numpy does.
-template <class Leaf>
-__device__ double np_tree(int64_t n, Leaf& leaf) {
-  constexpr int kDepth = 12;
+// Frames of the explicit recursion stack (depth <= 7 for n <= 8192).
+struct NpFrames {
+  static constexpr int kDepth = 12;
   int64_t fs[kDepth], fn[kDepth];
   double fv[kDepth];
   int ph[kDepth];
+};
+
+// The pairwise tree over [0, n) (n <= 8192): leaf(s, len) gives each leaf's
+// value, the tree adds them as numpy does.  F holds the stack: a thread's own
+// frames, or frames in LDS for a single thread of a block.
+template <class Leaf, class Frames>
+__device__ double np_tree_in(int64_t n, Leaf& leaf, Frames& F) {
   int sp = 0;
-  fs[0] = 0;
-  fn[0] = n;
-  ph[0] = 0;
+  F.fs[0] = 0;
+  F.fn[0] = n;
+  F.ph[0] = 0;
   double ret = 0.0;
   for (;;) {
-    if (ph[sp] == 0 && fn[sp] <= kNpLeaf) {
-      ret = leaf(fs[sp], fn[sp]);
-    } else if (ph[sp] < 2) {
-      int64_t n2 = fn[sp] / 2;
+    if (F.ph[sp] == 0 && F.fn[sp] <= kNpLeaf) {
+      ret = leaf(F.fs[sp], F.fn[sp]);
+    } else if (F.ph[sp] < 2) {
+      int64_t n2 = F.fn[sp] / 2;
       n2 -= n2 % 8;
-      if (ph[sp] == 1) fv[sp] = ret;
-      fs[sp + 1] = ph[sp] == 0 ? fs[sp] : fs[sp] + n2;
-      fn[sp + 1] = ph[sp] == 0 ? n2 : fn[sp] - n2;
-      ph[sp + 1] = 0;
-      ph[sp] += 1;
+      if (F.ph[sp] == 1) F.fv[sp] = ret;
+      F.fs[sp + 1] = F.ph[sp] == 0 ? F.fs[sp] : F.fs[sp] + n2;
+      F.fn[sp + 1] = F.ph[sp] == 0 ? n2 : F.fn[sp] - n2;
+      F.ph[sp + 1] = 0;
+      F.ph[sp] += 1;
       ++sp;
       continue;
     } else {
-      ret = fv[sp] + ret;
+      ret = F.fv[sp] + ret;
     }
     if (sp == 0) return ret;
     --sp;
   }
+}
+
+template <class Leaf>
+__device__ double np_tree(int64_t n, Leaf& leaf) {
+  NpFrames F;
+  return np_tree_in(n, leaf, F);
 }
 
 template <class G>
