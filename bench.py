@@ -534,9 +534,10 @@ def cluster_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, c
                 reps=5):
     """Connected-component clustering (kf = kt = 1) of the pixels the selection
     leg's workload selects, all 128 lags in one stream-ordered call (index map
-    of 128 x 65 x 38,400 cells, union-find, root scan, member sort, per-cluster
-    sums and subnet/subrho), timed with HIP events over ``reps`` calls on a
-    selection already resident on the device.  CPU baseline: the oracle's
+    of 128 x 65 x 38,400 cells, initialised once and restored by each call;
+    union-find, root scan, member sort, per-cluster sums and subnet/subrho),
+    timed with HIP events over ``reps`` calls on a selection already resident
+    on the device.  CPU baseline: the oracle's
     cluster_pixels (one thread) on lags 0-3, whose clusters must match the
     GPU's bit for bit."""
     import ctypes as C
@@ -562,6 +563,7 @@ def cluster_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, c
     d.n_freq, d.n_time, d.kf, d.kt = n_freq, n_time, 1, 1
     d.dt, d.half_rate, d.n_sub = 1.0 / cache["rate"], cache["rate"] / 2, _n_sub(n_det)
     d.work, d.work_bytes = ptr(out["work"]), out["work"].numel()
+    d.index_map = out["index_map"]
     d.label, d.lag_cluster_off = ptr(out["label"]), ptr(out["lag_cluster_off_dev"])
     d.cluster_off, d.members = ptr(out["cluster_off"]), ptr(out["members"])
     d.c_energy, d.c_time, d.c_freq = ptr(out["energy"]), ptr(out["c_time"]), ptr(out["c_freq"])

@@ -519,10 +519,13 @@ int pycwb_max_energy_finish(double* cur, int64_t T, int M1, int zero_two_layers,
  * (n_pix): the pixels of each cluster in index order; per cluster (n_pix
  * capacity) c_energy, c_time, c_freq (numpy's summation order), c_subnet,
  * c_subrho.  work / work_bytes: device workspace of at least
- * pycwb_cluster_work_bytes (an int32 index map of n_lag * n_freq * n_time
- * cells, ten int32 arrays of n_pix and hipCUB's temporary storage).  One call
- * is stream-ordered and needs no host synchronisation; the cluster count is
- * lag_cluster_off[n_lag].  Limits: n_lag <= 65535, n_pix < 2^31 - 1. */
+ * pycwb_cluster_work_bytes (nine int32 and four float64 values per pixel and
+ * hipCUB's temporary storage).  index_map: int32 (n_lag, n_freq, n_time) cells holding -1 on
+ * entry (pycwb_cluster_map_init sets them once; every call leaves them -1
+ * again, so one map serves every later call of that shape or smaller on the
+ * same stream).  One call is stream-ordered and needs no host
+ * synchronisation; the cluster count is lag_cluster_off[n_lag].  Limits:
+ * n_lag <= 65535, n_pix < 2^31 - 1. */
 typedef struct pycwb_cluster_desc {
   int n_lag;
   const int64_t* lag_off;
@@ -542,6 +545,7 @@ typedef struct pycwb_cluster_desc {
   double n_sub;
   void* work;
   size_t work_bytes;
+  int32_t* index_map;
   int64_t* label;
   int64_t* lag_cluster_off;
   int64_t* cluster_off;
@@ -553,6 +557,7 @@ typedef struct pycwb_cluster_desc {
   double* c_subrho;
 } pycwb_cluster_desc;
 
+int pycwb_cluster_map_init(int32_t* index_map, int64_t cells, void* stream);
 int pycwb_cluster_work_bytes(const pycwb_cluster_desc* desc, size_t* bytes);
 int pycwb_cluster_pixels(const pycwb_cluster_desc* desc, void* stream);
 

@@ -134,7 +134,8 @@ class ClusterDesc(C.Structure):
                 ("n_ifo", C.c_int), ("n_freq", C.c_int), ("n_time", C.c_int), ("kf", C.c_int),
                 ("kt", C.c_int), ("dt", C.c_double), ("half_rate", C.c_double),
                 ("n_sub", C.c_double), ("work", vp), ("work_bytes", C.c_size_t),
-                ("label", vp), ("lag_cluster_off", vp), ("cluster_off", vp), ("members", vp),
+                ("index_map", vp), ("label", vp), ("lag_cluster_off", vp), ("cluster_off", vp),
+                ("members", vp),
                 ("c_energy", vp), ("c_time", vp), ("c_freq", vp), ("c_subnet", vp),
                 ("c_subrho", vp)]
 
@@ -180,6 +181,7 @@ SIGNATURES = {
     "pycwb_max_energy_shift": (i32, [vp, i64, vp, i32, vp, vp]),
     "pycwb_max_energy_packet": (i32, [C.POINTER(PacketDesc), vp]),
     "pycwb_max_energy_finish": (i32, [vp, i64, i32, i32, vp]),
+    "pycwb_cluster_map_init": (i32, [vp, i64, vp]),
     "pycwb_cluster_work_bytes": (i32, [C.POINTER(ClusterDesc), C.POINTER(C.c_size_t)]),
     "pycwb_cluster_pixels": (i32, [C.POINTER(ClusterDesc), vp]),
     "pycwb_threshold_work_bytes": (i32, [C.POINTER(ThresholdDesc), C.POINTER(C.c_size_t)]),

@@ -15,6 +15,7 @@ n_sub = 2 iGamma^-1(n_ifo - 1, 0.314) is the reference's host constant
 from __future__ import annotations
 
 import ctypes as C
+import threading
 
 import numpy as np
 import torch
@@ -23,6 +24,25 @@ from .. import _lib
 from ..device import lib, ptr, require_gpu, stream_ptr
 from ..types import Cluster, ClusterMeta, FragmentCluster, PixelArrays
 from .veto_threshold import _igamma_inv_upper
+
+
+_maps: dict = {}
+_maps_lock = threading.Lock()
+
+
+def _index_map(dev: torch.device, cells: int, stream) -> torch.Tensor:
+    """The device's cached int32 index map (grow-only), all -1 between calls:
+    pycwb_cluster_map_init fills a new one, every clustering call restores the
+    cells it touched.  Calls that share it must not overlap (one stream)."""
+    with _maps_lock:
+        m = _maps.get(dev.index)
+        if m is None or m.numel() < cells:
+            _maps.pop(dev.index, None)
+            m = torch.empty(cells, dtype=torch.int32, device=dev)
+            _lib.check(lib().pycwb_cluster_map_init(ptr(m), cells, stream_ptr(stream)),
+                       "pycwb_cluster_map_init")
+            _maps[dev.index] = m
+        return m
 
 
 def _n_sub(n_ifo: int) -> float:
@@ -73,8 +93,17 @@ def cluster_pixels_device(sel: dict, kt: int = 1, kf: int = 1, rate: float = 0.0
     work = torch.empty(max(int(nb.value), 1), dtype=torch.uint8, device=dev)
     d.work, d.work_bytes = ptr(work), int(nb.value)
     stream = stream or torch.cuda.current_stream()
-    _lib.check(lib().pycwb_cluster_pixels(C.byref(d), stream_ptr(stream)), "pycwb_cluster_pixels")
+    if n > 0:
+        d.index_map = ptr(_index_map(dev, max(n_lag * n_freq * n_time, 1), stream))
+    try:
+        _lib.check(lib().pycwb_cluster_pixels(C.byref(d), stream_ptr(stream)),
+                   "pycwb_cluster_pixels")
+    except Exception:
+        with _maps_lock:            # the map may not have been restored
+            _maps.pop(dev.index, None)
+        raise
     out["work"] = work
+    out["index_map"] = d.index_map
     stream.synchronize()
     out["lag_cluster_off"] = out["lag_cluster_off_dev"].cpu().numpy()
     out["n_clusters"] = int(out["lag_cluster_off"][-1])

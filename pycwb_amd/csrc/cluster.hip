@@ -11,11 +11,14 @@
 // components' smallest indices and its labels follow them in pixel order).
 //
 // Kernels (every lag of a call in one grid each):
+//   k_cl_fill     cell    (pycwb_cluster_map_init) the index map set to -1
 //   k_cl_init     pixel   lag of each pixel (binary search of lag_off), parent
 //                         and sort values set to the identity
-//   k_cl_fill     cell    the index map set to -1
 //   k_cl_scatter  pixel   atomicMax of the pixel index into its cell: the cell
 //                         holds its last pixel, as the reference's map does
+//                         (the caller's index map holds -1 everywhere on
+//                         entry: pycwb_cluster_map_init once, then every call
+//                         restores it)
 //   k_cl_union    pixel   lock-free union-find over the (2kf+1) x (2kt+1)
 //                         window; the larger root is linked under the smaller
 //                         by compare-and-swap, so a root is always the
@@ -27,8 +30,11 @@
 //   (sort)                stable radix sort of (cluster id, pixel index): the
 //                         members of each cluster in pixel order
 //   k_cl_members  slot    members and cluster offsets
+//   k_cl_unscatter pixel  the pixels' cells set back to -1
+//   k_cl_terms    slot    per-pixel subnet/subrho terms (kernels.py:150-189)
 //   k_cl_stats    cluster energy, c_time, c_freq (numpy's add.reduce order,
-//                         np_sum.hh), subnet and subrho (kernels.py:141-191)
+//                         np_sum.hh), subnet and subrho: the terms summed in
+//                         member order (kernels.py:141-191)
 // The scan and the sort are hipCUB's device-wide primitives.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -67,6 +73,7 @@ struct ClArgs {
   uint32_t* key_out;
   int32_t* val_in;
   int32_t* val_out;
+  double* terms;  // (n, 4) per-pixel subnet/subrho terms in member order
   // outputs
   int64_t* label;
   int64_t* lag_cluster_off;
@@ -152,6 +159,14 @@ __global__ void k_cl_scatter(ClArgs a) {
   atomicMax(a.map + cell(a, a.lag_of[i], f, t), (int32_t)i);
 }
 
+__global__ void k_cl_unscatter(ClArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int64_t f = a.freq[i], t = a.time[i];
+  if (!on_grid(a, f, t)) return;
+  a.map[cell(a, a.lag_of[i], f, t)] = -1;
+}
+
 __global__ void k_cl_union(ClArgs a) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.n) return;
@@ -229,6 +244,44 @@ struct FreqOf {  // f * (rate / 2)
   __device__ double operator()(int64_t k) const { return (double)f[m[k]] * s; }
 };
 
+// Per-pixel terms of _subnet_subrho_numba (kernels.py:150-189), one thread per
+// member slot p (pixel members[p], asnr = sqrt of its detector energies):
+// terms[4p .. 4p+3] = (rho term, e_tot - e_max, e_max, subnet term).  The
+// cluster sums add them in member order, as the reference's loop does.
+__global__ void k_cl_terms(ClArgs a) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= a.n) return;
+  const int64_t p = a.members[q];
+  const int n_ifo = a.n_ifo;
+  const double n_sub = a.n_sub;
+  double amp_sum = 0.0, e_max = 0.0, e_tot = 0.0, nsd = 0.0, msd = 0.0;
+  for (int d = 0; d < n_ifo; ++d) {
+    const double amp = sqrt(a.det_energy[p * n_ifo + d]);
+    const double x = amp * amp;
+    const double r = a.noise_rms ? a.noise_rms[p * n_ifo + d] : 1.0;
+    const double v = r * r;
+    amp_sum += fabs(amp);
+    if (x > e_max) {
+      e_max = x;
+      msd = v;
+    }
+    e_tot += x;
+    if (v > 0.0) nsd += 1.0 / v;
+  }
+  const double a_fac = amp_sum > 0.0 ? e_tot / (amp_sum * amp_sum) : 1.0;
+  const double y = e_tot - e_max;
+  const double x_corr = y * (1.0 + y / (e_max + 1.0e-5));
+  if (msd > 0.0) nsd -= 1.0 / msd;
+  const double v_corr = (2.0 * e_max - e_tot) * msd * nsd / 10.0;
+  const double a_cut = (x_corr + n_sub) != 0.0 ? x_corr / (x_corr + n_sub) : 0.0;
+  const double denom = x_corr + (v_corr > 0.0 ? v_corr : 1.0e-5);
+  double* t = a.terms + 4 * q;
+  t[0] = (1.0 - a_fac) * (e_tot - n_sub * 2.0);
+  t[1] = e_tot - e_max;
+  t[2] = e_max;
+  t[3] = (e_tot * x_corr / denom) * (a_cut > 0.5 ? a_cut : 0.0);
+}
+
 __global__ void k_cl_stats(ClArgs a) {
   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t total = a.rank[a.n];
@@ -243,37 +296,14 @@ __global__ void k_cl_stats(ClArgs a) {
     a.c_subrho[c] = 0.0;
     return;
   }
-  // kernels.py:141-191 with asnr = sqrt(pixel detector energy)
-  const int n_ifo = a.n_ifo;
-  const double n_sub = a.n_sub;
+  // kernels.py:141-191: the loop's four running sums over the pixel terms
   double rho = 0.0, e_sub_total = 0.0, e_max_sum = 0.0, acc = 0.0;
+  const double* t = a.terms + 4 * b;
   for (int64_t k = 0; k < m; ++k) {
-    const int64_t p = mem[k];
-    double amp_sum = 0.0, e_max = 0.0, e_tot = 0.0, nsd = 0.0, msd = 0.0;
-    for (int d = 0; d < n_ifo; ++d) {
-      const double amp = sqrt(a.det_energy[p * n_ifo + d]);
-      const double x = amp * amp;
-      const double r = a.noise_rms ? a.noise_rms[p * n_ifo + d] : 1.0;
-      const double v = r * r;
-      amp_sum += fabs(amp);
-      if (x > e_max) {
-        e_max = x;
-        msd = v;
-      }
-      e_tot += x;
-      if (v > 0.0) nsd += 1.0 / v;
-    }
-    const double a_fac = amp_sum > 0.0 ? e_tot / (amp_sum * amp_sum) : 1.0;
-    rho += (1.0 - a_fac) * (e_tot - n_sub * 2.0);
-    const double y = e_tot - e_max;
-    const double x_corr = y * (1.0 + y / (e_max + 1.0e-5));
-    if (msd > 0.0) nsd -= 1.0 / msd;
-    const double v_corr = (2.0 * e_max - e_tot) * msd * nsd / 10.0;
-    e_sub_total += e_tot - e_max;
-    e_max_sum += e_max;
-    const double a_cut = (x_corr + n_sub) != 0.0 ? x_corr / (x_corr + n_sub) : 0.0;
-    const double denom = x_corr + (v_corr > 0.0 ? v_corr : 1.0e-5);
-    acc += (e_tot * x_corr / denom) * (a_cut > 0.5 ? a_cut : 0.0);
+    rho += t[4 * k];
+    e_sub_total += t[4 * k + 1];
+    e_max_sum += t[4 * k + 2];
+    acc += t[4 * k + 3];
   }
   a.c_subnet[c] = acc / (e_max_sum + e_sub_total + 0.01);
   a.c_subrho[c] = rho >= 0.0 ? sqrt(rho) : __builtin_nan("");
@@ -305,7 +335,7 @@ int key_bits(int64_t n) {
 }
 
 struct Layout {
-  size_t map, lag_of, parent, root, flag, rank, key_in, key_out, val_in, val_out, temp;
+  size_t lag_of, parent, root, flag, rank, key_in, key_out, val_in, val_out, terms, temp;
   size_t temp_bytes, total;
 };
 
@@ -318,7 +348,6 @@ int layout(const pycwb_cluster_desc* d, Layout* L) {
     o = align_up(o + bytes);
     return at;
   };
-  L->map = take(sizeof(int32_t) * (size_t)cells);
   L->lag_of = take(sizeof(int32_t) * (size_t)n);
   L->parent = take(sizeof(int32_t) * (size_t)n);
   L->root = take(sizeof(int32_t) * (size_t)n);
@@ -328,6 +357,7 @@ int layout(const pycwb_cluster_desc* d, Layout* L) {
   L->key_out = take(sizeof(uint32_t) * (size_t)n);
   L->val_in = take(sizeof(int32_t) * (size_t)n);
   L->val_out = take(sizeof(int32_t) * (size_t)n);
+  L->terms = take(4 * sizeof(double) * (size_t)n);
   size_t scan_bytes = 0, sort_bytes = 0;
   if (n > 0) {
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t*)nullptr,
@@ -363,6 +393,15 @@ int check_desc(const pycwb_cluster_desc* d) {
 
 extern "C" {
 
+int pycwb_cluster_map_init(int32_t* index_map, int64_t cells, void* stream) {
+  if (!index_map || cells < 1) return PYCWB_ERR_ARG;
+  (void)hipGetLastError();
+  const int64_t blocks = (cells + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_cl_fill, dim3((unsigned)(blocks < 16384 ? blocks : 16384)), dim3(kBlock),
+                     0, (hipStream_t)stream, index_map, cells);
+  return check_launch("k_cl_fill");
+}
+
 int pycwb_cluster_work_bytes(const pycwb_cluster_desc* d, size_t* bytes) {
   if (!bytes) return PYCWB_ERR_ARG;
   int rc = check_desc(d);
@@ -378,7 +417,8 @@ int pycwb_cluster_pixels(const pycwb_cluster_desc* d, void* stream) {
   if (rc != PYCWB_OK) return rc;
   Layout L;
   if ((rc = layout(d, &L)) != PYCWB_OK) return rc;
-  if (d->n_pix > 0 && (!d->work || d->work_bytes < L.total)) return PYCWB_ERR_ARG;
+  if (d->n_pix > 0 && (!d->work || d->work_bytes < L.total || !d->index_map))
+    return PYCWB_ERR_ARG;
   const hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
   ClArgs a;
@@ -388,11 +428,12 @@ int pycwb_cluster_pixels(const pycwb_cluster_desc* d, void* stream) {
   a.n_ifo = d->n_ifo; a.n_freq = d->n_freq; a.n_time = d->n_time; a.kf = d->kf; a.kt = d->kt;
   a.dt = d->dt; a.half_rate = d->half_rate; a.n_sub = d->n_sub;
   char* w = (char*)d->work;
-  a.map = (int32_t*)(w + L.map); a.lag_of = (int32_t*)(w + L.lag_of);
+  a.map = d->index_map; a.lag_of = (int32_t*)(w + L.lag_of);
   a.parent = (int32_t*)(w + L.parent); a.root = (int32_t*)(w + L.root);
   a.flag = (int32_t*)(w + L.flag); a.rank = (int32_t*)(w + L.rank);
   a.key_in = (uint32_t*)(w + L.key_in); a.key_out = (uint32_t*)(w + L.key_out);
   a.val_in = (int32_t*)(w + L.val_in); a.val_out = (int32_t*)(w + L.val_out);
+  a.terms = (double*)(w + L.terms);
   a.label = d->label; a.lag_cluster_off = d->lag_cluster_off; a.cluster_off = d->cluster_off;
   a.members = d->members;
   a.c_energy = d->c_energy; a.c_time = d->c_time; a.c_freq = d->c_freq;
@@ -405,17 +446,14 @@ int pycwb_cluster_pixels(const pycwb_cluster_desc* d, void* stream) {
   const int64_t n = d->n_pix;
   const unsigned gp = (unsigned)((n + kBlock - 1) / kBlock);
   const unsigned gp1 = (unsigned)((n + kBlock) / kBlock);
-  const int64_t cells = (int64_t)d->n_lag * d->n_freq * d->n_time;
-  const int64_t fill_blocks = (cells + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_cl_init, dim3(gp), dim3(kBlock), 0, s, a);
   if ((rc = check_launch("k_cl_init")) != PYCWB_OK) return rc;
-  hipLaunchKernelGGL(k_cl_fill, dim3((unsigned)(fill_blocks < 16384 ? fill_blocks : 16384)),
-                     dim3(kBlock), 0, s, a.map, cells);
-  if ((rc = check_launch("k_cl_fill")) != PYCWB_OK) return rc;
   hipLaunchKernelGGL(k_cl_scatter, dim3(gp), dim3(kBlock), 0, s, a);
   if ((rc = check_launch("k_cl_scatter")) != PYCWB_OK) return rc;
   hipLaunchKernelGGL(k_cl_union, dim3(gp), dim3(kBlock), 0, s, a);
   if ((rc = check_launch("k_cl_union")) != PYCWB_OK) return rc;
+  hipLaunchKernelGGL(k_cl_unscatter, dim3(gp), dim3(kBlock), 0, s, a);
+  if ((rc = check_launch("k_cl_unscatter")) != PYCWB_OK) return rc;
   hipLaunchKernelGGL(k_cl_root, dim3(gp1), dim3(kBlock), 0, s, a);
   if ((rc = check_launch("k_cl_root")) != PYCWB_OK) return rc;
   size_t tb = L.temp_bytes;
@@ -435,6 +473,10 @@ int pycwb_cluster_pixels(const pycwb_cluster_desc* d, void* stream) {
     return rc;
   hipLaunchKernelGGL(k_cl_members, dim3(gp), dim3(kBlock), 0, s, a);
   if ((rc = check_launch("k_cl_members")) != PYCWB_OK) return rc;
+  if (d->n_ifo >= 2) {
+    hipLaunchKernelGGL(k_cl_terms, dim3(gp), dim3(kBlock), 0, s, a);
+    if ((rc = check_launch("k_cl_terms")) != PYCWB_OK) return rc;
+  }
   hipLaunchKernelGGL(k_cl_stats, dim3(gp), dim3(kBlock), 0, s, a);
   return check_launch("k_cl_stats");
 }

@@ -98,6 +98,7 @@ struct NpLeafSum {
 // np.sum of g(0), ..., g(n - 1), one thread.
 template <class G>
 __device__ double np_reduce(const G& g, int64_t n) {
+  if (n <= kNpLeaf) return 0.0 + np_leaf(g, 0, n);  // one chunk that is one leaf
   double r = 0.0;
   for (int64_t s = 0; s < n; s += kNpChunk) {
     NpLeafSum<G> leaf{g, s};

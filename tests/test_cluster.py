@@ -335,3 +335,26 @@ def test_fragment_select_and_remove_rejected():
     assert len(fc.clusters) == 1
     with pytest.raises(ValueError):
         fc.select("likelihood", 1.0)
+
+
+@pytest.mark.gpu
+def test_gpu_cluster_index_map_is_restored(oracle):
+    """The cached index map is left all -1 by every call: A, B (other pixels,
+    same shape), then A again give the oracle's answer each time."""
+    from pycwb_amd.coherence_native.clustering import _n_sub
+
+    print("case cluster index map reuse", flush=True)
+    rng = np.random.default_rng(21)
+    n_freq, n_time, rate = 33, 400, 32.0
+    runs = []
+    for seed in (1, 2):
+        f, t = _random_pixels(np.random.default_rng(seed), n_freq, n_time, 900)
+        n = f.size
+        runs.append({"lag_off": np.array([0, n]), "frequency": f, "time": t,
+                     "energy": rng.exponential(5.0, n),
+                     "pix_det_energy": rng.exponential(2.0, (n, 3))})
+    for sel in (runs[0], runs[1], runs[0]):
+        _, g = _gpu_run(sel, n_freq, n_time, 1, 1, rate)
+        o = oracle.cluster_run(sel["frequency"], sel["time"], sel["energy"],
+                               sel["pix_det_energy"], n_freq, n_time, 1, 1, rate, _n_sub(3))
+        _assert_same_as_oracle(g, o)
