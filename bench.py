@@ -20,8 +20,10 @@ argmax then 20 projected-gradient steps of the continuous sky statistic) and
 "select" (network-pixel selection of 128 lags of 5 detectors' 600 s maps),
 "maxenergy" (time-delay max-energy map of a 600 s series, 87 shifted copies)
 "cluster" (connected-component clustering of the 128 lags' selected
-pixels) and "threshold" (the pixel-energy threshold's array statistics of the
-same maps).
+pixels), "threshold" (the pixel-energy threshold's array statistics of the
+same maps) and "e2e" (5 detectors' 600 s with 200 injections through 7 WDM
+levels of max-energy, threshold, selection and clustering into one likelihood
+batch at HEALPix order 8).
 
 Prints one JSON line.
 """
@@ -699,6 +701,195 @@ def threshold_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, cpu=True, 
     return res
 
 
+def _inject_sine_gaussians(x, ifos, rate, n_inj, seed):
+    """Linearly polarised sine-Gaussians (Q = 9, f0 uniform in 50-1000 Hz,
+    network SNR uniform in 8-40, uniform-sphere directions, times uniform in
+    the series' middle) added to the detectors' white unit-variance series x
+    (n, L), in place.  Returns (times, f0s, directions (phi, lat))."""
+    from pycwb_amd import sky
+
+    rng = np.random.default_rng(seed)
+    dets = [sky.Detector(n) for n in ifos]
+    L = x.shape[1]
+    times = np.sort(rng.uniform(20.0, L / rate - 20.0, n_inj))
+    f0s = rng.uniform(50.0, 1000.0, n_inj)
+    snrs = rng.uniform(8.0, 40.0, n_inj)
+    phis = rng.uniform(0.0, 2 * np.pi, n_inj)
+    lats = np.arcsin(rng.uniform(-1.0, 1.0, n_inj))
+    for k in range(n_inj):
+        tau = 9.0 / (np.sqrt(2.0) * np.pi * f0s[k])
+        nh = np.array([np.cos(lats[k]) * np.cos(phis[k]), np.cos(lats[k]) * np.sin(phis[k]),
+                       np.sin(lats[k])])
+        i0 = int((times[k] - 8 * tau) * rate)
+        i1 = int((times[k] + 8 * tau) * rate) + 1
+        t = np.arange(i0, i1) / rate
+        hs = []
+        for d in dets:
+            shift = float(nh @ (d.vertex - dets[0].vertex) / sky.C_LIGHT)
+            fp = float(d.antenna_pattern_gha(np.array([-phis[k]]), np.array([lats[k]]))[0][0])
+            tt = t + shift - times[k]
+            hs.append(fp * np.exp(-(tt / tau) ** 2) * np.sin(2 * np.pi * f0s[k] * tt))
+        scale = snrs[k] / np.sqrt(sum(float(h @ h) for h in hs))
+        for i, h in enumerate(hs):
+            x[i, i0:i1] += scale * h
+    return times, f0s, np.stack([phis, lats], axis=1)
+
+
+def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, td_size=12,
+            bpp=0.001, seg_edge=2.0, max_delay_s=0.04, pattern=3):
+    """The end-to-end case: 5 detectors (H1 L1 V1 K1 I1), 600 s of white
+    (whitened) noise at 4096 Hz with 200 sine-Gaussian injections, 7 WDM
+    levels (M = 8 .. 512).  Per level, in the order of coherence_native/
+    setup.py and pipeline.py: t2w of the 5 series, time-delay max-energy maps
+    (pattern 3, 40 ms, every 4th sample) with the host Gamma-to-Gauss,
+    compute_threshold (bpp 0.001, shape path), zero-lag pixel selection and
+    clustering (kt = 2, kf = 3); then TD amplitudes of every cluster of every
+    level straight into one likelihood batch (K = 598 at 16384 Hz) and the
+    batched regulator + scan + argmax at HEALPix order 8 (786,432 directions).
+    Wall time per stage (device synchronised at each stage's end); the
+    super-clustering across levels is not part of this build, so every level's
+    clusters go to the likelihood.  Found: injections with a cluster within
+    0.1 s whose argmax direction reproduces the injected detector delays to
+    1 ms."""
+    import torch
+
+    from pycwb_amd import sky, synth
+    from pycwb_amd.batch import BatchedSkyScan, ClusterBatch, net_tables
+    from pycwb_amd.coherence_native.clustering import cluster_pixels_device
+    from pycwb_amd.coherence_native.projection import gamma_to_gauss, max_energy_device
+    from pycwb_amd.coherence_native.selection import (_build_selection_cache,
+                                                      select_network_pixels_device)
+    from pycwb_amd.coherence_native.veto_threshold import compute_threshold
+    from pycwb_amd.tdamp import TDBatchInputs
+    from pycwb_amd.wdm import WDM
+
+    from types import SimpleNamespace
+
+    dev = torch.device("cuda", 0)
+    ifos = synth.NETWORK_5
+    n = len(ifos)
+    L = seconds * rate
+    t0 = time.perf_counter()
+    x = np.random.default_rng(11).normal(size=(n, L))
+    inj_t, inj_f, inj_dir = _inject_sine_gaussians(x, ifos, rate, n_inj, 12)
+    xd = torch.from_numpy(x).to(dev)
+    torch.cuda.synchronize()
+    stages = {"data": time.perf_counter() - t0}
+    td_rate = rate * up_tdf
+    K = int(math.ceil(sky.max_network_delay(ifos, ifos[0]) * td_rate)) + 1
+    net = synth.make_network(ifos, order=order, K=K, td_rate=td_rate)
+    nt = net_tables([net], device=dev)
+    add = lambda k, dt: stages.__setitem__(k, stages.get(k, 0.0) + dt)
+    levels = []
+    for M in (8, 16, 32, 64, 128, 256, 512):
+        w = WDM(M, M, 6, 10)
+        t = time.perf_counter()
+        tf = w.t2w_device(xd)                                       # (n, M+1, N)
+        torch.cuda.synchronize()
+        add("t2w", time.perf_counter() - t)
+        dt_bin, df = M / rate, rate / 2.0 / M
+        t = time.perf_counter()
+        energy = [max_energy_device(xd[i], w, int(max_delay_s * rate), 4, pattern, seg_edge,
+                                    int(round(1 / dt_bin)), 32.0, 2000.0, df)
+                  for i in range(n)]
+        host = [e.t().contiguous().cpu().numpy() for e in energy]
+        add("max_energy_gpu", time.perf_counter() - t)
+        t = time.perf_counter()
+        maps, alp = [], 0.0
+        for e in host:
+            g, a = gamma_to_gauss(e, seg_edge, int(round(1 / dt_bin)))
+            maps.append(SimpleNamespace(data=g, dt=dt_bin, df=df, wavelet_rate=int(round(1 / dt_bin)),
+                                        start=0.0, stop=float(seconds), f_low=32.0,
+                                        f_high=2000.0))
+            alp += a
+        add("gamma_to_gauss_host", time.perf_counter() - t)
+        t = time.perf_counter()
+        Eo = compute_threshold(maps, bpp, alp=alp / n, edge=seg_edge)
+        torch.cuda.synchronize()
+        add("threshold", time.perf_counter() - t)
+        t = time.perf_counter()
+        cache = _build_selection_cache(maps, edge=seg_edge, device=dev)
+        sel = select_network_pixels_device(cache, Eo, np.zeros((1, n), np.int64), stream=stream)
+        torch.cuda.synchronize()
+        add("selection", time.perf_counter() - t)
+        t = time.perf_counter()
+        cl = cluster_pixels_device(sel, kt=2, kf=3, rate=cache["rate"], stream=stream)
+        nc = cl["n_clusters"]
+        np_ = int(sel["n_pix"])
+        off = cl["cluster_off"][:nc + 1].cpu().numpy()
+        mem = cl["members"][:np_]
+        pix = (sel["time"][mem] * (M + 1) + sel["frequency"][mem]).to(torch.int32)
+        ctime = cl["c_time"][:nc].cpu().numpy()
+        torch.cuda.synchronize()
+        add("clustering", time.perf_counter() - t)
+        levels.append({"M": M, "w": w, "tf": tf, "pix": pix, "sizes": np.diff(off), "ctime": ctime,
+                       "Eo": Eo, "selected": np_})
+        log(f"e2e level M={M}: Eo={Eo:.3f}, {np_} pixels selected, {nc} clusters")
+    # one likelihood batch: every cluster of every level
+    t = time.perf_counter()
+    sizes = np.concatenate([lv["sizes"] for lv in levels]).astype(np.int64)
+    C_ = sizes.size
+    nd = 2 * K + 1
+    rms_off = np.concatenate([[0], np.cumsum(sizes * n)[:-1]]).astype(np.int64)
+    td_off = np.concatenate([[0], np.cumsum(sizes * n * nd)[:-1]]).astype(np.int64)
+    td = torch.empty(2 * int(sizes.sum()) * n * nd, dtype=torch.float32, device=dev)
+    rms = torch.full((int(sizes.sum()) * n,), 1.0 / math.sqrt(n), dtype=torch.float32,
+                     device=dev)
+    c0 = 0
+    for lv in levels:
+        bank = lv["w"].set_td_filter(td_size, up_tdf)
+        if lv["sizes"].size:
+            for i in range(n):
+                inp = TDBatchInputs.from_device(lv["tf"][i], bank)
+                inp.extract_td_into(lv["pix"], K, td, int(td_off[c0]), i, n, stream=stream)
+        c0 += lv["sizes"].size
+        lv["tf"] = None
+    batch = ClusterBatch(n_pix=sizes.astype(np.int32), net=np.zeros(C_, np.int32),
+                         K=np.full(C_, K, np.int32), nifo=np.full(C_, n, np.int32),
+                         rms_off=rms_off, td_off=td_off, rms=rms, td=td)
+    torch.cuda.synchronize()
+    add("td_amplitudes", time.perf_counter() - t)
+    valid = torch.arange(net.n_sky, dtype=torch.int64, device=dev)
+    scan = BatchedSkyScan(batch, nt, valid, 0.5 * 0.5 * 2 / 3, 0.5, 0.5, block=64)
+    scan.launch(stream)                                            # warm (graph build)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    scan.launch(stream)
+    torch.cuda.synchronize()
+    add("likelihood_batch", time.perf_counter() - t)
+    res = scan.results()
+    scan.close()
+    # recovery: injections with a cluster within 0.1 s whose argmax delays match
+    ctimes = np.concatenate([lv["ctime"] for lv in levels])
+    ml = net.ml
+    found = 0
+    for k in range(n_inj):
+        near = np.nonzero(np.abs(ctimes - inj_t[k]) < 0.1)[0]
+        if near.size == 0:
+            continue
+        phi, lat = inj_dir[k]
+        nh = np.array([np.cos(lat) * np.cos(phi), np.cos(lat) * np.sin(phi), np.sin(lat)])
+        ref = sky.Detector(ifos[0]).vertex
+        dl = np.array([nh @ (sky.Detector(d).vertex - ref) / sky.C_LIGHT for d in ifos])
+        rec = (ml[:, res["l_max"][near]] - ml[0, res["l_max"][near]]) / td_rate
+        if np.any(np.all(np.abs(rec - dl[:, None]) < 1e-3, axis=0)):
+            found += 1
+    total = sum(v for k, v in stages.items() if k != "data")
+    evals = int(sizes.sum()) * net.n_sky
+    out = {"detectors": n, "seconds": seconds, "rate": rate, "injections": n_inj,
+           "levels": [{"M": lv["M"], "Eo": round(lv["Eo"], 4), "selected": lv["selected"],
+                       "clusters": int(lv["sizes"].size)} for lv in levels],
+           "clusters": C_, "pixels": int(sizes.sum()), "healpix_order": order, "K": K,
+           "likelihood_evals": evals,
+           "likelihood_evals_per_s": evals / stages["likelihood_batch"],
+           "stage_s": {k: round(v, 4) for k, v in stages.items()},
+           "wall_s_without_data": round(total, 3),
+           "injections_found_with_delays": found}
+    log(f"e2e: {C_} clusters ({int(sizes.sum())} px) from 7 levels in {total:.2f} s "
+        f"(stages {out['stage_s']}); {found}/{n_inj} injections recovered with their delays")
+    return out
+
+
 def maxenergy_leg(stream, log, M=64, seconds=600, rate=4096, max_delay_s=0.042, ds=4,
                   pattern=3, cpu=True):
     """Time-delay max-energy map of one detector's 600 s series at M = 64:
@@ -778,8 +969,9 @@ def main():
     ap.add_argument("--no-maxenergy", action="store_true")
     ap.add_argument("--no-cluster", action="store_true")
     ap.add_argument("--no-threshold", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk", "select",
-                                       "maxenergy", "cluster", "threshold"), default=None,
+                                       "maxenergy", "cluster", "threshold", "e2e"), default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="first copy this many GiB device-to-device (counter calibration)")
@@ -825,6 +1017,9 @@ def main():
     if args.only == "select":
         print(json.dumps({"select": select_leg(torch.cuda.current_stream(), log,
                                                cpu=not args.no_cpu)}), flush=True)
+        return 0
+    if args.only == "e2e":
+        print(json.dumps({"e2e": e2e_leg(torch.cuda.current_stream(), log)}), flush=True)
         return 0
     if args.only == "threshold":
         print(json.dumps({"threshold": threshold_leg(torch.cuda.current_stream(), log,
@@ -957,6 +1152,7 @@ def main():
     maxen = None if args.no_maxenergy else maxenergy_leg(stream, log, cpu=not args.no_cpu)
     clus = None if args.no_cluster else cluster_leg(stream, log, cpu=not args.no_cpu)
     thr = None if args.no_threshold else threshold_leg(stream, log, cpu=not args.no_cpu)
+    e2e = None if args.no_e2e else e2e_leg(stream, log)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
@@ -1002,6 +1198,7 @@ def main():
         "maxenergy": maxen,
         "cluster": clus,
         "threshold": thr,
+        "e2e": e2e,
     }
     print(json.dumps(out), flush=True)
     bs.close()

@@ -53,6 +53,24 @@ class TDBatchInputs:
                    M=int(td_filters.M), n_coeffs=nc, J=J,
                    trig=up(np.stack([np.sin(q), np.cos(q)], axis=1), torch.float64))
 
+    @classmethod
+    def from_device(cls, data: torch.Tensor, td_filters) -> "TDBatchInputs":
+        """``data``: complex128 (M+1, n_time) device map (a t2w_device output
+        slice); the padded planes are built on the device."""
+        dev = data.device
+        nc = int(td_filters.n_coeffs)
+        planes = torch.view_as_real(data.transpose(0, 1).contiguous())   # (n_time, M+1, 2)
+        pad = torch.zeros((nc, planes.shape[1], 2), dtype=planes.dtype, device=dev)
+        padded = torch.cat([pad, planes, pad], dim=0).to(torch.float32)
+        J = int(td_filters.max_delay)
+        q = np.arange(4 * J) * (np.pi / (2.0 * J))
+        up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device=dev,
+                                                                  dtype=torch.float64)
+        return cls(padded00=padded[..., 0].contiguous(), padded90=padded[..., 1].contiguous(),
+                   T0=up(np.asarray(td_filters.T0).T), Tx=up(np.asarray(td_filters.Tx).T),
+                   M=int(td_filters.M), n_coeffs=nc, J=J,
+                   trig=up(np.stack([np.sin(q), np.cos(q)], axis=1)))
+
     def _check_pixels(self, pixel_indices) -> torch.Tensor:
         dev = self.padded00.device
         pix = torch.as_tensor(np.asarray(pixel_indices.cpu() if isinstance(pixel_indices,

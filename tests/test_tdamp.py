@@ -167,3 +167,25 @@ def test_gpu_td_chain_delay_property():
         err = max(np.abs(td[keep00, K + d] - s.real[m, n][keep00]).max(),
                   np.abs(td[keep90, 3 * K + 1 + d] - s.imag[m, n][keep90]).max()) / scale
         assert err < 2e-4, (d, err)
+
+
+@pytest.mark.gpu
+def test_gpu_td_inputs_from_device_map_match_host_map():
+    """TDBatchInputs.from_device (planes padded on the device from a t2w_device
+    output) gives the same planes and amplitudes as from_map on the host copy."""
+    import torch
+
+    from pycwb_amd.tdamp import TDBatchInputs
+    from pycwb_amd.wdm import WDM
+
+    print("case td inputs from device map", flush=True)
+    M, nc, K = 16, 12, 40
+    w = WDM(M, M, 6, 10)
+    bank = w.set_td_filter(nc, 2)
+    x = torch.from_numpy(_signal(4096)).to("cuda:0")
+    tf_dev = w.t2w_device(x)[0]                       # (M+1, N) complex128
+    a = TDBatchInputs.from_device(tf_dev, bank)
+    b = TDBatchInputs.from_map(tf_dev.cpu().numpy(), bank, torch.device("cuda", 0))
+    assert torch.equal(a.padded00, b.padded00) and torch.equal(a.padded90, b.padded90)
+    pix = np.arange(0, tf_dev.shape[1] * (M + 1), 7, dtype=np.int32)
+    assert torch.equal(a.extract_td_vecs(pix, K), b.extract_td_vecs(pix, K))
