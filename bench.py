@@ -741,9 +741,10 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
     (whitened) noise at 4096 Hz with 200 sine-Gaussian injections, 7 WDM
     levels (M = 8 .. 512).  Per level, in the order of coherence_native/
     setup.py and pipeline.py: t2w of the 5 series, time-delay max-energy maps
-    (pattern 3, 40 ms, every 4th sample) with the host Gamma-to-Gauss,
+    (pattern 3, 40 ms, every 4th sample) and their Gamma-to-Gauss,
     compute_threshold (bpp 0.001, shape path), zero-lag pixel selection and
-    clustering (kt = 2, kf = 3); then TD amplitudes of every cluster of every
+    clustering (kt = 2, kf = 3), the maps staying on the device; then TD
+    amplitudes of every cluster of every
     level straight into one likelihood batch (K = 598 at 16384 Hz) and the
     batched regulator + scan + argmax at HEALPix order 8 (786,432 directions).
     Wall time per stage (device synchronised at each stage's end); the
@@ -756,14 +757,12 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
     from pycwb_amd import sky, synth
     from pycwb_amd.batch import BatchedSkyScan, ClusterBatch, net_tables
     from pycwb_amd.coherence_native.clustering import cluster_pixels_device
-    from pycwb_amd.coherence_native.projection import gamma_to_gauss, max_energy_device
-    from pycwb_amd.coherence_native.selection import (_build_selection_cache,
-                                                      select_network_pixels_device)
-    from pycwb_amd.coherence_native.veto_threshold import compute_threshold
+    from pycwb_amd.coherence_native.projection import gamma_to_gauss_device, max_energy_device
+    from pycwb_amd.coherence_native.selection import (select_network_pixels_device,
+                                                      selection_cache_device)
+    from pycwb_amd.coherence_native.veto_threshold import compute_threshold_device
     from pycwb_amd.tdamp import TDBatchInputs
     from pycwb_amd.wdm import WDM
-
-    from types import SimpleNamespace
 
     dev = torch.device("cuda", 0)
     ifos = synth.NETWORK_5
@@ -782,7 +781,10 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
     add = lambda k, dt: stages.__setitem__(k, stages.get(k, 0.0) + dt)
     levels = []
     for M in (8, 16, 32, 64, 128, 256, 512):
-        w = WDM(M, M, 6, 10)
+        t = time.perf_counter()
+        w = WDM(M, M, 6, 10)                 # filters of the level: per-job setup
+        w.set_td_filter(td_size, up_tdf)
+        add("setup", time.perf_counter() - t)
         t = time.perf_counter()
         tf = w.t2w_device(xd)                                       # (n, M+1, N)
         torch.cuda.synchronize()
@@ -792,23 +794,25 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
         energy = [max_energy_device(xd[i], w, int(max_delay_s * rate), 4, pattern, seg_edge,
                                     int(round(1 / dt_bin)), 32.0, 2000.0, df)
                   for i in range(n)]
-        host = [e.t().contiguous().cpu().numpy() for e in energy]
-        add("max_energy_gpu", time.perf_counter() - t)
+        torch.cuda.synchronize()
+        add("max_energy", time.perf_counter() - t)
         t = time.perf_counter()
         maps, alp = [], 0.0
-        for e in host:
-            g, a = gamma_to_gauss(e, seg_edge, int(round(1 / dt_bin)))
-            maps.append(SimpleNamespace(data=g, dt=dt_bin, df=df, wavelet_rate=int(round(1 / dt_bin)),
-                                        start=0.0, stop=float(seconds), f_low=32.0,
-                                        f_high=2000.0))
+        for e in energy:
+            g, a = gamma_to_gauss_device(e, seg_edge, int(round(1 / dt_bin)), stream=stream)
+            maps.append(g)
             alp += a
-        add("gamma_to_gauss_host", time.perf_counter() - t)
+        stack = torch.stack([g.t() for g in maps]).contiguous()      # (n, M+1, N)
+        torch.cuda.synchronize()
+        add("gamma_to_gauss", time.perf_counter() - t)
         t = time.perf_counter()
-        Eo = compute_threshold(maps, bpp, alp=alp / n, edge=seg_edge)
+        Eo = compute_threshold_device(stack, bpp, alp=alp / n, edge=seg_edge,
+                                      wavelet_rate=int(round(1 / dt_bin)))
         torch.cuda.synchronize()
         add("threshold", time.perf_counter() - t)
         t = time.perf_counter()
-        cache = _build_selection_cache(maps, edge=seg_edge, device=dev)
+        cache = selection_cache_device(stack, dt_bin, df, edge=seg_edge, f_low=32.0,
+                                       f_high=2000.0, stop=float(seconds))
         sel = select_network_pixels_device(cache, Eo, np.zeros((1, n), np.int64), stream=stream)
         torch.cuda.synchronize()
         add("selection", time.perf_counter() - t)
@@ -837,7 +841,7 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
                      device=dev)
     c0 = 0
     for lv in levels:
-        bank = lv["w"].set_td_filter(td_size, up_tdf)
+        bank = lv["w"].td_filters
         if lv["sizes"].size:
             for i in range(n):
                 inp = TDBatchInputs.from_device(lv["tf"][i], bank)
@@ -858,6 +862,16 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
     torch.cuda.synchronize()
     add("likelihood_batch", time.perf_counter() - t)
     res = scan.results()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    scan.launch_stage(0, stream)
+    ev[0].record(stream)
+    scan.launch_stage(1, stream)
+    ev[1].record(stream)
+    scan.launch_stage(2, stream)
+    ev[2].record(stream)
+    scan.launch_stage(3, stream)
+    torch.cuda.synchronize()
+    stage_ms = {"regulator": ev[0].elapsed_time(ev[1]), "scan": ev[1].elapsed_time(ev[2])}
     scan.close()
     # recovery: injections with a cluster within 0.1 s whose argmax delays match
     ctimes = np.concatenate([lv["ctime"] for lv in levels])
@@ -874,7 +888,7 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
         rec = (ml[:, res["l_max"][near]] - ml[0, res["l_max"][near]]) / td_rate
         if np.any(np.all(np.abs(rec - dl[:, None]) < 1e-3, axis=0)):
             found += 1
-    total = sum(v for k, v in stages.items() if k != "data")
+    total = sum(v for k, v in stages.items() if k not in ("data", "setup"))
     evals = int(sizes.sum()) * net.n_sky
     out = {"detectors": n, "seconds": seconds, "rate": rate, "injections": n_inj,
            "levels": [{"M": lv["M"], "Eo": round(lv["Eo"], 4), "selected": lv["selected"],
@@ -883,7 +897,8 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
            "likelihood_evals": evals,
            "likelihood_evals_per_s": evals / stages["likelihood_batch"],
            "stage_s": {k: round(v, 4) for k, v in stages.items()},
-           "wall_s_without_data": round(total, 3),
+           "likelihood_stage_ms": {k: round(v, 2) for k, v in stage_ms.items()},
+           "wall_s_without_data_and_setup": round(total, 3),
            "injections_found_with_delays": found}
     log(f"e2e: {C_} clusters ({int(sizes.sum())} px) from 7 levels in {total:.2f} s "
         f"(stages {out['stage_s']}); {found}/{n_inj} injections recovered with their delays")

@@ -598,6 +598,47 @@ typedef struct pycwb_threshold_desc {
 int pycwb_threshold_work_bytes(const pycwb_threshold_desc* desc, size_t* bytes);
 int pycwb_threshold_stats(const pycwb_threshold_desc* desc, void* stream);
 
+/* ---- Gamma-to-Gauss of a max-energy map ----
+ * Replaces the array work of TimeFrequencyMap.Gamma2Gauss
+ * (pycwb/types/time_frequency_map.py:143-246, called by
+ * time_delay_max_energy_numba, coherence_native/time_delay_numba.py:664);
+ * the Gamma fit on the resulting scalars stays on the host.  flat: the map
+ * time-major (n = layers * time bins, flat[t * layers + f]), region
+ * [nL, nR), window [nL, nR].  Phases, in order, each after the host has read
+ * the previous one's results:
+ *   1: count[0] = #(flat > 0.001); sorted = the window of flat, ascending
+ *      (the median is sorted[split - nL]);
+ *   2: the region's values in (lo, hi), in order (*n_valid of them), summed
+ *      per 8192-value chunk in numpy's order with their logs beside
+ *      (chunk_sums, 2 per chunk);
+ *   3: out = amp - ALP (1 + log(amp / ALP)) where amp = flat * alp / avr >= ALP,
+ *      else 0; count[1] = #(out[nL .. n - nL) > 1e-5); sorted = the window of
+ *      out, ascending;
+ *   4: out *= scale.
+ * work: at least pycwb_g2g_work_bytes.  n < 2^31 - 1. */
+typedef struct pycwb_g2g_desc {
+  const double* flat;
+  int64_t n;
+  int64_t nL;
+  int64_t nR;
+  void* work;
+  size_t work_bytes;
+  uint64_t* count;
+  double* sorted;
+  double lo;
+  double hi;
+  double* chunk_sums;
+  int64_t* n_valid;
+  double alp;
+  double avr;
+  double ALP;
+  double scale;
+  double* out;
+} pycwb_g2g_desc;
+
+int pycwb_g2g_work_bytes(const pycwb_g2g_desc* desc, size_t* bytes);
+int pycwb_g2g_phase(const pycwb_g2g_desc* desc, int phase, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -147,6 +147,14 @@ class ThresholdDesc(C.Structure):
                 ("counts", vp), ("sorted", vp), ("chunk_sums", vp)]
 
 
+class G2gDesc(C.Structure):
+    _fields_ = [("flat", vp), ("n", C.c_int64), ("nL", C.c_int64), ("nR", C.c_int64),
+                ("work", vp), ("work_bytes", C.c_size_t), ("count", vp), ("sorted", vp),
+                ("lo", C.c_double), ("hi", C.c_double), ("chunk_sums", vp), ("n_valid", vp),
+                ("alp", C.c_double), ("avr", C.c_double), ("ALP", C.c_double),
+                ("scale", C.c_double), ("out", vp)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "pycwb_version": (C.c_char_p, []),
@@ -186,6 +194,8 @@ SIGNATURES = {
     "pycwb_cluster_pixels": (i32, [C.POINTER(ClusterDesc), vp]),
     "pycwb_threshold_work_bytes": (i32, [C.POINTER(ThresholdDesc), C.POINTER(C.c_size_t)]),
     "pycwb_threshold_stats": (i32, [C.POINTER(ThresholdDesc), vp]),
+    "pycwb_g2g_work_bytes": (i32, [C.POINTER(G2gDesc), C.POINTER(C.c_size_t)]),
+    "pycwb_g2g_phase": (i32, [C.POINTER(G2gDesc), i32, vp]),
 }
 
 

@@ -9,6 +9,8 @@ coherence_native.selection (selection.py:14-243).
                                           (pycwb_select_count / _emit)
   select_network_pixels_device            the same, the pixels left on the
                                           device (clustering's input)
+  selection_cache_device                  the cache of maps already on the
+                                          device
 
 The energy maps are the real parts of the detectors' TF maps (the
 max-energy maps the coherence setup produces; veto_threshold.py:75-95).
@@ -55,15 +57,32 @@ def _build_selection_cache(tf_maps, edge: float = 0.0, lag_shifts_by_lag=None,
     n_freq, n_time = arrays[0].shape
     if any(a.shape != (n_freq, n_time) for a in arrays[1:]):
         raise ValueError("all detector TF maps must have the same shape")
-    dt = float(tf_maps[0].dt)
+    device = device or require_gpu()
+    tf0 = tf_maps[0]
+    stack = torch.from_numpy(np.ascontiguousarray(np.stack(arrays))).to(device)
+    return selection_cache_device(
+        stack, float(tf0.dt), float(getattr(tf0, "df", 0.0) or 0.0), edge=edge,
+        f_low=getattr(tf0, "f_low", 0.0), f_high=getattr(tf0, "f_high", None),
+        start=float(getattr(tf0, "start", 0.0)), stop=float(getattr(tf0, "stop", 0.0)),
+        lag_shifts_by_lag=lag_shifts_by_lag,
+        f_high_out=float(getattr(tf0, "f_high", (n_freq - 1) * float(getattr(tf0, "df", 0.0)
+                                                                      or 0.0)) or 0.0))
+
+
+def selection_cache_device(stack: torch.Tensor, dt: float, df: float, edge: float = 0.0,
+                           f_low=0.0, f_high=None, start: float = 0.0, stop: float = 0.0,
+                           lag_shifts_by_lag=None, f_high_out=None) -> dict:
+    """The selection cache of maps already on the device: ``stack`` float64
+    (n_ifo, n_freq, n_time) energy maps; the band, edge and lag bins as
+    _build_selection_cache computes them (selection.py:31-122); ``f_high_out``
+    is the payload's f_high (the band edge when not given)."""
+    n_ifo, n_freq, n_time = (int(v) for v in stack.shape)
     if dt <= 0.0:
         raise ValueError("tf_map.dt must be positive")
     rate = 1.0 / dt
     edge_bins = int(max(0, float(edge) * rate + 0.001))
-    f_low = float(getattr(tf_maps[0], "f_low", 0.0) or 0.0)
-    f_high_attr = getattr(tf_maps[0], "f_high", None)
-    df = float(getattr(tf_maps[0], "df", 0.0) or 0.0)
-    f_high = float(f_high_attr) if f_high_attr is not None else float((n_freq - 1) * df)
+    f_low = float(f_low or 0.0)
+    f_high = float(f_high) if f_high is not None else float((n_freq - 1) * df)
     ib, ie = 1, n_freq
     if df > 0:
         for idx_f, freq in enumerate(np.arange(n_freq, dtype=np.float64) * df):
@@ -76,17 +95,15 @@ def _build_selection_cache(tf_maps, edge: float = 0.0, lag_shifts_by_lag=None,
     shifts = None
     if lag_shifts_by_lag is not None:
         rows = list(lag_shifts_by_lag)
-        shifts = (np.vstack([_shift_bins_from_lag_shifts(r, len(arrays), rate) for r in rows])
-                  if rows else np.empty((0, len(arrays)), dtype=np.int64))
-    device = device or require_gpu()
-    tf0 = tf_maps[0]
+        shifts = (np.vstack([_shift_bins_from_lag_shifts(r, n_ifo, rate) for r in rows])
+                  if rows else np.empty((0, n_ifo), dtype=np.int64))
     return {
-        "arrays_stack": torch.from_numpy(np.ascontiguousarray(np.stack(arrays))).to(device),
-        "n_ifo": len(arrays), "n_freq": n_freq, "n_time": n_time, "dt": dt, "rate": rate,
+        "arrays_stack": stack.to(torch.float64).contiguous(),
+        "n_ifo": n_ifo, "n_freq": n_freq, "n_time": n_time, "dt": dt, "rate": rate,
         "edge_bins": edge_bins, "valid_start": edge_bins, "valid_stop": n_time - edge_bins,
         "nn_valid": n_time - 2 * edge_bins, "ib": ib, "ie": ie,
-        "start": float(getattr(tf0, "start", 0.0)), "stop": float(getattr(tf0, "stop", 0.0)),
-        "f_low": f_low, "f_high": float(getattr(tf0, "f_high", (n_freq - 1) * df) or 0.0),
+        "start": start, "stop": stop, "f_low": f_low,
+        "f_high": f_high if f_high_out is None else float(f_high_out),
         "shift_bins_by_lag": shifts,
     }
 

@@ -82,20 +82,47 @@ def compute_threshold(tf_maps, bpp: float, alp: float | None = None,
     arithmetic."""
     if not tf_maps or not hasattr(tf_maps[0], "data"):
         raise ValueError("compute_threshold requires TF maps")
-    n_ifo = len(tf_maps)
     dev = require_gpu()
     arrays = [_real64(t.data) for t in tf_maps]
+    two_d = arrays[0].ndim == 2
     arrays = [a if a.ndim == 2 else a.reshape(1, -1) for a in arrays]
     if any(a.shape != arrays[0].shape for a in arrays[1:]):
         raise ValueError("all detector TF maps must have the same shape")
-    n_freq, n_time = arrays[0].shape
     stack = torch.from_numpy(np.ascontiguousarray(np.stack(arrays))).to(dev)
+    dt = None
+    if alp is None and two_d and edge is not None:
+        n_time = arrays[0].shape[1]
+        crops = set()
+        for t in tf_maps:
+            e = 0
+            if hasattr(t, "dt") and t.dt > 0:
+                e = int(max(0, round(float(edge) / float(t.dt))))
+                if not (e > 0 and n_time > 2 * e):
+                    e = 0
+            crops.add(e)
+        if len(crops) > 1:
+            raise ValueError("detector TF maps are cropped to different shapes")
+        dt = float(tf_maps[0].dt) if getattr(tf_maps[0], "dt", 0) and tf_maps[0].dt > 0 \
+            else None
+    return compute_threshold_device(stack, bpp, alp=alp, edge=edge,
+                                    wavelet_rate=getattr(tf_maps[0], "wavelet_rate", None),
+                                    dt=dt, two_d=two_d)
+
+
+def compute_threshold_device(stack: torch.Tensor, bpp: float, alp: float | None = None,
+                             edge: float | None = None, wavelet_rate=None, dt=None,
+                             two_d: bool = True) -> float:
+    """compute_threshold on maps already on the device: ``stack`` float64
+    (n_ifo, layers, time bins) (a 1-D map as one row, ``two_d`` False);
+    ``wavelet_rate`` for the shape (alp) path, ``dt`` for the edge crop of the
+    no-shape path."""
+    n_ifo, n_freq, n_time = (int(v) for v in stack.shape)
+    stack = stack.to(torch.float64).contiguous()
     hi = n_ifo * 100.0
     if alp is not None:
-        pw0 = tf_maps[0]
-        M = int(np.asarray(pw0.data).shape[0]) if np.asarray(pw0.data).ndim == 2 else 1
+        M = n_freq if two_d else 1
         size = n_freq * n_time
-        nL = int(float(edge or 0.0) * pw0.wavelet_rate * M)
+        nL = int(float(edge or 0.0) * wavelet_rate * M)
         nR = size - nL - 1
         start, stop, _ = slice(nL, nR).indices(size)
         count = max(0, stop - start)
@@ -112,18 +139,9 @@ def compute_threshold(tf_maps, bpp: float, alp: float | None = None,
         return avr * _igamma_inv_upper(alp_fit, bpp_corr) / alp_fit / 2.0
 
     crop = 0
-    if np.asarray(tf_maps[0].data).ndim == 2 and edge is not None:
-        crops = set()
-        for t in tf_maps:
-            e = 0
-            if hasattr(t, "dt") and t.dt > 0:
-                e = int(max(0, round(float(edge) / float(t.dt))))
-                if not (e > 0 and n_time > 2 * e):
-                    e = 0
-            crops.add(e)
-        if len(crops) > 1:
-            raise ValueError("detector TF maps are cropped to different shapes")
-        crop = crops.pop()
+    if two_d and edge is not None and dt is not None and dt > 0:
+        e = int(max(0, round(float(edge) / float(dt))))
+        crop = e if (e > 0 and n_time > 2 * e) else 0
     ncol = n_time - 2 * crop
     n_total = n_freq * ncol
     if n_total == 0:

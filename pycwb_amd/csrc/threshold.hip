@@ -217,6 +217,11 @@ struct Above {
   __device__ bool operator()(const double& x) const { return x > 1.0e-3; }
 };
 
+struct Inside {  // lo < x < hi
+  double lo, hi;
+  __device__ bool operator()(const double& x) const { return x > lo && x < hi; }
+};
+
 struct Plain {
   const double* x;
   __device__ double operator()(int64_t i) const { return x[i]; }
@@ -336,6 +341,57 @@ __global__ void __launch_bounds__(kChunkBlock) k_th_chunks(const double* x, cons
     out[2 * blockIdx.x] = tv[0][0];
     out[2 * blockIdx.x + 1] = tl[0][0];
   }
+}
+
+// ---- Gamma-to-Gauss of a max-energy map (TimeFrequencyMap.Gamma2Gauss) ----
+
+// partial[block] = number of x[begin .. end) above thr in the block's share.
+__global__ void __launch_bounds__(kBlock) k_g2g_count(const double* x, int64_t begin,
+                                                     int64_t end, double thr,
+                                                     unsigned long long* partial) {
+  unsigned long long c = 0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t k = begin + (int64_t)blockIdx.x * kBlock + threadIdx.x; k < end; k += stride)
+    c += x[k] > thr ? 1ull : 0ull;
+  for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_down(c, o);
+  __shared__ unsigned long long sc[kBlock / kWave];
+  if ((threadIdx.x & (kWave - 1)) == 0) sc[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += sc[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_g2g_reduce(const unsigned long long* partial,
+                                                      int n_blocks, unsigned long long* out) {
+  __shared__ unsigned long long s[kBlock];
+  unsigned long long c = 0;
+  for (int b = threadIdx.x; b < n_blocks; b += kBlock) c += partial[b];
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+// out = amp - ALP (1 + log(amp / ALP)) where amp = x alp / avr >= ALP, else 0
+// (time_frequency_map.py:196-204: amp = flat * alp / avr in that order).
+__global__ void k_g2g_map(const double* x, int64_t n, double alp, double avr, double ALP,
+                          double* out) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += stride) {
+    const double amp = x[k] * alp / avr;
+    out[k] = amp >= ALP ? amp - ALP * (1.0 + log(amp / ALP)) : 0.0;
+  }
+}
+
+__global__ void k_g2g_scale(double* x, int64_t n, double s) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += stride) x[k] *= s;
 }
 
 int check_launch(const char* what) {
@@ -480,6 +536,137 @@ int pycwb_threshold_stats(const pycwb_threshold_desc* d, void* stream) {
     if ((rc = check_launch("k_th_chunks")) != PYCWB_OK) return rc;
   }
   return PYCWB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct G2gLayout {
+  size_t compact, partial, temp, temp_bytes, total;
+};
+
+int g2g_layout(const pycwb_g2g_desc* d, G2gLayout* L) {
+  const int64_t win = d->nR - d->nL + 1;
+  size_t sort_bytes = 0, sel_bytes = 0;
+  if (hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const double*)nullptr,
+                                        (double*)nullptr, (int)win) != hipSuccess ||
+      hipcub::DeviceSelect::If(nullptr, sel_bytes, (const double*)nullptr, (double*)nullptr,
+                               (int64_t*)nullptr, (int)(win - 1), Inside{0.0, 0.0}) !=
+          hipSuccess) {
+    record_error("pycwb_g2g_work_bytes", "hipCUB storage query failed");
+    return PYCWB_ERR_HIP;
+  }
+  size_t o = 0;
+  L->compact = o;
+  o = align_up(o + sizeof(double) * (size_t)win);
+  L->partial = o;
+  o = align_up(o + sizeof(unsigned long long) * kValueBlocks);
+  L->temp_bytes = sort_bytes > sel_bytes ? sort_bytes : sel_bytes;
+  L->temp = o;
+  o = align_up(o + (L->temp_bytes ? L->temp_bytes : 1));
+  L->total = o;
+  return PYCWB_OK;
+}
+
+int g2g_count(const double* x, int64_t begin, int64_t end, double thr,
+              unsigned long long* partial, unsigned long long* out, hipStream_t s) {
+  const int64_t want = (end - begin + kBlock - 1) / kBlock;
+  const int blocks = (int)(want < 1 ? 1 : (want < kValueBlocks ? want : kValueBlocks));
+  hipLaunchKernelGGL(k_g2g_count, dim3((unsigned)blocks), dim3(kBlock), 0, s, x, begin, end,
+                     thr, partial);
+  int rc = check_launch("k_g2g_count");
+  if (rc != PYCWB_OK) return rc;
+  hipLaunchKernelGGL(k_g2g_reduce, dim3(1), dim3(kBlock), 0, s,
+                     (const unsigned long long*)partial, blocks, out);
+  return check_launch("k_g2g_reduce");
+}
+
+int g2g_sort(const double* x, int64_t n, double* out, char* temp, size_t temp_bytes,
+             hipStream_t s) {
+  size_t tb = temp_bytes;
+  const hipError_t e = hipcub::DeviceRadixSort::SortKeys(temp, tb, x, out, (int)n, 0, 64, s);
+  if (e != hipSuccess) {
+    record_error("gamma-to-gauss sort", hipGetErrorString(e));
+    return PYCWB_ERR_HIP;
+  }
+  return PYCWB_OK;
+}
+
+int g2g_check(const pycwb_g2g_desc* d) {
+  if (!d || !d->flat || d->n < 4 || d->n >= INT32_MAX || d->nL < 0 || d->nR <= d->nL ||
+      d->nR > d->n - 1)
+    return PYCWB_ERR_ARG;
+  return PYCWB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pycwb_g2g_work_bytes(const pycwb_g2g_desc* d, size_t* bytes) {
+  if (!bytes) return PYCWB_ERR_ARG;
+  int rc = g2g_check(d);
+  if (rc != PYCWB_OK) return rc;
+  G2gLayout L;
+  if ((rc = g2g_layout(d, &L)) != PYCWB_OK) return rc;
+  *bytes = L.total;
+  return PYCWB_OK;
+}
+
+int pycwb_g2g_phase(const pycwb_g2g_desc* d, int phase, void* stream) {
+  int rc = g2g_check(d);
+  if (rc != PYCWB_OK) return rc;
+  G2gLayout L;
+  if ((rc = g2g_layout(d, &L)) != PYCWB_OK) return rc;
+  if (!d->work || d->work_bytes < L.total) return PYCWB_ERR_ARG;
+  const hipStream_t s = (hipStream_t)stream;
+  char* w = (char*)d->work;
+  unsigned long long* partial = (unsigned long long*)(w + L.partial);
+  const int64_t win = d->nR - d->nL + 1;
+  (void)hipGetLastError();
+  const unsigned grid = (unsigned)((d->n + kBlock - 1) / kBlock < 4096
+                                       ? (d->n + kBlock - 1) / kBlock : 4096);
+  switch (phase) {
+    case 1:  // count(flat > 0.001) over the map; the window flat[nL .. nR] sorted
+      if (!d->count || !d->sorted) return PYCWB_ERR_ARG;
+      if ((rc = g2g_count(d->flat, 0, d->n, 1.0e-3, partial,
+                          (unsigned long long*)d->count, s)) != PYCWB_OK)
+        return rc;
+      return g2g_sort(d->flat + d->nL, win, d->sorted, w + L.temp, L.temp_bytes, s);
+    case 2: {  // region values in (lo, hi), in order; numpy-order sums and log sums
+      if (!d->chunk_sums || !d->n_valid) return PYCWB_ERR_ARG;
+      size_t tb = L.temp_bytes;
+      double* compact = (double*)(w + L.compact);
+      const hipError_t e = hipcub::DeviceSelect::If(w + L.temp, tb, d->flat + d->nL, compact,
+                                                    d->n_valid, (int)(win - 1),
+                                                    Inside{d->lo, d->hi}, s);
+      if (e != hipSuccess) {
+        record_error("gamma-to-gauss select", hipGetErrorString(e));
+        return PYCWB_ERR_HIP;
+      }
+      const int64_t chunks = (win - 1 + kNpChunk - 1) / kNpChunk;
+      hipLaunchKernelGGL(k_th_chunks, dim3((unsigned)(chunks < 1 ? 1 : chunks)),
+                         dim3(kChunkBlock), 0, s, (const double*)compact,
+                         (const int64_t*)d->n_valid, d->chunk_sums);
+      return check_launch("k_th_chunks");
+    }
+    case 3:  // the transformed map; count(out[nL .. n - nL) > 1e-5); its window sorted
+      if (!d->out || !d->count || !d->sorted) return PYCWB_ERR_ARG;
+      hipLaunchKernelGGL(k_g2g_map, dim3(grid), dim3(kBlock), 0, s, d->flat, d->n, d->alp,
+                         d->avr, d->ALP, d->out);
+      if ((rc = check_launch("k_g2g_map")) != PYCWB_OK) return rc;
+      if ((rc = g2g_count(d->out, d->nL, d->n - d->nL, 1.0e-5, partial,
+                          (unsigned long long*)d->count + 1, s)) != PYCWB_OK)
+        return rc;
+      return g2g_sort(d->out + d->nL, win, d->sorted, w + L.temp, L.temp_bytes, s);
+    case 4:  // out *= scale
+      if (!d->out) return PYCWB_ERR_ARG;
+      hipLaunchKernelGGL(k_g2g_scale, dim3(grid), dim3(kBlock), 0, s, d->out, d->n, d->scale);
+      return check_launch("k_g2g_scale");
+    default:
+      return PYCWB_ERR_ARG;
+  }
 }
 
 }  // extern "C"

@@ -8,7 +8,9 @@ over 4); the max-energy map with no delay step equals the packet energy of the
 unshifted map, and with delays equals the maximum over the maps of shifted
 copies built independently in NumPy.
 GPU: the batched HIP path against the oracle for every pattern (1e-10 of the
-map's maximum: different FFTs), and the TF-map facade.
+map's maximum: different FFTs), the device Gamma-to-Gauss against the host
+one (order statistics and counts exact, the map and ALP to 1e-12: the device
+log is not NumPy's), and the TF-map facade.
 """
 
 import numpy as np
@@ -161,8 +163,9 @@ def test_gpu_max_energy_facade():
     cur = max_energy_device(torch.from_numpy(x).to("cuda:0"), w, int(rate * 0.01), 2, 3, 1.0,
                             int(1 / tf.dt), 32.0, 400.0, tf.df)
     ref, alp_ref = gamma_to_gauss(cur.t().contiguous().cpu().numpy(), 1.0, int(1 / tf.dt))
-    assert alp == alp_ref > 0
-    np.testing.assert_array_equal(new.data, ref)
+    assert alp_ref > 0
+    np.testing.assert_allclose(alp, alp_ref, rtol=1e-12)
+    np.testing.assert_allclose(new.data, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
     # without the stored series the map is reconstructed with w2t first
     tfm2 = SimpleNamespace(data=tf.data, dt=tf.dt, df=tf.df, edge=1.0, f_low=32.0,
                            f_high=400.0, wavelet=w, len_timeseries=x.size, ts_data=None)
@@ -171,3 +174,28 @@ def test_gpu_max_energy_facade():
     np.testing.assert_allclose(new2.data[:, e:-e], new.data[:, e:-e], rtol=1e-3, atol=1e-3)
     with pytest.raises(NotImplementedError):
         max_energy(tfm, 0.01, 2, 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,edge,zeros", [((33, 4000), 0.0, 0.0), ((65, 3000), 1.0, 0.4),
+                                              ((9, 20000), 2.0, 0.0), ((17, 500), 0.5, 0.9)])
+def test_gpu_gamma_to_gauss_matches_host(shape, edge, zeros):
+    import torch
+
+    from pycwb_amd.coherence_native.projection import gamma_to_gauss, gamma_to_gauss_device
+
+    print(f"case gamma-to-gauss {shape} edge={edge}", flush=True)
+    rng = np.random.default_rng(shape[1])
+    E = rng.gamma(1.7, 1.3, size=shape)
+    E[rng.random(shape) < zeros] = 0.0
+    ref, alp_ref = gamma_to_gauss(E.copy(), edge, 16)
+    out, alp = gamma_to_gauss_device(torch.from_numpy(np.ascontiguousarray(E.T)).to("cuda:0"),
+                                     edge, 16)
+    assert alp_ref > 0
+    np.testing.assert_allclose(alp, alp_ref, rtol=1e-12)
+    g = out.t().cpu().numpy()
+    np.testing.assert_allclose(g, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    # a map with nothing above the floor: the reference gives up, so does the device
+    z = torch.zeros((40, 9), dtype=torch.float64, device="cuda:0")
+    same, a0 = gamma_to_gauss_device(z, 0.0, 16)
+    assert a0 == 0.0 and same is z
