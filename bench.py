@@ -872,6 +872,8 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
     scan.launch_stage(3, stream)
     torch.cuda.synchronize()
     stage_ms = {"regulator": ev[0].elapsed_time(ev[1]), "scan": ev[1].elapsed_time(ev[2])}
+    active = scan.active_pairs()
+    active_fraction = float(active.sum()) / float(batch.evals(net.n_sky))
     scan.close()
     # recovery: injections with a cluster within 0.1 s whose argmax delays match
     ctimes = np.concatenate([lv["ctime"] for lv in levels])
@@ -898,6 +900,7 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
            "likelihood_evals_per_s": evals / stages["likelihood_batch"],
            "stage_s": {k: round(v, 4) for k, v in stages.items()},
            "likelihood_stage_ms": {k: round(v, 2) for k, v in stage_ms.items()},
+           "likelihood_active_fraction": round(active_fraction, 4),
            "wall_s_without_data_and_setup": round(total, 3),
            "injections_found_with_delays": found}
     log(f"e2e: {C_} clusters ({int(sizes.sum())} px) from 7 levels in {total:.2f} s "
