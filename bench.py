@@ -736,7 +736,7 @@ def _inject_sine_gaussians(x, ifos, rate, n_inj, seed):
 
 
 def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, td_size=12,
-            bpp=0.001, seg_edge=2.0, max_delay_s=0.04, pattern=3):
+            bpp=0.001, seg_edge=2.0, max_delay_s=0.04, pattern=3, cpu=True):
     """The end-to-end case: 5 detectors (H1 L1 V1 K1 I1), 600 s of white
     (whitened) noise at 4096 Hz with 200 sine-Gaussian injections, 7 WDM
     levels (M = 8 .. 512).  Per level, in the order of coherence_native/
@@ -751,7 +751,8 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
     super-clustering across levels is not part of this build, so every level's
     clusters go to the likelihood.  Found: injections with a cluster within
     0.1 s whose argmax direction reproduces the injected detector delays to
-    1 ms."""
+    1 ms.  Parity sample (with ``cpu``): two clusters' regulator and l_max
+    against the oracle (16 threads) on the batch's own amplitudes."""
     import torch
 
     from pycwb_amd import sky, synth
@@ -874,6 +875,29 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
     stage_ms = {"regulator": ev[0].elapsed_time(ev[1]), "scan": ev[1].elapsed_time(ev[2])}
     active = scan.active_pairs()
     active_fraction = float(active.sum()) / float(batch.evals(net.n_sky))
+    parity = None
+    if cpu:
+        from oracle import oracle as ora
+
+        picks = [int(np.argmin(np.abs(sizes - 10))), int(np.argmin(np.abs(sizes - 40)))]
+        vh = np.arange(net.n_sky, dtype=np.int64)
+        En = nt.En[0]
+        parity = []
+        for c in picks:
+            P = int(sizes[c])
+            rms_c = batch.cluster_rms(c).cpu().numpy()
+            td_c = batch.cluster_td(c).cpu().numpy()                  # (P, n, 2K+1, 2)
+            t00 = np.ascontiguousarray(td_c[..., 0].transpose(2, 1, 0))
+            t90 = np.ascontiguousarray(td_c[..., 1].transpose(2, 1, 0))
+            reg_o = ora.calculate_dpf(net.FP_t, net.FX_t, rms_c, net.n_sky, n,
+                                      0.5 * 0.5 * 2 / 3, En, vh, threads=16)
+            REG = np.array([np.float32(0.5 * np.sqrt(2)), res["REG1"][c], 0.0], np.float32)
+            l_o, _, _ = ora.scan_sky(n, P, net.n_sky, net.FP_t, net.FX_t, rms_c, t00, t90,
+                                     net.ml, REG, 0.5, 0.5, En, vh, threads=16)
+            parity.append({"cluster": c, "pixels": P,
+                           "regulator_bit_exact": bool(reg_o == res["regulator"][c]),
+                           "l_max_bit_exact": bool(int(l_o) == int(res["l_max"][c]))})
+        log(f"e2e parity sample: {parity}")
     scan.close()
     # recovery: injections with a cluster within 0.1 s whose argmax delays match
     ctimes = np.concatenate([lv["ctime"] for lv in levels])
@@ -901,6 +925,7 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
            "stage_s": {k: round(v, 4) for k, v in stages.items()},
            "likelihood_stage_ms": {k: round(v, 2) for k, v in stage_ms.items()},
            "likelihood_active_fraction": round(active_fraction, 4),
+           "parity_sample": parity,
            "wall_s_without_data_and_setup": round(total, 3),
            "injections_found_with_delays": found}
     log(f"e2e: {C_} clusters ({int(sizes.sum())} px) from 7 levels in {total:.2f} s "
@@ -1037,7 +1062,8 @@ def main():
                                                cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.only == "e2e":
-        print(json.dumps({"e2e": e2e_leg(torch.cuda.current_stream(), log)}), flush=True)
+        print(json.dumps({"e2e": e2e_leg(torch.cuda.current_stream(), log,
+                                         cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.only == "threshold":
         print(json.dumps({"threshold": threshold_leg(torch.cuda.current_stream(), log,
@@ -1170,7 +1196,7 @@ def main():
     maxen = None if args.no_maxenergy else maxenergy_leg(stream, log, cpu=not args.no_cpu)
     clus = None if args.no_cluster else cluster_leg(stream, log, cpu=not args.no_cpu)
     thr = None if args.no_threshold else threshold_leg(stream, log, cpu=not args.no_cpu)
-    e2e = None if args.no_e2e else e2e_leg(stream, log)
+    e2e = None if args.no_e2e else e2e_leg(stream, log, cpu=not args.no_cpu)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
