@@ -57,10 +57,14 @@ def cluster_pixels_device(sel: dict, kt: int = 1, kf: int = 1, rate: float = 0.0
     ``label``, ``members``, ``cluster_off`` and per-cluster ``energy``,
     ``c_time``, ``c_freq``, ``subnet``, ``subrho`` (capacity n), and host
     ``lag_cluster_off`` (n_lag + 1) after one synchronisation."""
-    dev = sel["frequency"].device if isinstance(sel["frequency"], torch.Tensor) else require_gpu()
     lo_h = np.asarray(sel["lag_off"], dtype=np.int64)
     n_lag = lo_h.size - 1
+    if n_lag < 1 or lo_h[0] != 0 or np.any(np.diff(lo_h) < 0):
+        raise ValueError("lag_off must start at 0 and not decrease")
     n = int(lo_h[-1])
+    if any(int(sel[k].shape[0]) < n for k in ("frequency", "time", "energy", "pix_det_energy")):
+        raise ValueError("pixel arrays shorter than lag_off[-1]")
+    dev = sel["frequency"].device if isinstance(sel["frequency"], torch.Tensor) else require_gpu()
     lo_d = sel.get("lag_off_dev")
     if lo_d is None:
         lo_d = torch.from_numpy(np.ascontiguousarray(lo_h)).to(dev)

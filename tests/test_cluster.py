@@ -358,3 +358,19 @@ def test_gpu_cluster_index_map_is_restored(oracle):
         o = oracle.cluster_run(sel["frequency"], sel["time"], sel["energy"],
                                sel["pix_det_energy"], n_freq, n_time, 1, 1, rate, _n_sub(3))
         _assert_same_as_oracle(g, o)
+
+
+def test_cluster_device_rejects_bad_lag_offsets():
+    """Host-side checks run before any device work (no GPU needed)."""
+    from pycwb_amd.coherence_native.clustering import cluster_pixels_device
+
+    z = np.zeros(3, np.int64)
+    for lo in ([1, 3], [0, 3, 2], [0]):
+        with pytest.raises(ValueError):
+            cluster_pixels_device({"lag_off": np.array(lo), "frequency": z, "time": z,
+                                   "energy": np.zeros(3), "pix_det_energy": np.zeros((3, 2)),
+                                   "n_freq": 4, "n_time": 4})
+    with pytest.raises(ValueError):
+        cluster_pixels_device({"lag_off": np.array([0, 5]), "frequency": z, "time": z,
+                               "energy": np.zeros(3), "pix_det_energy": np.zeros((3, 2)),
+                               "n_freq": 4, "n_time": 4})
