@@ -117,7 +117,8 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
  *   td  at td + 2*td_off[c]  (n_pix[c], n, 2K[c]+1) {00,90} float pairs
  *   network net[c]: FP/FX at FP/FX + net_off[net] (n_sky, n), ml at
  *   ml + net_off[net] (n, n_sky), energy threshold net_En[net].
- * Scratch: tickets (2*n_groups uint32), counts, best (per cluster).
+ * Scratch: tickets (16*n_groups uint32: per group and phase, one work queue
+ * per XCD), counts, best (per cluster).
  * Outputs per cluster: REG1 (float32), l_max (int64), sky_stat_max (float64),
  * reg_f64 (float64, may be NULL). */
 typedef struct pycwb_batch_desc {

@@ -23,6 +23,7 @@ TILE_ENERGY_BOUND = 4
 REGULATOR_BOUNDS = 8
 DEFAULT_FLAGS = SKIP_MASKED_PIXELS | REGULATOR_EARLY_EXIT | TILE_ENERGY_BOUND | REGULATOR_BOUNDS
 WSTATS_PER_CLUSTER = 16
+TICKETS_PER_GROUP = 16   # uint32 ticket counters per detector group of a batch
 
 _lock = threading.Lock()
 _lib = None

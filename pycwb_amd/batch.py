@@ -303,7 +303,8 @@ class BatchedSkyScan:
         self.d_K = torch.from_numpy(batch.K).to(dev)
         self.d_rms_off = torch.from_numpy(batch.rms_off).to(dev)
         self.d_td_off = torch.from_numpy(batch.td_off).to(dev)
-        self.tickets = torch.zeros(2 * len(groups), dtype=torch.int32, device=dev)
+        self.tickets = torch.zeros(_lib.TICKETS_PER_GROUP * len(groups), dtype=torch.int32,
+                                   device=dev)
         self.counts = torch.zeros(C_, dtype=torch.int32, device=dev)
         self.best = torch.zeros(C_, dtype=torch.int64, device=dev)
         self.REG1 = torch.zeros(C_, dtype=torch.float32, device=dev)

@@ -698,9 +698,10 @@ __global__ void k_argmax_finish(const unsigned long long* __restrict__ best,
 // ---------------------------------------------------------------------------
 // Batched family.  Clusters are grouped by detector count n (one persistent
 // kernel instantiation per n, so each keeps the register budget of its own n).
-// Inside a group, persistent blocks pull (cluster, sky tile) items from a
-// ticket counter; tickets are cluster-major in the group's order (largest
-// work first), so concurrent blocks share one cluster's amplitudes in L2.
+// Inside a group, persistent blocks pull (cluster, sky tile) items from
+// ticket counters, one queue per XCD; each queue is cluster-major in the
+// group's order (largest work first) over its eighth of the tiles, so an
+// XCD's concurrent blocks share one cluster's amplitudes in its L2.
 // ---------------------------------------------------------------------------
 struct BatchDev {
   int n_clusters;
@@ -758,6 +759,14 @@ __global__ __launch_bounds__(256) void k_batch_weights(BatchDev b) {
 // detectors: at four detectors the scan spills about 20 VGPRs there, and
 // still beats the spill-free 3-wave build by 3 % (occupancy hides more gather
 // latency than the spill traffic costs).
+// Work queues per group and phase: 8, one per XCD (1.2 % faster at config 2
+// than one shared queue: 1839 against 1862 ms per step), or 1 with
+// PYCWB_XCD_QUEUES=0.
+#ifndef PYCWB_XCD_QUEUES
+#define PYCWB_XCD_QUEUES 1
+#endif
+constexpr int kTicketsPerGroup = PYCWB_XCD_QUEUES ? 16 : 2;  // per phase: 8 queues or 1
+
 #ifndef PYCWB_OCC_REG_SMALL
 #define PYCWB_OCC_REG_SMALL 5
 #endif
@@ -778,10 +787,97 @@ struct PhaseOccupancy {
 };
 
 template <int N, int BLOCK, int PHASE>
+__device__ __forceinline__ void batch_item(const BatchDev& b, int c, int tile) {
+  const int64_t t = (int64_t)tile * BLOCK + threadIdx.x;
+  const bool active = t < b.n_valid;
+  const int net = b.net[c];
+  int64_t k = active ? (b.perm ? (int64_t)b.perm[(int64_t)net * b.n_valid + t] : t) : 0;
+  k = k < 0 ? 0 : (k >= b.n_valid ? b.n_valid - 1 : k);
+  const int64_t s = b.valid[k];
+  const float* FP = b.FP + b.net_off[net];
+  const float* FX = b.FX + b.net_off[net];
+  const float* rms = b.rms + b.rms_off[c];
+  const int P = b.npix[c];
+  if (PHASE == 2) {
+    // energy test only: how many (direction, pixel) pairs take the full path
+    const int K = b.K[c], nd = 2 * K + 1;
+    const int32_t* ml = b.ml + b.net_off[net];
+    const float2* td = b.td + b.td_off[c];
+    const double En = b.net_En[net];
+    int tap[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      int t = ml[(int64_t)i * b.S + s] + K;
+      tap[i] = i * nd + (t < 0 ? 0 : (t >= nd ? nd - 1 : t));
+    }
+    unsigned cnt = 0;
+    if (active) {
+      for (int j = 0; j < P; ++j) {
+        float v00[N], v90[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const float2 v = td[(int64_t)j * N * nd + tap[i]];
+          v00[i] = v.x;
+          v90[i] = v.y;
+        }
+        float e, et;
+        int m;
+        pixel_energy<N>(v00, v90, En, e, m, et);
+        cnt += pixel_contributes(e, m) ? 1u : 0u;
+      }
+    }
+    block_count<BLOCK>(cnt, &b.counts[c]);
+  } else if (PHASE == 0) {
+    const double* ws = (b.wstats && (b.flags & PYCWB_REGULATOR_BOUNDS))
+                           ? b.wstats + (int64_t)c * kWStats : nullptr;
+    const unsigned v = active ? regulator_lane<N>(FP, FX, rms, P, s, b.gamma,
+                                                  (b.flags & PYCWB_REGULATOR_EARLY_EXIT) != 0,
+                                                  ws)
+                              : 0u;
+    block_count<BLOCK>(v, &b.counts[c]);
+  } else {
+    const double En = b.net_En[net];
+    const float RR = (float)regulator_value(b.n_valid, b.counts[c], En);
+    const int32_t* ml = b.ml + b.net_off[net];
+    const float2* td = b.td + b.td_off[c];
+    const unsigned long long key =
+        scan_lane<N, BLOCK / 64>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR, En, b.netCC, s,
+                                 k, active,
+                     (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
+                     (b.flags & PYCWB_TILE_ENERGY_BOUND) != 0, nullptr);
+    block_argmax<BLOCK>(key, &b.best[c]);
+  }
+}
+
+template <int N, int BLOCK, int PHASE>
 __global__ __launch_bounds__(BLOCK, (PhaseOccupancy<PHASE, N>::waves)) void k_batch_persistent(
-    BatchDev b, int first, int count,
-                                                            unsigned* ticket) {
+    BatchDev b, int first, int count, unsigned* ticket) {
   __shared__ unsigned s_item;
+#if PYCWB_XCD_QUEUES
+  // Eight queues per group, one per XCD (workgroup i runs on XCD i mod 8):
+  // queue q holds the tiles [q * per, (q + 1) * per) of every cluster, cluster
+  // by cluster, so an XCD's workgroups take neighbouring tiles (neighbouring
+  // delay tuples, the same amplitude lines) and share its L2.  A queue that
+  // runs dry moves on to the next XCD's queue; every queue's counter only
+  // grows, so each item is taken once and every workgroup ends.
+  const unsigned tiles = (unsigned)b.tiles_per_cluster;
+  const unsigned per = (tiles + 7u) / 8u;
+  const unsigned n_items = (unsigned)count * per;
+  const int xcd = (int)(blockIdx.x & 7u);
+  for (int qi = 0; qi < 8; ++qi) {
+    const int q = (xcd + qi) & 7;
+    for (;;) {
+      if (threadIdx.x == 0) s_item = atomicAdd(ticket + q, 1u);
+      __syncthreads();
+      const unsigned item = s_item;
+      __syncthreads();
+      if (item >= n_items) break;
+      const unsigned tile = (unsigned)q * per + item % per;
+      if (tile >= tiles) continue;
+      batch_item<N, BLOCK, PHASE>(b, b.order[first + (int)(item / per)], (int)tile);
+    }
+  }
+#else
   const unsigned n_items = (unsigned)count * (unsigned)b.tiles_per_cluster;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
@@ -789,68 +885,10 @@ __global__ __launch_bounds__(BLOCK, (PhaseOccupancy<PHASE, N>::waves)) void k_ba
     const unsigned item = s_item;
     __syncthreads();
     if (item >= n_items) break;  // every block reaches this once the tickets run out
-    const int c = b.order[first + (int)(item / (unsigned)b.tiles_per_cluster)];
-    const int tile = (int)(item % (unsigned)b.tiles_per_cluster);
-    const int64_t t = (int64_t)tile * BLOCK + threadIdx.x;
-    const bool active = t < b.n_valid;
-    const int net = b.net[c];
-    int64_t k = active ? (b.perm ? (int64_t)b.perm[(int64_t)net * b.n_valid + t] : t) : 0;
-    k = k < 0 ? 0 : (k >= b.n_valid ? b.n_valid - 1 : k);
-    const int64_t s = b.valid[k];
-    const float* FP = b.FP + b.net_off[net];
-    const float* FX = b.FX + b.net_off[net];
-    const float* rms = b.rms + b.rms_off[c];
-    const int P = b.npix[c];
-    if (PHASE == 2) {
-      // energy test only: how many (direction, pixel) pairs take the full path
-      const int K = b.K[c], nd = 2 * K + 1;
-      const int32_t* ml = b.ml + b.net_off[net];
-      const float2* td = b.td + b.td_off[c];
-      const double En = b.net_En[net];
-      int tap[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        int t = ml[(int64_t)i * b.S + s] + K;
-        tap[i] = i * nd + (t < 0 ? 0 : (t >= nd ? nd - 1 : t));
-      }
-      unsigned cnt = 0;
-      if (active) {
-        for (int j = 0; j < P; ++j) {
-          float v00[N], v90[N];
-#pragma unroll
-          for (int i = 0; i < N; ++i) {
-            const float2 v = td[(int64_t)j * N * nd + tap[i]];
-            v00[i] = v.x;
-            v90[i] = v.y;
-          }
-          float e, et;
-          int m;
-          pixel_energy<N>(v00, v90, En, e, m, et);
-          cnt += pixel_contributes(e, m) ? 1u : 0u;
-        }
-      }
-      block_count<BLOCK>(cnt, &b.counts[c]);
-    } else if (PHASE == 0) {
-      const double* ws = (b.wstats && (b.flags & PYCWB_REGULATOR_BOUNDS))
-                             ? b.wstats + (int64_t)c * kWStats : nullptr;
-      const unsigned v = active ? regulator_lane<N>(FP, FX, rms, P, s, b.gamma,
-                                                    (b.flags & PYCWB_REGULATOR_EARLY_EXIT) != 0,
-                                                    ws)
-                                : 0u;
-      block_count<BLOCK>(v, &b.counts[c]);
-    } else {
-      const double En = b.net_En[net];
-      const float RR = (float)regulator_value(b.n_valid, b.counts[c], En);
-      const int32_t* ml = b.ml + b.net_off[net];
-      const float2* td = b.td + b.td_off[c];
-      const unsigned long long key =
-          scan_lane<N, BLOCK / 64>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR, En, b.netCC, s,
-                                   k, active,
-                       (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
-                       (b.flags & PYCWB_TILE_ENERGY_BOUND) != 0, nullptr);
-      block_argmax<BLOCK>(key, &b.best[c]);
-    }
+    batch_item<N, BLOCK, PHASE>(b, b.order[first + (int)(item / (unsigned)b.tiles_per_cluster)],
+                                (int)(item % (unsigned)b.tiles_per_cluster));
   }
+#endif
 }
 
 __global__ void k_batch_finish(BatchDev b) {
@@ -1183,7 +1221,7 @@ int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
   const BatchDev& d = b->dev;
   if (stage == 0) {
     hipLaunchKernelGGL(k_batch_init, dim3(grid_for(d.n_clusters, 4096)), dim3(256), 0, st, d,
-                       2 * b->n_groups);
+                       kTicketsPerGroup * b->n_groups);
     HIP_TRY(hipGetLastError());
     if (d.wstats && (d.flags & PYCWB_REGULATOR_BOUNDS)) {
       hipLaunchKernelGGL(k_batch_weights, dim3((unsigned)d.n_clusters), dim3(256), 0, st, d);
@@ -1201,7 +1239,7 @@ int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
   // tickets, so it runs after stage 0 and before anything that reads counts.
   const int phase = stage == 4 ? 2 : stage - 1;
   for (int g = 0; g < b->n_groups; ++g) {
-    unsigned* ticket = d.tickets + 2 * g + (phase == 1 ? 1 : 0);
+    unsigned* ticket = d.tickets + kTicketsPerGroup * g + (phase == 1 ? kTicketsPerGroup / 2 : 0);
     int rc = PYCWB_ERR_ARG;
     const int n = b->group_nifo[g], first = b->group_first[g], cnt = b->group_count[g];
 #define PHASES(BLK)                                                                \
