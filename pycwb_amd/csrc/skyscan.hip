@@ -293,16 +293,18 @@ __device__ __forceinline__ unsigned regulator_lane(const float* __restrict__ FP,
   return q > gamma ? 1u : 0u;
 }
 
+// Wave-wide min / max of v over all 64 lanes (every lane active): the result
+// is the same on every lane, so it is read into a scalar register.
 __device__ __forceinline__ int wave_min_int(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 // Widest summed tap range (over detectors) for which the tile energy bound is
