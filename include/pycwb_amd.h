@@ -120,7 +120,7 @@ int pycwb_sky_scan_plain(int n_ifo, int n_pix, int64_t n_sky, int n_delay, const
  * Scratch: tickets (16*n_groups uint32: per group and phase, one work queue
  * per XCD), counts, best (per cluster).
  * Outputs per cluster: REG1 (float32), l_max (int64), sky_stat_max (float64),
- * reg_f64 (float64, may be NULL). */
+ * reg_f64 (float64, may be NULL).  n_sky < 2^31 - 1. */
 typedef struct pycwb_batch_desc {
   int n_clusters;
   int64_t n_sky;

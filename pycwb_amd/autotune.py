@@ -5,7 +5,9 @@ Tunes the two shape parameters of the persistent work-queue kernels:
   grid  : persistent workgroups (waves per SIMD of resident work).
 It times the regulator and scan stages together (the persistent kernels
 the shape affects) with HIP events on a seeded calibration batch drawn from
-the same cluster-size distribution, keeps the fastest shape (among shapes
+the same cluster-size distribution (384 clusters by default: with fewer, the
+few largest clusters' tails decide the timing and favour shapes that are
+slower on a full batch), keeps the fastest shape (among shapes
 within 2 % of the fastest, the largest grid: more resident waves hide the
 regulator's latency better on the full batch), and caches the choice per
 (device, detector counts, K, directions) in
@@ -70,7 +72,7 @@ def time_stages(bs, reps: int = 2) -> float:
     return best
 
 
-def tune_batched(net_list, nets, valid, gamma, delta, netCC, n_calib: int = 96,
+def tune_batched(net_list, nets, valid, gamma, delta, netCC, n_calib: int = 384,
                  use_cache: bool = True, log=None) -> dict:
     """Returns {"block", "grid", "source", "ms"} for the batched regulator + scan."""
     from .batch import BatchedSkyScan, synth_batch

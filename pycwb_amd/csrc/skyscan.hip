@@ -788,6 +788,146 @@ struct PhaseOccupancy {
                  : (PHASE == 1 ? (N <= 3 ? PYCWB_OCC_SCAN_SMALL : PYCWB_OCC_SCAN_LARGE) : 8);
 };
 
+// Regulator work item: kRegPool tiles of BLOCK directions, each wavefront
+// taking kRegPool * 64 consecutive delay-order positions.  A direction's sweep
+// stops as soon as its term bounds decide its count (regulator_lane), so sweep
+// lengths differ widely between neighbouring directions and a wavefront that
+// swept one tile ran until its slowest lane ended (at order 7 that doubles the
+// sweep's wavefront time).  Here a first pass computes the bounds of all the
+// wavefront's directions with every lane busy and counts those decided before
+// the first pixel; the others go to a list in LDS, and the sweep hands the
+// next listed direction to each lane whose direction is decided, checking
+// every 16 pixels.  A direction leaves the sweep only on a bound that decides
+// its count, so the count equals that of the full sweep.  Bounds are kept in
+// float32 rounded outwards (lower bound down, upper bound up).
+#ifndef PYCWB_REG_POOL
+#define PYCWB_REG_POOL 8
+#endif
+constexpr int kRegPool = PYCWB_REG_POOL;
+
+__device__ __forceinline__ int lane_rank(unsigned long long mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+template <int N, int BLOCK>
+__device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int item) {
+  constexpr int kList = kRegPool * 64;
+  __shared__ int s_dir[BLOCK / 64][kList];
+  __shared__ float s_lb[BLOCK / 64][kList];
+  __shared__ float s_ub[BLOCK / 64][kList];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int net = b.net[c];
+  const float* FP = b.FP + b.net_off[net];
+  const float* FX = b.FX + b.net_off[net];
+  const float* rms = b.rms + b.rms_off[c];
+  const int P = b.npix[c];
+  const bool early = (b.flags & PYCWB_REGULATOR_EARLY_EXIT) != 0;
+  const double* ws = (b.wstats && (b.flags & PYCWB_REGULATOR_BOUNDS))
+                         ? b.wstats + (int64_t)c * kWStats : nullptr;
+  const double den = (double)P + 0.01;
+  const double thr_hi = b.gamma * b.gamma * den * (1.0 + 1e-9);
+  const double thr_lo = b.gamma * b.gamma * den * (1.0 - 1e-9);
+  const double sum_lo = 1.0 - (double)P * 1.2e-7, sum_hi = 1.0 + (double)P * 1.2e-7;
+  const int64_t t0 = ((int64_t)item * (BLOCK / 64) + wave) * kList;
+  unsigned cnt = 0;
+
+  // pass 1: bounds and the check before the first pixel
+  int n_open = 0;
+  for (int g = 0; g < kRegPool; ++g) {
+    const int64_t t = t0 + g * 64 + lane;
+    const bool active = t < b.n_valid;
+    int64_t k = active ? (b.perm ? (int64_t)b.perm[(int64_t)net * b.n_valid + t] : t) : 0;
+    k = k < 0 ? 0 : (k >= b.n_valid ? b.n_valid - 1 : k);
+    const int64_t s = b.valid[k];
+    bool open = active;
+    float lb = 0.f, ub = -1.f;
+    if (active && ws) {
+      float fp0[N], fx0[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        fp0[i] = FP[s * N + i];
+        fx0[i] = FX[s * N + i];
+      }
+      const RegBounds bd = regulator_bounds<N>(fp0, fx0, ws);
+      if (early) {
+        if ((double)P * bd.t_lb * sum_lo > thr_hi) {
+          cnt += 1u;
+          open = false;
+        } else if (bd.t_ub >= 0.0 && (double)P * bd.t_ub * sum_hi < thr_lo) {
+          open = false;
+        }
+      }
+      lb = __double2float_rd(bd.t_lb);
+      ub = bd.t_ub >= 0.0 ? __double2float_ru(bd.t_ub) : -1.f;
+    }
+    const unsigned long long m = __ballot(open);
+    if (open) {
+      const int slot = n_open + lane_rank(m);
+      s_dir[wave][slot] = (int)s;
+      s_lb[wave][slot] = lb;
+      s_ub[wave][slot] = ub;
+    }
+    n_open += __popcll(m);
+  }
+  __syncthreads();
+
+  // pass 2: the sweep, a listed direction per idle lane
+  int next = 0;
+  bool busy = false;
+  int j = 0;
+  float NI = 0.f, lb = 0.f, ub = -1.f;
+  unsigned NN = 0;
+  float fp0[N], fx0[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) fp0[i] = fx0[i] = 0.f;
+  for (;;) {
+    const unsigned long long idle = __ballot(!busy);
+    if (!busy) {
+      const int slot = next + lane_rank(idle);
+      if (slot < n_open) {
+        const int64_t s = s_dir[wave][slot];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          fp0[i] = FP[s * N + i];
+          fx0[i] = FX[s * N + i];
+        }
+        lb = s_lb[wave][slot];
+        ub = s_ub[wave][slot];
+        j = 0;
+        NI = 0.f;
+        NN = 0u;
+        busy = true;
+      }
+    }
+    next += __popcll(idle);
+    if (__ballot(busy) == 0ull) break;  // every listed direction decided
+    if (busy) {
+      const int stop = j + 16 < P ? j + 16 : P;
+      for (; j < stop; ++j) {
+        float r[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = rms[(int64_t)j * N + i];
+        regulator_pixel<N>(r, fp0, fx0, NI, NN);
+      }
+      if (j >= P) {
+        const double q = __builtin_sqrt((double)NI / ((double)NN + 0.01));
+        cnt += q > b.gamma ? 1u : 0u;
+        busy = false;
+      } else if (early) {
+        const double rest = (double)(P - j);
+        if (((double)NI + rest * (double)lb) * sum_lo > thr_hi) {
+          cnt += 1u;
+          busy = false;
+        } else if (ub >= 0.f && ((double)NI + rest * (double)ub) * sum_hi < thr_lo) {
+          busy = false;
+        }
+      }
+    }
+  }
+  block_count<BLOCK>(cnt, &b.counts[c]);
+}
+
 template <int N, int BLOCK, int PHASE>
 __device__ __forceinline__ void batch_item(const BatchDev& b, int c, int tile) {
   const int64_t t = (int64_t)tile * BLOCK + threadIdx.x;
@@ -829,14 +969,6 @@ __device__ __forceinline__ void batch_item(const BatchDev& b, int c, int tile) {
       }
     }
     block_count<BLOCK>(cnt, &b.counts[c]);
-  } else if (PHASE == 0) {
-    const double* ws = (b.wstats && (b.flags & PYCWB_REGULATOR_BOUNDS))
-                           ? b.wstats + (int64_t)c * kWStats : nullptr;
-    const unsigned v = active ? regulator_lane<N>(FP, FX, rms, P, s, b.gamma,
-                                                  (b.flags & PYCWB_REGULATOR_EARLY_EXIT) != 0,
-                                                  ws)
-                              : 0u;
-    block_count<BLOCK>(v, &b.counts[c]);
   } else {
     const double En = b.net_En[net];
     const float RR = (float)regulator_value(b.n_valid, b.counts[c], En);
@@ -851,6 +983,22 @@ __device__ __forceinline__ void batch_item(const BatchDev& b, int c, int tile) {
   }
 }
 
+// Work items per cluster: tiles of BLOCK directions (scan, energy count) or
+// groups of kRegPool tiles (regulator).
+template <int PHASE>
+__device__ __forceinline__ int item_count(const BatchDev& b) {
+  return PHASE == 0 ? (b.tiles_per_cluster + kRegPool - 1) / kRegPool : b.tiles_per_cluster;
+}
+
+template <int N, int BLOCK, int PHASE>
+__device__ __forceinline__ void run_item(const BatchDev& b, int c, int item) {
+  if constexpr (PHASE == 0) {
+    regulator_item<N, BLOCK>(b, c, item);
+  } else {
+    batch_item<N, BLOCK, PHASE>(b, c, item);
+  }
+}
+
 template <int N, int BLOCK, int PHASE>
 __global__ __launch_bounds__(BLOCK, (PhaseOccupancy<PHASE, N>::waves)) void k_batch_persistent(
     BatchDev b, int first, int count, unsigned* ticket) {
@@ -862,7 +1010,7 @@ __global__ __launch_bounds__(BLOCK, (PhaseOccupancy<PHASE, N>::waves)) void k_ba
   // delay tuples, the same amplitude lines) and share its L2.  A queue that
   // runs dry moves on to the next XCD's queue; every queue's counter only
   // grows, so each item is taken once and every workgroup ends.
-  const unsigned tiles = (unsigned)b.tiles_per_cluster;
+  const unsigned tiles = (unsigned)item_count<PHASE>(b);
   const unsigned per = (tiles + 7u) / 8u;
   const unsigned n_items = (unsigned)count * per;
   const int xcd = (int)(blockIdx.x & 7u);
@@ -876,19 +1024,19 @@ __global__ __launch_bounds__(BLOCK, (PhaseOccupancy<PHASE, N>::waves)) void k_ba
       if (item >= n_items) break;
       const unsigned tile = (unsigned)q * per + item % per;
       if (tile >= tiles) continue;
-      batch_item<N, BLOCK, PHASE>(b, b.order[first + (int)(item / per)], (int)tile);
+      run_item<N, BLOCK, PHASE>(b, b.order[first + (int)(item / per)], (int)tile);
     }
   }
 #else
-  const unsigned n_items = (unsigned)count * (unsigned)b.tiles_per_cluster;
+  const unsigned tiles = (unsigned)item_count<PHASE>(b);
+  const unsigned n_items = (unsigned)count * tiles;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
     __syncthreads();
     const unsigned item = s_item;
     __syncthreads();
     if (item >= n_items) break;  // every block reaches this once the tickets run out
-    batch_item<N, BLOCK, PHASE>(b, b.order[first + (int)(item / (unsigned)b.tiles_per_cluster)],
-                                (int)(item % (unsigned)b.tiles_per_cluster));
+    run_item<N, BLOCK, PHASE>(b, b.order[first + (int)(item / tiles)], (int)(item % tiles));
   }
 #endif
 }
@@ -1125,7 +1273,7 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
   if (!d || !out) return PYCWB_ERR_ARG;
   *out = nullptr;
   if (d->n_clusters < 1 || d->n_sky < 1 || d->n_valid < 1 || d->n_valid > d->n_sky ||
-      d->n_valid >= (int64_t)0xffffffff)
+      d->n_sky >= (int64_t)0x7fffffff)
     return PYCWB_ERR_ARG;
   if (d->block != 64 && d->block != 128 && d->block != 256) return PYCWB_ERR_ARG;
   if (d->grid < 1) return PYCWB_ERR_ARG;

@@ -85,6 +85,36 @@ def test_batched_with_sky_mask(oracle):
         assert r["l_max"][c] in set(vh.tolist())
 
 
+@pytest.mark.parametrize("gamma", [0.1667, 0.12, 0.25])
+def test_regulator_count_over_many_directions(oracle, gamma):
+    """The regulator's per-wavefront direction pools at order 5 (12,288
+    directions, 24 pools of 512): per-direction sweeps that end at very
+    different pixels (gamma near the middle of the quality distribution),
+    counts equal to the oracle's full sweep."""
+    from pycwb_amd.batch import BatchedSkyScan, net_tables, synth_batch
+
+    print(f"case regulator pools gamma={gamma}", flush=True)
+    nets = _nets(5, 8)
+    sizes = np.array([700, 1, 300, 64], dtype=np.int64)
+    ids = np.array([0, 0, 1, 1])
+    dev = torch.device("cuda", 0)
+    b = synth_batch(nets, sizes, ids, seed=31, snr=4.0, device=dev)
+    nt = net_tables(nets, device=dev)
+    vh = np.arange(nets[0].n_sky, dtype=np.int64)
+    valid = torch.from_numpy(vh).to(dev)
+    bs = BatchedSkyScan(b, nt, valid, gamma, 0.5, 0.5, block=64, grid=40)
+    bs.launch()
+    r = bs.results()
+    for c, k in enumerate(ids):
+        net = nets[k]
+        rms = b.cluster_rms(c).cpu().numpy()
+        reg, q = oracle.calculate_dpf(net.FP_t, net.FX_t, rms, net.n_sky, net.n_ifo, gamma,
+                                      nt.En[k], vh, return_quality=True)
+        assert r["regulator"][c] == reg, c
+        if c == 0:  # a mixed sky: the count decides direction by direction
+            assert 0.3 < float(np.mean(q > gamma)) < 0.95
+
+
 def test_exact_skip_and_order_are_bit_identical():
     """The masked-pixel skip, the tile energy bound, the regulator early exit
     and the delay-tuple lane order change nothing in the results."""
