@@ -708,6 +708,7 @@ __global__ void k_argmax_finish(const unsigned long long* __restrict__ best,
 struct BatchDev {
   int n_clusters;
   int tiles_per_cluster;
+  int reg_pool;             // tiles per regulator work item (1, 2, 4 or kRegPool)
   int64_t S;
   const int64_t* valid;
   int64_t n_valid;
@@ -788,8 +789,8 @@ struct PhaseOccupancy {
                  : (PHASE == 1 ? (N <= 3 ? PYCWB_OCC_SCAN_SMALL : PYCWB_OCC_SCAN_LARGE) : 8);
 };
 
-// Regulator work item: kRegPool tiles of BLOCK directions, each wavefront
-// taking kRegPool * 64 consecutive delay-order positions.  A direction's sweep
+// Regulator work item: reg_pool (at most kRegPool) tiles of BLOCK directions,
+// each wavefront taking reg_pool * 64 consecutive delay-order positions.  A direction's sweep
 // stops as soon as its term bounds decide its count (regulator_lane), so sweep
 // lengths differ widely between neighbouring directions and a wavefront that
 // swept one tile ran until its slowest lane ended (at order 7 that doubles the
@@ -829,12 +830,13 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
   const double thr_hi = b.gamma * b.gamma * den * (1.0 + 1e-9);
   const double thr_lo = b.gamma * b.gamma * den * (1.0 - 1e-9);
   const double sum_lo = 1.0 - (double)P * 1.2e-7, sum_hi = 1.0 + (double)P * 1.2e-7;
-  const int64_t t0 = ((int64_t)item * (BLOCK / 64) + wave) * kList;
+  const int pool = b.reg_pool;
+  const int64_t t0 = ((int64_t)item * (BLOCK / 64) + wave) * (pool * 64);
   unsigned cnt = 0;
 
   // pass 1: bounds and the check before the first pixel
   int n_open = 0;
-  for (int g = 0; g < kRegPool; ++g) {
+  for (int g = 0; g < pool; ++g) {
     const int64_t t = t0 + g * 64 + lane;
     const bool active = t < b.n_valid;
     int64_t k = active ? (b.perm ? (int64_t)b.perm[(int64_t)net * b.n_valid + t] : t) : 0;
@@ -984,10 +986,11 @@ __device__ __forceinline__ void batch_item(const BatchDev& b, int c, int tile) {
 }
 
 // Work items per cluster: tiles of BLOCK directions (scan, energy count) or
-// groups of kRegPool tiles (regulator).
+// groups of reg_pool tiles (regulator).
 template <int PHASE>
 __device__ __forceinline__ int item_count(const BatchDev& b) {
-  return PHASE == 0 ? (b.tiles_per_cluster + kRegPool - 1) / kRegPool : b.tiles_per_cluster;
+  return PHASE == 0 ? (b.tiles_per_cluster + b.reg_pool - 1) / b.reg_pool
+                    : b.tiles_per_cluster;
 }
 
 template <int N, int BLOCK, int PHASE>
@@ -1296,6 +1299,14 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
   BatchDev& v = b->dev;
   v.n_clusters = d->n_clusters;
   v.tiles_per_cluster = (int)tiles;
+  // The largest regulator pool that still leaves at least 16 work items per
+  // persistent workgroup: with fewer, the largest clusters' last items decide
+  // the stage time (16 clusters of the config-2 size distribution: regulator
+  // 7.1 ms with one direction per lane, 9.1 ms with pools of 2 tiles).
+  v.reg_pool = kRegPool;
+  while (v.reg_pool > 1 &&
+         (int64_t)d->n_clusters * ((tiles + v.reg_pool - 1) / v.reg_pool) < 16 * (int64_t)d->grid)
+    v.reg_pool /= 2;
   v.S = d->n_sky;
   v.valid = d->sky_valid_indices;
   v.n_valid = d->n_valid;

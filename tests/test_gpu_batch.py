@@ -85,15 +85,17 @@ def test_batched_with_sky_mask(oracle):
         assert r["l_max"][c] in set(vh.tolist())
 
 
-@pytest.mark.parametrize("gamma", [0.1667, 0.12, 0.25])
-def test_regulator_count_over_many_directions(oracle, gamma):
+@pytest.mark.parametrize("gamma,grid", [(0.1667, 5), (0.12, 10), (0.25, 20), (0.1667, 400)])
+def test_regulator_count_over_many_directions(oracle, gamma, grid):
     """The regulator's per-wavefront direction pools at order 5 (12,288
-    directions, 24 pools of 512): per-direction sweeps that end at very
-    different pixels (gamma near the middle of the quality distribution),
-    counts equal to the oracle's full sweep."""
+    directions, 192 tiles of 64 per cluster): per-direction sweeps that end
+    at very different pixels (gamma near the middle of the quality
+    distribution), counts equal to the oracle's full sweep.  The pool size
+    follows the batch's work-item count: 8 tiles at grid 5, 4 at grid 10, 2
+    at grid 20, 1 (one direction per lane) at grid 400."""
     from pycwb_amd.batch import BatchedSkyScan, net_tables, synth_batch
 
-    print(f"case regulator pools gamma={gamma}", flush=True)
+    print(f"case regulator pools gamma={gamma} grid={grid}", flush=True)
     nets = _nets(5, 8)
     sizes = np.array([700, 1, 300, 64], dtype=np.int64)
     ids = np.array([0, 0, 1, 1])
@@ -102,7 +104,7 @@ def test_regulator_count_over_many_directions(oracle, gamma):
     nt = net_tables(nets, device=dev)
     vh = np.arange(nets[0].n_sky, dtype=np.int64)
     valid = torch.from_numpy(vh).to(dev)
-    bs = BatchedSkyScan(b, nt, valid, gamma, 0.5, 0.5, block=64, grid=40)
+    bs = BatchedSkyScan(b, nt, valid, gamma, 0.5, 0.5, block=64, grid=grid)
     bs.launch()
     r = bs.results()
     for c, k in enumerate(ids):
