@@ -759,12 +759,11 @@ __global__ __launch_bounds__(256) void k_batch_weights(BatchDev b) {
 // Waves per SIMD the compiler must leave room for (caps VGPRs), per phase and
 // detector count (small: up to three detectors).  Measured on MI355X, the
 // regulator runs fastest at 5 and the scan at 4 for both three and four
-// detectors: at four detectors the scan spills about 20 VGPRs there, and
-// still beats the spill-free 3-wave build by 3 % (occupancy hides more gather
-// latency than the spill traffic costs).
-// Work queues per group and phase: 8, one per XCD (1.2 % faster at config 2
-// than one shared queue: 1839 against 1862 ms per step), or 1 with
-// PYCWB_XCD_QUEUES=0.
+// detectors (a 3-wave scan build, which spilled nothing when the 4-wave one
+// still spilled, was 3 % slower: occupancy hides gather latency).
+// Work queues per group and phase: 8, one per XCD (1.2 % faster than one
+// shared queue on 4,096 clusters of 16-4,096 pixels at HEALPix order 7: 1839
+// against 1862 ms per step), or 1 with PYCWB_XCD_QUEUES=0.
 #ifndef PYCWB_XCD_QUEUES
 #define PYCWB_XCD_QUEUES 1
 #endif
@@ -1301,8 +1300,8 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
   v.tiles_per_cluster = (int)tiles;
   // The largest regulator pool that still leaves at least 16 work items per
   // persistent workgroup: with fewer, the largest clusters' last items decide
-  // the stage time (16 clusters of the config-2 size distribution: regulator
-  // 7.1 ms with one direction per lane, 9.1 ms with pools of 2 tiles).
+  // the stage time (16 clusters of 16-4,096 pixels at HEALPix order 7:
+  // regulator 7.1 ms with one direction per lane, 9.1 ms with pools of 2).
   v.reg_pool = kRegPool;
   while (v.reg_pool > 1 &&
          (int64_t)d->n_clusters * ((tiles + v.reg_pool - 1) / v.reg_pool) < 16 * (int64_t)d->grid)
