@@ -4,8 +4,7 @@ Restates, without healpy or astropy:
 
 * ``_build_sky_directions`` (pycwb/types/detector.py:1220-1240): the HEALPix RING
   grid of order ``k`` (nside = 2**k) in cWB's Earth-fixed frame, returned as
-  ``(phi_geo, latitude)`` in radians.  ``pix2ang_ring`` follows Gorski et al.
-  2005 (ApJ 622, 759), with the polar-cap form of healpix_cxx for accuracy.
+  ``(phi_geo, latitude)`` in radians (``healpix.pix2ang_ring``).
 * ``earth_centered_vectors`` (pycwb/types/detector.py:1061-1123): WGS84
   geodetic -> ECEF vertex (the ellipsoid formula behind astropy's
   ``EarthLocation.from_geodetic``) and the arm response tensors built with
@@ -24,6 +23,7 @@ import math
 import numpy as np
 
 from .detectors import DETECTORS
+from .healpix import pix2ang_ring  # noqa: F401  (the grid of build_sky_directions)
 
 C_LIGHT = 299792458.0          # m/s (astropy.constants.c)
 WGS84_A = 6378137.0            # m
@@ -37,56 +37,6 @@ def nside_of(order: int) -> int:
 def npix_of(order: int) -> int:
     n = nside_of(order)
     return 12 * n * n
-
-
-def pix2ang_ring(nside: int, ipix: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    """HEALPix RING pixel -> (theta, phi) radians (Gorski et al. 2005, eqs. 4-9)."""
-    ipix = np.asarray(ipix, dtype=np.int64)
-    npix = 12 * nside * nside
-    ncap = 2 * nside * (nside - 1)
-    theta = np.empty(ipix.shape, dtype=np.float64)
-    phi = np.empty(ipix.shape, dtype=np.float64)
-    fact2 = 4.0 / npix
-    fact1 = (nside << 1) * fact2
-
-    north = ipix < ncap
-    south = ipix >= npix - ncap
-    equ = ~(north | south)
-
-    if np.any(north):
-        p = ipix[north]
-        iring = (1 + _isqrt(1 + 2 * p)) >> 1
-        iphi = p + 1 - 2 * iring * (iring - 1)
-        tmp = (iring * iring).astype(np.float64) * fact2
-        z = 1.0 - tmp
-        sth = np.sqrt(tmp * (2.0 - tmp))
-        theta[north] = np.arctan2(sth, z)
-        phi[north] = (iphi - 0.5) * (0.5 * np.pi / iring)
-    if np.any(equ):
-        p = ipix[equ] - ncap
-        iring = p // (4 * nside) + nside
-        iphi = p % (4 * nside) + 1
-        fodd = np.where(((iring + nside) & 1) != 0, 1.0, 0.5)
-        z = (2 * nside - iring) * fact1
-        theta[equ] = np.arccos(z)
-        phi[equ] = (iphi - fodd) * (np.pi * 0.75 * fact1)
-    if np.any(south):
-        p = npix - ipix[south]
-        iring = (1 + _isqrt(2 * p - 1)) >> 1
-        iphi = 4 * iring + 1 - (p - 2 * iring * (iring - 1))
-        tmp = (iring * iring).astype(np.float64) * fact2
-        z = tmp - 1.0
-        sth = np.sqrt(tmp * (2.0 - tmp))
-        theta[south] = np.arctan2(sth, z)
-        phi[south] = (iphi - 0.5) * (0.5 * np.pi / iring)
-    return theta, phi
-
-
-def _isqrt(v: np.ndarray) -> np.ndarray:
-    r = np.floor(np.sqrt(v.astype(np.float64))).astype(np.int64)
-    r = np.where(r * r > v, r - 1, r)
-    r = np.where((r + 1) * (r + 1) <= v, r + 1, r)
-    return r
 
 
 def build_sky_directions(n_sky: int, healpix_order: int | None = None):
