@@ -280,20 +280,29 @@ def evaluate_cluster_likelihood(nIFO, cluster, config, MRAcatalog=None, strains=
                              " instead.")
         xtalk = XTalk.load(MRAcatalog, dump=True)
     if setup is None:
-        sc_ml = sc_fp = sc_fx = None
-        if supercluster_setup is not None:
-            sc_ml = supercluster_setup.get("ml_likelihood", supercluster_setup.get("ml"))
-            sc_fp = supercluster_setup.get("FP_likelihood", supercluster_setup.get("FP"))
-            sc_fx = supercluster_setup.get("FX_likelihood", supercluster_setup.get("FX"))
+        sc = supercluster_setup or {}
+        sc_ml = sc.get("ml_likelihood", sc.get("ml"))
+        sc_fp = sc.get("FP_likelihood", sc.get("FP"))
+        sc_fx = sc.get("FX_likelihood", sc.get("FX"))
         if strains is None and sc_ml is None:
             raise ValueError("likelihood(): setup, strains, or supercluster_setup must be "
                              "provided.")
-        setup = prepare_likelihood_inputs(config, strains, nIFO, ml=sc_ml, FP=sc_fp, FX=sc_fx)
+        # the coarse grid of big clusters comes from the supercluster stage too
+        # (likelihood.py:212-219)
+        setup = prepare_likelihood_inputs(
+            config, strains, nIFO, ml=sc_ml, FP=sc_fp, FX=sc_fx,
+            ml_big=sc.get("ml_big_cluster"), FP_big=sc.get("FP_big_cluster"),
+            FX_big=sc.get("FX_big_cluster"),
+            big_cluster_healpix_order=sc.get("big_cluster_healpix_order"))
     if config is None:
         raise ValueError("likelihood(): config must be provided.")
     t_start = time.perf_counter()
     dev = require_gpu()
     pa = cluster.pixel_arrays
+    # noise floor of each pixel from the TF noise maps (likelihood.py:260-263):
+    # the noise weights below are derived from it
+    if nRMS is not None and len(nRMS) == nIFO:
+        pa.populate_noise_rms(nRMS)
     n_pix = len(pa)
     big = _select_grid(setup, config, n_pix)
     nets = _device_net(setup, big, dev)
@@ -381,17 +390,26 @@ def evaluate_fragment_clusters(config, fragment_clusters, strains, MRAcatalog, n
             cl.cluster_id = k + 1
             todo.append((lag, k, cl))
     results = [[] for _ in fragment_clusters]
-    # big-grid clusters and ragged TD tables take the per-cluster path
+    # The batch scans every cluster over the segment's valid set on the fine
+    # grid.  Big-grid clusters, and clusters whose mask differs from the
+    # segment's at their own time (ICRS masks), take the per-cluster path.
+    seg_valid = np.asarray(setup["sky_valid_indices"], dtype=np.int64)
     batchable, single = [], []
     for item in todo:
-        (single if _select_grid(setup, config, len(item[2].pixel_arrays)) else batchable).append(
-            item)
+        cl = item[2]
+        big = _select_grid(setup, config, len(cl.pixel_arrays))
+        own = None if big else sky_valid_indices_for_cluster(setup, cl)
+        same = own is None or np.array_equal(np.asarray(own, dtype=np.int64), seg_valid)
+        (batchable if not big and same else single).append(item)
     accepted = {}
     if batchable:
+        if nRMS is not None and len(nRMS) == nIFO:
+            for _, _, cl in batchable:
+                cl.pixel_arrays.populate_noise_rms(nRMS)
         accepted.update(_evaluate_batch(config, setup, batchable, xtalk, nIFO, dev))
     for lag, k, cl in single:
-        res = evaluate_cluster_likelihood(nIFO, cl, config, cluster_id=k + 1, setup=setup,
-                                          xtalk=xtalk)
+        res = evaluate_cluster_likelihood(nIFO, cl, config, cluster_id=k + 1, nRMS=nRMS,
+                                          setup=setup, xtalk=xtalk)
         if res[0] is not None:
             accepted[(lag, k)] = res
     for (lag, k) in sorted(accepted):

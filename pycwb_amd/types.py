@@ -99,6 +99,23 @@ class PixelArrays:
             raise ValueError("td_amp vectors have non-uniform lengths; cannot form a dense array")
         return self.td_amp_flat[:rows * tsize].reshape(P, n, tsize)
 
+    def populate_noise_rms(self, nRMS: list) -> None:
+        """Per-pixel noise floor from the detectors' TF noise maps
+        (pixel_arrays.py:367-399): map row round(f * nf / layers), column
+        time // layers (both clipped to the map), |value| as float32, kept only
+        where positive."""
+        freq = self.frequency.astype(np.int64)
+        layers = self.layers.astype(np.int64)
+        tbin = np.where(layers > 0, self.time.astype(np.int64) // np.maximum(layers, 1), 0)
+        for i in range(self._n_ifo):
+            arr = np.asarray(nRMS[i].data, dtype=np.float64)
+            nf, nt = arr.shape
+            fb = np.where(layers > 0,
+                          np.round(freq * nf / np.maximum(layers, 1)).astype(np.int64), 0)
+            vals = np.abs(arr[np.clip(fb, 0, nf - 1), np.clip(tbin, 0, nt - 1)]).astype(
+                np.float32)
+            self.noise_rms[i] = np.where(vals > 0.0, vals, self.noise_rms[i])
+
     def set_waveform_data(self, wave, w_90, asnr, a_90, core_mask, energy_plus, energy_cross):
         """pixel_arrays.py:264-289."""
         self.wave[:] = np.asarray(wave, dtype=np.float32)
