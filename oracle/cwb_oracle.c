@@ -29,6 +29,13 @@
 
 #define NMAPS 11
 
+/* Planted-mistake switch.  0 in every use except tests/test_oracle_pins.py,
+ * which sets it to k for one check at a time to show that the closed-form pins
+ * catch the k-th plausible transcription slip (a flipped sign, a dropped term
+ * or factor, a wrong index or operand).  Each MUT(k) site names its slip. */
+int oracle_mutant = 0;
+#define MUT(k) (oracle_mutant == (k))
+
 /* ------------------------------------------------------------------------- */
 /* dpf_np_loops_vec (likelihoodWP/dpf.py:342-453), mixed precision.          */
 /* rms is (P, n) row-major.  f, F are (P, n); fp, fx, si, co, ni are (P).    */
@@ -61,6 +68,7 @@ static double dpf_mixed(const float *Fp0, const float *Fx0, const float *rms,
         fp[i] = (AP + nn) / 2.f;            /* avg_vec */
         si[i] = sqrtf((1.f - cc) / 2.f);    /* sin_from_cc */
         co[i] = s > 0.f ? sqrtf((1.f + cc) / 2.f) : -sqrtf((1.f + cc) / 2.f);
+        if (MUT(6)) co[i] = -co[i];                   /* slip: rotation sign */
     }
     /* dpf.py:423-438 */
     for (int i = 0; i < P; ++i) {
@@ -78,7 +86,7 @@ static double dpf_mixed(const float *Fp0, const float *Fx0, const float *rms,
             F[i * n + j] -= f[i * n + j] * t;
             fx[i] += F[i * n + j] * F[i * n + j];
             float f2 = f[i * n + j] * f[i * n + j];   /* float32 ** 4 = (x*x)*(x*x) */
-            ni[i] += f2 * f2;
+            ni[i] += MUT(5) ? f2 * f[i * n + j] : f2 * f2;   /* slip: f^3 */
         }
     }
     /* dpf.py:441-453 */
@@ -270,7 +278,7 @@ static float load_data_mixed(const float *v00, const float *v90, int n, int P, d
         EE += et[i];
     }
     *M = NN;
-    return EE / 2.f;
+    return MUT(7) ? EE : EE / 2.f;                   /* slip: dropped 1/2 */
 }
 
 /* avx_GW_ps (sky_stat.py:92-219).  Returns NN (number of mask pixels). */
@@ -300,7 +308,7 @@ static int64_t gw_mixed(const float *v00, const float *v90, const float *f, cons
         h = h * h + H * H;
         float Hx = xx * xx + XX * XX;
         double Fd = sqrt((double)Hx / (h + (double)o));
-        float R = 0.1f + RR / (et[i] + o);
+        float R = MUT(1) ? 0.1f - RR / (et[i] + o) : 0.1f + RR / (et[i] + o); /* slip: sign */
         Fd = Fd * (double)R - (double)fx[i];
         Fd = Fd > 0.0 ? Fd : 0.0;
         Fd = (double)mask[i] / ((double)fx[i] + Fd + (double)o);
@@ -312,7 +320,8 @@ static int64_t gw_mixed(const float *v00, const float *v90, const float *f, cons
         NN += mask[i];
         mk[i] = (float)(a + (double)mask[i] - 1.0);
         for (int j = 0; j < n; ++j) {
-            p[j * P + i] = f[i * n + j] * au[i] + F[i * n + j] * av[i];
+            p[j * P + i] = MUT(2) ? f[i * n + j] * au[i] - F[i * n + j] * av[i]  /* slip */
+                                  : f[i * n + j] * au[i] + F[i * n + j] * av[i];
             q[j * P + i] = f[i * n + j] * AU[i] + F[i * n + j] * AV[i];
         }
     }
@@ -377,19 +386,19 @@ static stat_out stat_mixed(const float *v00, const float *v90, const float *s, c
             C += A * A;
             ss += s_ * s_;
             SS += S_ * S_;
-            float d = s[j * P + i] - v00[j * P + i];
+            float d = s[j * P + i] - (MUT(3) ? v90 : v00)[j * P + i];   /* slip: index */
             float D = S[j * P + i] - v90[j * P + i];
             rr += d * d;
             RR += D * D;
         }
         int64_t mk = mask[i] >= 0.f ? 1 : 0;
-        c = c / (xs * xs + o);
+        c = c / ((MUT(4) ? XS * XS : xs * xs) + o);   /* slip: operand */
         C = C / (XS * XS + o);
         double ll = (double)mk * (double)(ss + SS);
         double ssd = (double)ss * (1.0 - (double)c);
         double SSd = (double)SS * (1.0 - (double)C);
         ec[i] = (float)((double)mk * (ssd + SSd));
-        gn[i] = (float)(((double)mk * 2.0) * (double)mask[i]);
+        gn[i] = (float)(((double)mk * (MUT(9) ? 1.0 : 2.0)) * (double)mask[i]);  /* slip */
         rn[i] = (float)((double)mk * (double)(rr + RR));
         double a = 2.0 * fabs((double)ec[i]);
         float A = rn[i] + gn[i] + o;
@@ -457,7 +466,7 @@ void oracle_scan_sky(int n_ifo, int n_pix, int n_sky, int n_delay, const float *
             stat_out st = stat_mixed(t.v00, t.v90, t.p, t.q, t.si2, t.co2, t.mk, n, P, t.ec,
                                      t.gn, t.rn);
             if (M == 0) continue; /* defined behaviour: gate out (reference: 0/0 raises) */
-            double CH = (double)st.No / ((double)(n * M) + sqrt((double)M));
+            double CH = (double)st.No / ((double)(n * M) + (MUT(8) ? 0.0 : sqrt((double)M)));
             double cc = CH > 1.0 ? CH : 1.0;
             double Co = (double)st.EC / (((double)st.EC + (double)st.No * cc) -
                                          (double)(M * (n - 1)));
@@ -712,7 +721,7 @@ static double packet_mixed(const float *v00, const float *v90, const float *mask
         float _ss = _si * _si;
         float _nn = sqrtf(_cc + _ss);
         float a = (float)sqrt((_x + (double)_nn) / 2.0);
-        float A = (float)sqrt(fabs((_x - (double)_nn) / 2.0));
+        float A = (float)sqrt(fabs((_x + (MUT(10) ? 0.0 : -(double)_nn)) / 2.0)); /* slip */
         double cc = (double)_co / ((double)_nn + o);
         float sgn = _si > 0.f ? 1.f : -1.f;
         si[j] = (float)sqrt(((double)1.f - cc) / 2.0);
@@ -863,7 +872,7 @@ void oracle_sky_statistics(int n_ifo, int n_pix, int n_sky, int n_delay, int64_t
         double gn = t.gn[i];
         double rc = gn < 2.0 ? 1.0 : 0.0;
         double nn = gn * (1.0 - rc);
-        rc = rc + nn * 0.5;
+        rc = rc + nn * (MUT(11) ? 1.0 : 0.5);                        /* slip: factor */
         rc = (double)t.ec[i] / (rc + 1e-9);
         double sig = mk * (ns > 0 ? 1.0 : 0.0);
         t.gn[i] = (float)(mk * gn * nx);
@@ -896,7 +905,7 @@ void oracle_sky_statistics(int n_ifo, int n_pix, int n_sky, int n_delay, int64_t
             }
             Npp += col * (double)mk;
         }
-        if (!pass) Np_eff = Npp * 4 / n - 1;
+        if (!pass) Np_eff = Npp * 4 / n - (MUT(12) ? 0 : 1);     /* slip: dropped -1 */
     }
     /* avx_loadNULL_ps: f32(data) - f32(signal) */
     float *n00 = malloc(sizeof(float) * n * P * 2), *n90 = n00 + n * P;
@@ -909,7 +918,7 @@ void oracle_sky_statistics(int n_ifo, int n_pix, int n_sky, int n_delay, int64_t
     }
     double Em = xtalk_sum_f64(sd00, sd90, lookup, xt, t.mk, n, P);
     double Np = xtalk_sum_f64(dn00, dn90, lookup, xt, t.mk, n, P);
-    double Lm = Em - Np - Gn;
+    double Lm = MUT(13) ? Em - Np + Gn : Em - Np - Gn;        /* slip: sign */
     double nrm = Em > 0 ? (Eo - Eh) / Em : 1.e9;
     if (nrm < 1) nrm = 1;
     Ec /= nrm;
