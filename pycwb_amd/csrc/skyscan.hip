@@ -353,27 +353,28 @@ __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
 // (pixel, owner) pair with the delayed amplitudes they loaded; once 64 pairs
 // wait, every lane evaluates one of them (pixel_rest) and stores its
 // contributions, and each owner adds its own contributions to its sums in
-// queue order, which is its pixel order.  Pairs beyond the first 64 move to
-// the front and wait for the next round.  The sums therefore see exactly the
-// terms of the per-lane sweep in the same order (bit-identical results) while
-// the expensive chain runs with every lane busy.
+// queue order, which is its pixel order (a direction enqueues at most one pair
+// per pixel, and pixels are enqueued in increasing order).  Pairs beyond the
+// first 64 move to the front and wait for the next round.  The sums therefore
+// see exactly the terms of the per-lane sweep in the same order (bit-identical
+// results) while the expensive chain runs with every lane busy.
 constexpr int kQueue = 128;  // pair slots: at most 63 waiting + 64 of one pixel
 
 template <int N>
 struct WaveQueue {
   float qv[kQueue][2 * N];        // v00 (0..N) and v90 (N..2N) of each pair
   int qjd[kQueue];                // (pixel << 6) | owner lane
-  unsigned long long eb[kQueue];  // owners of each queued pixel event
-  int ebase[kQueue];              // first pair slot of each event
+  unsigned long long own[64];     // per owner lane: its slots among those evaluated
   float pat[64][2 * N];           // F+ (0..N) and Fx (N..2N) of each lane's direction
   float c_et[64], c_ec[64], c_gn[64], c_rn[64], c_fpet[64], c_fxet[64];
   double c_llcc[64], c_ll[64];
   int c_flags[64];                // bit 0: m, bit 1: ec > 1e-9, bit 2: mk > 0
 };
 
-// Evaluate slots [0, cnt), add them to their owners' sums, drop them from the queue.
+// Evaluate slots [0, cnt), add them to their owners' sums, drop them from the
+// queue.  own[] is zero on entry and on exit.
 template <int N>
-__device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, int& E, int lane,
+__device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, int lane,
                                             const float* __restrict__ rms, int P, float rr,
                                             float RR, double En, SkySums& acc) {
   if (lane < cnt) {
@@ -404,37 +405,33 @@ __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, in
     q.c_llcc[lane] = o.ll * o.ccd;
     q.c_ll[lane] = o.ll;
     q.c_flags[lane] = o.m | (o.ec > 1e-9f ? 2 : 0) | (o.mk > 0.f ? 4 : 0);
+    __hip_atomic_fetch_or(&q.own[d], 1ull << lane, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   wave_sync();
-  const unsigned long long below = lanemask_lt(lane);
-  int kdrop = 0;
-  for (int e = 0; e < E; ++e) {
-    const int base = q.ebase[e];
-    if (base >= cnt) break;
-    const unsigned long long bm = q.eb[e];
-    if (base + __popcll(bm) <= cnt) kdrop = e + 1;
-    if ((bm >> lane) & 1ull) {
-      const int slot = base + __popcll(bm & below);
-      if (slot < cnt) {
-        const int f = q.c_flags[slot];
-        const float et = q.c_et[slot];
-        acc.M += f & 1;
-        acc.EE += et;
-        acc.Lr += q.c_llcc[slot];
-        acc.LL += q.c_ll[slot];
-        acc.EC += q.c_ec[slot];
-        acc.GN += q.c_gn[slot];
-        acc.RN += q.c_rn[slot];
-        acc.Mp += (f & 2) ? 1.0 : 0.0;
-        if (f & 4) {
-          acc.sde += et;
-          acc.pwe += q.c_fpet[slot];
-          acc.cwe += q.c_fxet[slot];
-        }
-      }
+  // each owner walks its own slots in increasing order
+  unsigned long long mine = q.own[lane];
+  q.own[lane] = 0ull;
+  while (mine) {
+    const int slot = __ffsll((long long)mine) - 1;
+    mine &= mine - 1ull;
+    const int f = q.c_flags[slot];
+    const float et = q.c_et[slot];
+    acc.M += f & 1;
+    acc.EE += et;
+    acc.Lr += q.c_llcc[slot];
+    acc.LL += q.c_ll[slot];
+    acc.EC += q.c_ec[slot];
+    acc.GN += q.c_gn[slot];
+    acc.RN += q.c_rn[slot];
+    acc.Mp += (f & 2) ? 1.0 : 0.0;
+    if (f & 4) {
+      acc.sde += et;
+      acc.pwe += q.c_fpet[slot];
+      acc.cwe += q.c_fxet[slot];
     }
   }
-  // move the waiting pairs and events to the front
+  // move the waiting pairs to the front
   const bool mv = lane + cnt < T;
   float v[2 * N];
   int jd = 0;
@@ -443,34 +440,12 @@ __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, in
     for (int i = 0; i < 2 * N; ++i) v[i] = q.qv[lane + cnt][i];
     jd = q.qjd[lane + cnt];
   }
-  const int ne = E - kdrop;
-  unsigned long long eb0 = 0ull, eb1 = 0ull;
-  int bs0 = 0, bs1 = 0;
-  if (lane < ne) {
-    eb0 = q.eb[kdrop + lane];
-    bs0 = q.ebase[kdrop + lane];
-  }
-  if (lane + 64 < ne) {
-    eb1 = q.eb[kdrop + lane + 64];
-    bs1 = q.ebase[kdrop + lane + 64];
-  }
   wave_sync();
   if (mv) {
 #pragma unroll
     for (int i = 0; i < 2 * N; ++i) q.qv[lane][i] = v[i];
     q.qjd[lane] = jd;
   }
-  if (lane < ne) {
-    bs0 -= cnt;
-    for (; bs0 < 0; ++bs0) eb0 &= eb0 - 1ull;  // the partly evaluated event
-    q.eb[lane] = eb0;
-    q.ebase[lane] = bs0;
-  }
-  if (lane + 64 < ne) {
-    q.eb[lane + 64] = eb1;
-    q.ebase[lane + 64] = bs1 - cnt;
-  }
-  E = ne;
   T -= cnt;
   wave_sync();
 }
@@ -507,8 +482,9 @@ __device__ __forceinline__ unsigned long long scan_lane(
     q.pat[lane][i] = fp0[i];
     q.pat[lane][N + i] = fx0[i];
   }
+  q.own[lane] = 0ull;
   const bool use_bound = skip_masked && tile_bound && span <= kMaxBoundSpan;  // wave-uniform
-  int T = 0, E = 0;  // queued pairs and pixel events (wave-uniform)
+  int T = 0;  // queued pairs (wave-uniform)
   wave_sync();
   SkySums acc;
   sums_init(acc);
@@ -558,15 +534,10 @@ __device__ __forceinline__ unsigned long long scan_lane(
             }
             q.qjd[slot] = (j << 6) | lane;
           }
-          if (lane == 0) {
-            q.eb[E] = bm;
-            q.ebase[E] = T;
-          }
-          E += 1;
           T += __popcll(bm);
           if (T >= 64) {
             wave_sync();
-            queue_flush<N>(q, 64, T, E, lane, rms, P, rr, RR, En, acc);
+            queue_flush<N>(q, 64, T, lane, rms, P, rr, RR, En, acc);
           }
         }
       } else {
@@ -583,7 +554,7 @@ __device__ __forceinline__ unsigned long long scan_lane(
   }
   if (T > 0) {
     wave_sync();
-    queue_flush<N>(q, T, T, E, lane, rms, P, rr, RR, En, acc);
+    queue_flush<N>(q, T, T, lane, rms, P, rr, RR, En, acc);
   }
   SkyStat st;
   sky_epilogue(acc, N, netCC, st);
