@@ -820,7 +820,7 @@ void oracle_sky_statistics(int n_ifo, int n_pix, int n_sky, int n_delay, int64_t
                            const float *REG, double En, const int32_t *lookup, const float *xt,
                            int xgb_rho_mode, double *scal, double *pd, double *pD, double *ps,
                            double *pS, float *nul00, float *nul90, float *pmask, float *gnc,
-                           float *ecv, float *eev, float *EEv, double *S_snr)
+                           float *ecv, float *eev, float *EEv, double *S_snr, float *pol)
 {
     const int n = n_ifo, P = n_pix, S = n_sky, offset = n_delay / 2;
     scan_tmp t = tmp_alloc(n, P);
@@ -931,6 +931,41 @@ void oracle_sky_statistics(int n_ifo, int n_pix, int n_sky, int n_delay, int64_t
         double pen = chi;
         rho = Ec > 0 ? sqrt(Ec / (1 + pen * ((pen > 1 ? pen : 1) - 1))) : 0;
         xrho = Ec > 0 ? sqrt(Ec * Rc / 2.) : 0;
+    }
+    /* avx_pol_ps (packet_ops.py:564-633), first call: p00_POL / p90_POL as
+     * (radius, angle) per pixel, float64 NumPy arithmetic with float32 results;
+     * the row sums over detectors start from 0.0 in detector order (NumPy's
+     * reduction below eight terms; the 8-term pairing for n = 8). */
+    if (pol) {
+        for (int i = 0; i < P; ++i) {
+            double m = t.mk[i] > 0.f ? 1.0 : 0.0, tt[4][8];
+            for (int j = 0; j < n; ++j) {
+                double pj = (double)t.v00[j * P + i] * m, qj = (double)t.v90[j * P + i] * m;
+                tt[0][j] = (double)t.f[i * n + j] * pj;
+                tt[1][j] = (double)t.f[i * n + j] * qj;
+                tt[2][j] = (double)t.F[i * n + j] * pj;
+                tt[3][j] = (double)t.F[i * n + j] * qj;
+            }
+            double sum[4];
+            for (int k = 0; k < 4; ++k) {
+                if (n < 8) {
+                    sum[k] = 0.0;
+                    for (int j = 0; j < n; ++j) sum[k] += tt[k][j];
+                } else {
+                    sum[k] = ((tt[k][0] + tt[k][1]) + (tt[k][2] + tt[k][3])) +
+                             ((tt[k][4] + tt[k][5]) + (tt[k][6] + tt[k][7]));
+                }
+            }
+            double xp = sum[0], XP = sum[1], xx = sum[2], XX = sum[3];
+            double fp = t.fp[i], fx = t.fx[i];
+            double sfp = sqrt(fp) + 1e-9, sfx = sqrt(fx) + 1e-9;
+            double rp = xp * xp / (fp + 1e-9) + xx * xx / (fx + 1e-9);
+            double RP = XP * XP / (fp + 1e-9) + XX * XX / (fx + 1e-9);
+            pol[i] = (float)sqrt(rp);
+            pol[P + i] = (float)atan2(xx / sfx, xp / sfp);
+            pol[2 * P + i] = (float)sqrt(RP);
+            pol[3 * P + i] = (float)atan2(XX / sfx, XP / sfp);
+        }
     }
     double v[16] = {Gn, Ec, Dc, Rc, Eh, Es, Np, Em, Lm, nrm, ccf, rho, xrho, Lo, Eo, Np_eff};
     memcpy(scal, v, sizeof(v));

@@ -168,7 +168,7 @@ def sky_statistics(l, n_ifo, n_pix, n_sky, FP, FX, rms, td00, td90, ml, REG, En,
     fn = lib.oracle_sky_statistics
     fn.restype = None
     fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64] + [vp] * 7 + [C.c_double] + \
-        [vp, vp, C.c_int, vp] + [vp] * 12
+        [vp, vp, C.c_int, vp] + [vp] * 13
     if lookup is None:
         lookup, xt = self_xtalk(n_pix)
     FP, FX, rms = _c(FP, np.float32), _c(FX, np.float32), _c(rms, np.float32)
@@ -180,13 +180,16 @@ def sky_statistics(l, n_ifo, n_pix, n_sky, FP, FX, rms, td00, td90, ml, REG, En,
     arr.update({k: np.zeros((n_ifo, n_pix), np.float32) for k in ("nul00", "nul90")})
     arr.update({k: np.zeros(n_pix, np.float32) for k in ("pixel_mask", "gn", "ec", "ee", "EE")})
     arr["S_snr"] = np.zeros(n_ifo)
+    pol = np.zeros((4, n_pix), np.float32)
     fn(n_ifo, n_pix, n_sky, td00.shape[0], int(l), _p(FP), _p(FX), _p(rms), _p(td00), _p(td90),
        _p(ml), _p(REG), En, _p(lookup), _p(xt), int(bool(xgb_rho_mode)), _p(scal),
        _p(arr["pd"]), _p(arr["pD"]), _p(arr["ps"]), _p(arr["pS"]), _p(arr["nul00"]),
        _p(arr["nul90"]), _p(arr["pixel_mask"]), _p(arr["gn"]), _p(arr["ec"]), _p(arr["ee"]),
-       _p(arr["EE"]), _p(arr["S_snr"]))
+       _p(arr["EE"]), _p(arr["S_snr"]), _p(pol))
     out = dict(zip(SKYSTAT_SCALARS, scal.tolist()))
     out.update(arr)
+    out.update({"p00_POL_r": pol[0], "p00_POL_a": pol[1], "p90_POL_r": pol[2],
+                "p90_POL_a": pol[3]})
     return out
 
 

@@ -26,6 +26,7 @@ constexpr int kBlock = 256;
 struct StatsWs32 {
   float *v00, *v90, *p, *q, *d00, *d90, *s00, *s90, *n00, *n90, *f, *F;
   float *et, *mk, *ee, *EE, *ec, *gn, *fp, *fx;
+  float *pr, *pa, *pR, *pA;   // avx_pol_ps: p00_POL (radius, angle), p90_POL
 };
 struct StatsWs64 {
   double *qn, *pn, *pd, *pD, *ps, *pS, *rnx;
@@ -40,6 +41,7 @@ __device__ inline StatsWs32 ws32(float* b, int n, int P) {
   float* t = b + 12 * nP;
   w.et = t; w.mk = t + P; w.ee = t + 2 * P; w.EE = t + 3 * P; w.ec = t + 4 * P;
   w.gn = t + 5 * P; w.fp = t + 6 * P; w.fx = t + 7 * P;
+  w.pr = t + 8 * P; w.pa = t + 9 * P; w.pR = t + 10 * P; w.pA = t + 11 * P;
   return w;
 }
 
@@ -139,6 +141,51 @@ __device__ void packet_ifo(const float* v00, const float* v90, const float* mask
   Ai_o = (float)(1.0 / ((double)A + o));
 }
 
+// NumPy's float64 sum of the N products of one row (add.reduce over a short
+// contiguous axis: from +0.0 in order below eight terms, the eight-accumulator
+// pairing at eight).
+template <int N>
+__device__ __forceinline__ double row_sum(const double (&t)[N]) {
+  if constexpr (N < 8) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s += t[i];
+    return s;
+  } else {
+    return ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+  }
+}
+
+// avx_pol_ps (packet_ops.py:564-633): polar coordinates of each pixel's masked
+// data in the DPF frame, float64 with float32 results --
+// r = sqrt(xp^2/(fp+1e-9) + xx^2/(fx+1e-9)), a = atan2(xx/(sqrt fx + 1e-9),
+// xp/(sqrt fp + 1e-9)) for the 00 phase (p00_POL) and the same for 90 (p90_POL).
+template <int N>
+__device__ __forceinline__ void pol_pixel(const float (&v00)[N], const float (&v90)[N],
+                                          const PixelEval<N>& o, const StatsWs32& w, int j) {
+  const double o9 = 1.e-9;
+  const double m = o.mk > 0.f ? 1.0 : 0.0;
+  double t0[N], t1[N], t2[N], t3[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double p = (double)v00[i] * m, q = (double)v90[i] * m;
+    t0[i] = (double)o.d.f[i] * p;
+    t1[i] = (double)o.d.f[i] * q;
+    t2[i] = (double)o.d.F[i] * p;
+    t3[i] = (double)o.d.F[i] * q;
+  }
+  const double xp = row_sum<N>(t0), XP = row_sum<N>(t1);
+  const double xx = row_sum<N>(t2), XX = row_sum<N>(t3);
+  const double fp = (double)o.d.fp, fx = (double)o.d.fx;
+  const double sfp = __builtin_sqrt(fp) + o9, sfx = __builtin_sqrt(fx) + o9;
+  const double rp = xp * xp / (fp + o9) + xx * xx / (fx + o9);
+  const double RP = XP * XP / (fp + o9) + XX * XX / (fx + o9);
+  w.pr[j] = (float)__builtin_sqrt(rp);
+  w.pa[j] = (float)atan2(xx / sfx, xp / sfp);
+  w.pR[j] = (float)__builtin_sqrt(RP);
+  w.pA[j] = (float)atan2(XX / sfx, XP / sfp);
+}
+
 template <int N>
 __device__ void stats_cluster(const StatsArgs& g, int c) {
   __shared__ double sh[kBlock / 64 * 16];
@@ -200,6 +247,7 @@ __device__ void stats_cluster(const StatsArgs& g, int c) {
     w.gn[j] = o.gn;
     w.fp[j] = o.d.fp;
     w.fx[j] = o.d.fx;
+    pol_pixel<N>(v00, v90, o, w, j);
   }
   __syncthreads();
   // ---- avx_packet_ps: data packet on lanes [0, N), signal packet on [64, 64+N) ----
@@ -440,9 +488,9 @@ __global__ __launch_bounds__(kBlock) void k_sky_stats(StatsArgs g, const int32_t
 
 extern "C" {
 
-// Scratch per cluster: (12 n + 8) P floats and (6 n + 1) P doubles.
+// Scratch per cluster: (12 n + 12) P floats and (6 n + 1) P doubles.
 size_t pycwb_sky_stats_scratch32(int n_ifo, int n_pix) {
-  return (size_t)(12 * n_ifo + 8) * (size_t)n_pix;
+  return (size_t)(12 * n_ifo + 12) * (size_t)n_pix;
 }
 size_t pycwb_sky_stats_scratch64(int n_ifo, int n_pix) {
   return (size_t)(6 * n_ifo + 1) * (size_t)n_pix;

@@ -185,27 +185,15 @@ def _single_batch(P, n, K, rms_d, packed) -> ClusterBatch:
                         rms=rms_d.reshape(-1), td=packed.reshape(-1))
 
 
-def _pol(v00, v90, mk, fp, fx, f, F):
-    """p00_POL / p90_POL of avx_pol_ps (packet_ops.py:602-633), float64 on the device."""
-    m = (mk > 0).double()
-    p, q = v00.double() * m, v90.double() * m
-    f, F = f.double(), F.double()
-    xp, XP = (f * p.T).sum(1), (f * q.T).sum(1)
-    xx, XX = (F * p.T).sum(1), (F * q.T).sum(1)
-    fp, fx = fp.double(), fx.double()
-    sfp, sfx = torch.sqrt(fp) + 1e-9, torch.sqrt(fx) + 1e-9
-    r = torch.sqrt(xp * xp / (fp + 1e-9) + xx * xx / (fx + 1e-9)).float()
-    R = torch.sqrt(XP * XP / (fp + 1e-9) + XX * XX / (fx + 1e-9)).float()
-    a = torch.atan2(xx / sfx, xp / sfp).float()
-    A = torch.atan2(XX / sfx, XP / sfp).float()
-    return (r, a), (R, A)
-
-
-def _sky_statistics_from_gpu(st: _GpuSkyStats, c: int, dev_arrays: dict) -> SkyStatistics:
+def _sky_statistics_from_gpu(st: _GpuSkyStats, c: int) -> SkyStatistics:
+    """SkyStatistics of cluster c.  p00_POL/p90_POL come from the kernel
+    (avx_pol_ps).  r00_POL/r90_POL are zeros: the reference computes them from
+    the first avx_pol_ps call's output arrays, which it leaves uninitialised
+    except at the last pixel (packet_ops.py:593-655, sky_statistics.py:246-251),
+    so they carry no defined values to reproduce."""
     sc = st.scal[c].cpu().numpy()
     s = dict(zip(SCALARS, sc.tolist()))
     a = st.cluster_arrays(c)
-    (r, ang), (R, A) = dev_arrays["pol"]
     z = np.zeros_like(a["pixel_mask"])
     f32 = np.float32
     return SkyStatistics(
@@ -218,7 +206,7 @@ def _sky_statistics_from_gpu(st: _GpuSkyStats, c: int, dev_arrays: dict) -> SkyS
         pixel_mask=a["pixel_mask"], gaussian_noise_correction=a["gaussian_noise_correction"],
         noise_amplitude_00=a["noise_amplitude_00"], noise_amplitude_90=a["noise_amplitude_90"],
         coherent_energy=a["coherent_energy"],
-        p00_POL=(r.cpu().numpy(), ang.cpu().numpy()), p90_POL=(R.cpu().numpy(), A.cpu().numpy()),
+        p00_POL=(a["p00_POL_r"], a["p00_POL_a"]), p90_POL=(a["p90_POL_r"], a["p90_POL_a"]),
         r00_POL=(z, z), r90_POL=(z, z), S_snr=a["S_snr"], f=a["f"], F=a["F"])
 
 
@@ -338,11 +326,6 @@ def evaluate_cluster_likelihood(nIFO, cluster, config, MRAcatalog=None, strains=
     batch = _single_batch(n_pix, nIFO, K, rms_d, packed)
     st = _GpuSkyStats(batch, nets, [(lookup, xt)])
     st.run(scan.l_max, scan.REG[1:2], reg0, setup["xgb_rho_mode"])
-    a = st.cluster_arrays(0)
-    dev_arr = {"pol": _pol(torch.from_numpy(a["v00"]).to(dev), torch.from_numpy(a["v90"]).to(dev),
-                           torch.from_numpy(a["pixel_mask"]).to(dev),
-                           torch.from_numpy(a["fp"]).to(dev), torch.from_numpy(a["fx"]).to(dev),
-                           torch.from_numpy(a["f"]).to(dev), torch.from_numpy(a["F"]).to(dev))}
     s64 = maps[6].double()
     e = torch.exp(s64 - s64.max())
     prob = (e / e.sum()).float()
@@ -354,7 +337,7 @@ def evaluate_cluster_likelihood(nIFO, cluster, config, MRAcatalog=None, strains=
         logger.info("Cluster rejected: non-positive sky_stat_max=%.3f", skymap.sky_stat_max)
         return None, None
     skymap.nProbability = prob.cpu().numpy()
-    sky_statistics = _sky_statistics_from_gpu(st, 0, dev_arr)
+    sky_statistics = _sky_statistics_from_gpu(st, 0)
     rejected = get_likelihood_rejection_reason(
         sky_statistics, En, setup["netEC_threshold"],
         net_rho_threshold=setup["net_rho_threshold"], xgb_rho_mode=setup["xgb_rho_mode"])
@@ -487,11 +470,7 @@ def _evaluate_batch(config, setup, items, xtalk, nIFO, dev) -> dict:
         skymap = SkyMapStatistics(l_max, *[m[i] for i in range(len(MAP_NAMES))],
                                   float(res["sky_stat_max"][c]))
         skymap.nProbability = (e / e.sum()).float().cpu().numpy()
-        a = st.cluster_arrays(c)
-        t = lambda x: torch.from_numpy(x).to(dev)
-        dev_arr = {"pol": _pol(t(a["v00"]), t(a["v90"]), t(a["pixel_mask"]), t(a["fp"]),
-                               t(a["fx"]), t(a["f"]), t(a["F"]))}
-        sky_statistics = _sky_statistics_from_gpu(st, c, dev_arr)
+        sky_statistics = _sky_statistics_from_gpu(st, c)
         lat, phi = setup["latitude_arr"], setup["phi_geo_arr"]
         theta = float(np.clip(np.degrees(float(np.pi / 2.0 - lat[l_max])), 0.0, 180.0))
         phi_deg = float(np.degrees(float(phi[l_max]))) % 360.0

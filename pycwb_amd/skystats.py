@@ -15,6 +15,7 @@ for which every cross-talk sum reduces to the per-detector energy p^2 + q^2.
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 import torch
@@ -29,7 +30,8 @@ SCALARS = ("Gn", "Ec", "Dc", "Rc", "Eh", "Es", "Np", "Em", "Lm", "norm", "cc", "
 _S32 = ("v00", "v90", "p", "q", "pd32", "pD32", "ps32", "pS32", "noise_amplitude_00",
         "noise_amplitude_90", "f", "F")
 _S32P = ("et", "pixel_mask", "energy_array_plus", "energy_array_cross", "coherent_energy",
-         "gaussian_noise_correction", "fp", "fx")
+         "gaussian_noise_correction", "fp", "fx", "p00_POL_r", "p00_POL_a", "p90_POL_r",
+         "p90_POL_a")
 _S64 = ("q_norm", "p_norm", "pd", "pD", "ps", "pS")
 
 
@@ -146,3 +148,70 @@ class SkyStatistics:
             o += n * P
         out["S_snr"] = self.S_snr[c, :n].cpu().numpy()
         return out
+
+
+def statistics_at_directions(batch, nets_host, theta, phi, REG1, reg0: float, En,
+                             xgb_rho_mode=False) -> dict:
+    """Best-sky statistics (Lm = Em - Np - Gn, Ec, rho, ...) of every cluster at
+    a continuous direction (theta = colatitude, phi), on the GPU.
+
+    The direction enters as the sky-gradient objective sees it: each detector's
+    delay tau_i = (R_i - R_ref) . n / c * rate between the two neighbouring
+    taps, linearly interpolated (clamped to the table), and F+/Fx at psi = 0
+    with gha = -phi, rounded to float32 like the grid's tables.  The
+    interpolated amplitudes become a one-tap cluster table and each cluster a
+    one-direction network, so the statistics are those of the best-sky kernel
+    (compute_statistics_at_sky_position, sky_statistics.py:26-290) unchanged.
+    ``nets_host[batch.net[c]]`` gives each cluster's detectors and tap rate;
+    ``En`` its network energy threshold per network."""
+    from .batch import ClusterBatch, NetTablesDev
+    from .skygrad import network_geometry
+
+    dev = batch.td.device
+    Cn = batch.n_clusters
+    theta = np.asarray(theta, dtype=np.float64).reshape(Cn)
+    phi = np.asarray(phi, dtype=np.float64).reshape(Cn)
+    geos = {}
+    fps, fxs, parts = [], [], []
+    n_arr = batch.nifo.astype(np.int64)
+    for c in range(Cn):
+        k = int(batch.net[c])
+        if k not in geos:
+            geos[k] = network_geometry(nets_host[k].ifos, nets_host[k].td_rate)
+        n, K = int(n_arr[c]), int(batch.K[c])
+        g = geos[k][:n]
+        st, ct, sp, cp = math.sin(theta[c]), math.cos(theta[c]), math.sin(phi[c]), \
+            math.cos(phi[c])
+        nh = np.array([st * cp, st * sp, ct])
+        x = np.array([sp, -cp, 0.0])
+        y = np.array([-cp * ct, -sp * ct, st])
+        D = g[:, 3:].reshape(n, 3, 3)
+        fps.append((np.einsum("a,iab,b->i", x, D, x) - np.einsum("a,iab,b->i", y, D, y))
+                   .astype(np.float32))
+        fxs.append((np.einsum("a,iab,b->i", x, D, y) + np.einsum("a,iab,b->i", y, D, x))
+                   .astype(np.float32))
+        u = np.clip(g[:, :3] @ nh + K, 0.0, 2 * K - 1e-9)
+        i0 = np.floor(u).astype(np.int64)
+        w = torch.from_numpy(u - i0).to(dev)[None, :, None]
+        tdc = batch.cluster_td(c).to(torch.float64)               # (P, n, 2K+1, 2)
+        idx = torch.from_numpy(i0).to(dev)
+        a = tdc[:, torch.arange(n, device=dev), idx]              # (P, n, 2)
+        b = tdc[:, torch.arange(n, device=dev), idx + 1]
+        parts.append(((1.0 - w) * a + w * b).to(torch.float32).reshape(-1))
+    P = batch.n_pix.astype(np.int64)
+    td_off = np.concatenate([[0], np.cumsum(P * n_arr)[:-1]]).astype(np.int64)
+    one = ClusterBatch(n_pix=batch.n_pix, net=np.arange(Cn, dtype=np.int32),
+                       K=np.zeros(Cn, np.int32), nifo=batch.nifo, rms_off=batch.rms_off,
+                       td_off=td_off, rms=batch.rms, td=torch.cat(parts))
+    En_c = np.asarray([En[int(batch.net[c])] for c in range(Cn)], dtype=np.float64)
+    t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=dt)
+    nets = NetTablesDev(
+        n_sky=1, nifo=n_arr.tolist(), En=En_c.tolist(),
+        off=t(np.concatenate([[0], np.cumsum(n_arr)[:-1]]), torch.int64),
+        FP=t(np.concatenate(fps), torch.float32), FX=t(np.concatenate(fxs), torch.float32),
+        ml=torch.zeros(int(n_arr.sum()), dtype=torch.int32, device=dev),
+        nifo_d=t(n_arr, torch.int32), En_d=t(En_c, torch.float64))
+    st_ = SkyStatistics(one, nets)
+    st_.run(torch.zeros(Cn, dtype=torch.int64, device=dev),
+            torch.as_tensor(np.asarray(REG1, np.float32), device=dev), reg0, xgb_rho_mode)
+    return st_.scalars()

@@ -199,6 +199,18 @@ def sky_statistics_self_xtalk(FP_l, FX_l, rms, v00, v90, En, limit="span"):
     norm = max((Eo - Eh) / Em if Em > 0 else 1e9, 1.0)
     Ec_n, Dc_n = Ec / norm, Dc / norm
     rho = np.sqrt(Ec_n * Rc / 2) if Ec_n > 0 else 0.0
-    return {"Gn": Gn, "Ec": Ec_n, "Dc": Dc_n, "Rc": Rc, "Eh": Eh, "Es": Es, "Np": Np,
+    # avx_pol_ps: radius = norm of the data's projection onto span(f, F); the
+    # angle's cosine = its component along the dominant direction over the radius
+    A, fp, fx, f, F, ni = dpf_frame(FP_l, FX_l, rms)
+    fhat = f / np.sqrt(fp)[:, None]
+    pol = {}
+    for name, v in (("p00", v00), ("p90", v90)):
+        vm = v * m
+        Q = np.linalg.qr(A)[0]                                  # (P, n, 2) orthonormal span
+        coef = np.einsum("pjk,jp->pk", Q, vm)
+        rad = np.sqrt((coef ** 2).sum(1))
+        pol[name + "_POL_r"] = rad
+        pol[name + "_POL_abscos"] = np.abs((vm.T * fhat).sum(1)) / np.where(rad > 0, rad, 1)
+    return {**pol, "Gn": Gn, "Ec": Ec_n, "Dc": Dc_n, "Rc": Rc, "Eh": Eh, "Es": Es, "Np": Np,
             "Em": Em, "Lm": Lm, "norm": norm, "rho": rho, "Lo": Lo, "Eo": Eo,
             "N_pix_effective": Np_eff, "pd": pd, "pD": pD, "ps": ps, "pS": pS}

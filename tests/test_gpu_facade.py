@@ -123,11 +123,12 @@ def test_nrms_maps_set_the_noise_weights(oracle):
     P = 90
     rng = np.random.default_rng(77)
     layers = np.full(P, 8, np.int32)
-    freq = rng.integers(0, 9, P).astype(np.int32)
+    freq = rng.integers(0, 8, P).astype(np.int32)
     tb = rng.integers(0, 40, P)
     times = (tb * 8 + freq).astype(np.int32)
     nRMS = [SimpleNamespace(data=rng.uniform(0.4, 2.5, size=(9, 40))) for _ in range(3)]
-    sigma = np.stack([np.abs(m.data[np.round(freq * 9 / 8).astype(int), tb]) for m in nRMS])
+    row = np.clip(np.round(freq * 9 / 8).astype(int), 0, 8)
+    sigma = np.stack([np.abs(m.data[row, tb]) for m in nRMS])
     sigma = sigma.astype(np.float32)
 
     def make():

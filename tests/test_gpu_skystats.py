@@ -81,3 +81,9 @@ def test_sky_statistics_match_oracle(oracle, use_catalog):
         np.testing.assert_allclose(g["gaussian_noise_correction"], o["gn"], rtol=1e-6,
                                    atol=1e-7)
         np.testing.assert_allclose(g["S_snr"], o["S_snr"], rtol=1e-9)
+        # avx_pol_ps: radii bit-exact (float64 NumPy order, correctly rounded
+        # sqrt), angles within one float32 ulp (the device and host atan2)
+        for k in ("p00_POL_r", "p90_POL_r"):
+            assert np.array_equal(g[k].view(np.uint32), o[k].view(np.uint32)), (c, k)
+        for k in ("p00_POL_a", "p90_POL_a"):
+            np.testing.assert_array_max_ulp(g[k], o[k], maxulp=1)

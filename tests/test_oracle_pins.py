@@ -22,7 +22,9 @@ regulator where the reference's formulas reduce to closed forms:
 * compute_statistics_at_sky_position with self-only cross-talk: packets as
   principal-axis coordinates normalised by the Gram eigenvalues (a, A), packet
   energies sum (a + A)^2 / 4, amplitudes at the mean (a + A)/2, N_eff = 2M - 1,
-  Lm = Em - Np - Gn.
+  Lm = Em - Np - Gn; the polar coordinates of avx_pol_ps: radius = norm of the
+  data's projection onto span(r F+, r Fx), |cos angle| = its dominant
+  component over the radius.
 
 ``oracle_mutant`` plants one transcription slip at a time in the oracle (a
 flipped sign, a dropped term or factor, a wrong index or operand in a4, a6,
@@ -152,9 +154,14 @@ def _stats_errors(oracle, net, cl, limit="span", En=12.0, n_dirs=6):
         for k in STAT_KEYS:
             s = scale if k not in ("Rc", "norm", "rho", "N_pix_effective") else 1.0
             worst = max(worst, abs(st[k] - d[k]) / (TOL * (abs(d[k]) + s)))
-        for k in ("pd", "pD", "ps", "pS"):
+        for k in ("pd", "pD", "ps", "pS", "p00_POL_r", "p90_POL_r"):
             amp = np.sqrt(scale)
             worst = max(worst, float(np.max(np.abs(st[k] - d[k]))) / (TOL * amp))
+        for ph in ("p00", "p90"):
+            live = d[ph + "_POL_r"] > 1e-3 * np.sqrt(scale)
+            cos = np.abs(np.cos(st[ph + "_POL_a"].astype(np.float64)))
+            worst = max(worst, float(np.max(np.abs(cos - d[ph + "_POL_abscos"])[live],
+                                            initial=0.0)) / (10 * TOL))
     return worst, checked
 
 
