@@ -196,6 +196,7 @@ def refinement_leg(n_clusters, order, K, steps, stream, log):
     td_ms = k[1].elapsed_time(k[2]) / reps
     v0, v = out["value0"].cpu().numpy(), out["value"].cpu().numpy()
     grid = r["sky_stat_max"]
+    lm = lm_criterion(b, nt, [net], bs, r, th0, ph0, out)
     th_t, ph_t = b.truth[:, 0], b.truth[:, 1]
     err0 = angular_distance(th0.cpu().numpy(), ph0.cpu().numpy(), th_t, ph_t)
     err1 = angular_distance(out["theta"].cpu().numpy(), out["phi"].cpu().numpy(), th_t, ph_t)
@@ -209,11 +210,41 @@ def refinement_leg(n_clusters, order, K, steps, stream, log):
            "median_error_deg_grid": float(np.degrees(np.median(err0))),
            "median_error_deg_refined": float(np.degrees(np.median(err1))),
            "frac_closer_to_truth": float(np.mean(err1 < err0)),
+           "Lm": lm,
            "data": "tap-correlated noise (sigma 2 taps), off-grid injections, SNR 6",
            "dtype": "f64 (angles), f32 output (amplitudes)"}
     log(f"refinement: {res}")
     bs.close()
     return res
+
+
+def lm_criterion(batch, nt, nets_host, bs, r, th0, ph0, out):
+    """The refinement's success criterion on Lm = Em - Np - Gn of the best-sky
+    statistics: at the refined direction (continuous delays, interpolated
+    amplitudes) against the grid argmax, both as the pipeline evaluates it
+    (integer delay taps) and with the same interpolation at the grid
+    direction."""
+    import torch
+
+    from pycwb_amd.skystats import SkyStatistics, statistics_at_directions
+
+    st = SkyStatistics(batch, nt)
+    st.run(bs.l_max, bs.REG1, bs.desc.reg0)
+    lm_grid = st.scalars()["Lm"]
+    reg1 = r["REG1"]
+    lm_ref = statistics_at_directions(batch, nets_host, out["theta"].cpu().numpy(),
+                                      out["phi"].cpu().numpy(), reg1, bs.desc.reg0,
+                                      nt.En)["Lm"]
+    lm_gi = statistics_at_directions(batch, nets_host, th0.cpu().numpy(), ph0.cpu().numpy(),
+                                     reg1, bs.desc.reg0, nt.En)["Lm"]
+    torch.cuda.synchronize()
+    return {"frac_refined_above_grid_Lm": float(np.mean(lm_ref > lm_grid)),
+            "frac_refined_above_grid_interpolated_Lm": float(np.mean(lm_ref > lm_gi)),
+            "median_Lm_grid": float(np.median(lm_grid)),
+            "median_Lm_refined": float(np.median(lm_ref)),
+            "note": "Lm = Em - Np - Gn of the best-sky statistics; grid = integer taps at "
+                    "the grid argmax; interpolated = the refinement's linear delay "
+                    "interpolation at the grid direction"}
 
 
 def wdm_leg(stream, log, seconds=600, rate=4096, n_det=5):
@@ -701,168 +732,44 @@ def threshold_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, cpu=True, 
     return res
 
 
-def _inject_sine_gaussians(x, ifos, rate, n_inj, seed):
-    """Linearly polarised sine-Gaussians (Q = 9, f0 uniform in 50-1000 Hz,
-    network SNR uniform in 8-40, uniform-sphere directions, times uniform in
-    the series' middle) added to the detectors' white unit-variance series x
-    (n, L), in place.  Returns (times, f0s, directions (phi, lat))."""
-    from pycwb_amd import sky
-
-    rng = np.random.default_rng(seed)
-    dets = [sky.Detector(n) for n in ifos]
-    L = x.shape[1]
-    times = np.sort(rng.uniform(20.0, L / rate - 20.0, n_inj))
-    f0s = rng.uniform(50.0, 1000.0, n_inj)
-    snrs = rng.uniform(8.0, 40.0, n_inj)
-    phis = rng.uniform(0.0, 2 * np.pi, n_inj)
-    lats = np.arcsin(rng.uniform(-1.0, 1.0, n_inj))
-    for k in range(n_inj):
-        tau = 9.0 / (np.sqrt(2.0) * np.pi * f0s[k])
-        nh = np.array([np.cos(lats[k]) * np.cos(phis[k]), np.cos(lats[k]) * np.sin(phis[k]),
-                       np.sin(lats[k])])
-        i0 = int((times[k] - 8 * tau) * rate)
-        i1 = int((times[k] + 8 * tau) * rate) + 1
-        t = np.arange(i0, i1) / rate
-        hs = []
-        for d in dets:
-            shift = float(nh @ (d.vertex - dets[0].vertex) / sky.C_LIGHT)
-            fp = float(d.antenna_pattern_gha(np.array([-phis[k]]), np.array([lats[k]]))[0][0])
-            tt = t + shift - times[k]
-            hs.append(fp * np.exp(-(tt / tau) ** 2) * np.sin(2 * np.pi * f0s[k] * tt))
-        scale = snrs[k] / np.sqrt(sum(float(h @ h) for h in hs))
-        for i, h in enumerate(hs):
-            x[i, i0:i1] += scale * h
-    return times, f0s, np.stack([phis, lats], axis=1)
-
-
-def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, td_size=12,
-            bpp=0.001, seg_edge=2.0, max_delay_s=0.04, pattern=3, cpu=True):
-    """The end-to-end case: 5 detectors (H1 L1 V1 K1 I1), 600 s of white
-    (whitened) noise at 4096 Hz with 200 sine-Gaussian injections, 7 WDM
-    levels (M = 8 .. 512).  Per level, in the order of coherence_native/
-    setup.py and pipeline.py: t2w of the 5 series, time-delay max-energy maps
-    (pattern 3, 40 ms, every 4th sample) and their Gamma-to-Gauss,
-    compute_threshold (bpp 0.001, shape path), zero-lag pixel selection and
-    clustering (kt = 2, kf = 3), the maps staying on the device; then TD
-    amplitudes of every cluster of every
-    level straight into one likelihood batch (K = 598 at 16384 Hz) and the
-    batched regulator + scan + argmax at HEALPix order 8 (786,432 directions).
-    Wall time per stage (device synchronised at each stage's end); the
-    super-clustering across levels is not part of this build, so every level's
-    clusters go to the likelihood.  Found: injections with a cluster within
-    0.1 s whose argmax direction reproduces the injected detector delays to
-    1 ms.  Parity sample (with ``cpu``): two clusters' regulator and l_max
-    against the oracle (16 threads) on the batch's own amplitudes."""
+def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, cpu=True,
+            refine_steps=20, cpu_rates=None):
+    """The end-to-end case: 5 detectors (H1 L1 V1 K1 I1), 600 s of coloured
+    Gaussian noise at 4096 Hz whitened to unit variance, with 200 injections
+    (sine-Gaussians and white-noise bursts, f0 50-1000 Hz, network SNR 8-40),
+    7 WDM levels (M = 8 .. 512), then one likelihood batch at HEALPix order 8
+    (786,432 directions) and the gradient refinement of every cluster
+    (pycwb_amd.segment).  Reports the wall time per stage, the roofline
+    fraction of the stages with an algorithmic count (t2w: HBM bytes; TD
+    amplitudes: float64 flops; likelihood: the scan's required flops), the
+    likelihood batch against the plain one-cluster-per-launch GPU path on the
+    same clusters and against the host CPU path (the oracle, 16 threads, on two
+    clusters of the batch; ``cpu_rates`` carries the other stages' measured
+    single-thread oracle rates), and the injections recovered."""
     import torch
 
-    from pycwb_amd import sky, synth
-    from pycwb_amd.batch import BatchedSkyScan, ClusterBatch, net_tables
-    from pycwb_amd.coherence_native.clustering import cluster_pixels_device
-    from pycwb_amd.coherence_native.projection import gamma_to_gauss_device, max_energy_device
-    from pycwb_amd.coherence_native.selection import (select_network_pixels_device,
-                                                      selection_cache_device)
-    from pycwb_amd.coherence_native.veto_threshold import compute_threshold_device
-    from pycwb_amd.tdamp import TDBatchInputs
-    from pycwb_amd.wdm import WDM
+    from pycwb_amd import segment, synth
+    from pycwb_amd.batch import PlainLoop
+    from pycwb_amd.skygrad import SkyGradient, refine_sky
 
     dev = torch.device("cuda", 0)
     ifos = synth.NETWORK_5
     n = len(ifos)
-    L = seconds * rate
     t0 = time.perf_counter()
-    x = np.random.default_rng(11).normal(size=(n, L))
-    inj_t, inj_f, inj_dir = _inject_sine_gaussians(x, ifos, rate, n_inj, 12)
+    x, inj = segment.synthetic_segment(ifos, seconds, rate, n_inj, seed=11)
     xd = torch.from_numpy(x).to(dev)
     torch.cuda.synchronize()
-    stages = {"data": time.perf_counter() - t0}
-    td_rate = rate * up_tdf
-    K = int(math.ceil(sky.max_network_delay(ifos, ifos[0]) * td_rate)) + 1
-    net = synth.make_network(ifos, order=order, K=K, td_rate=td_rate)
-    nt = net_tables([net], device=dev)
-    add = lambda k, dt: stages.__setitem__(k, stages.get(k, 0.0) + dt)
-    levels = []
-    for M in (8, 16, 32, 64, 128, 256, 512):
-        t = time.perf_counter()
-        w = WDM(M, M, 6, 10)                 # filters of the level: per-job setup
-        w.set_td_filter(td_size, up_tdf)
-        add("setup", time.perf_counter() - t)
-        t = time.perf_counter()
-        tf = w.t2w_device(xd)                                       # (n, M+1, N)
-        torch.cuda.synchronize()
-        add("t2w", time.perf_counter() - t)
-        dt_bin, df = M / rate, rate / 2.0 / M
-        t = time.perf_counter()
-        energy = [max_energy_device(xd[i], w, int(max_delay_s * rate), 4, pattern, seg_edge,
-                                    int(round(1 / dt_bin)), 32.0, 2000.0, df)
-                  for i in range(n)]
-        torch.cuda.synchronize()
-        add("max_energy", time.perf_counter() - t)
-        t = time.perf_counter()
-        maps, alp = [], 0.0
-        for e in energy:
-            g, a = gamma_to_gauss_device(e, seg_edge, int(round(1 / dt_bin)), stream=stream)
-            maps.append(g)
-            alp += a
-        stack = torch.stack([g.t() for g in maps]).contiguous()      # (n, M+1, N)
-        torch.cuda.synchronize()
-        add("gamma_to_gauss", time.perf_counter() - t)
-        t = time.perf_counter()
-        Eo = compute_threshold_device(stack, bpp, alp=alp / n, edge=seg_edge,
-                                      wavelet_rate=int(round(1 / dt_bin)))
-        torch.cuda.synchronize()
-        add("threshold", time.perf_counter() - t)
-        t = time.perf_counter()
-        cache = selection_cache_device(stack, dt_bin, df, edge=seg_edge, f_low=32.0,
-                                       f_high=2000.0, stop=float(seconds))
-        sel = select_network_pixels_device(cache, Eo, np.zeros((1, n), np.int64), stream=stream)
-        torch.cuda.synchronize()
-        add("selection", time.perf_counter() - t)
-        t = time.perf_counter()
-        cl = cluster_pixels_device(sel, kt=2, kf=3, rate=cache["rate"], stream=stream)
-        nc = cl["n_clusters"]
-        np_ = int(sel["n_pix"])
-        off = cl["cluster_off"][:nc + 1].cpu().numpy()
-        mem = cl["members"][:np_]
-        pix = (sel["time"][mem] * (M + 1) + sel["frequency"][mem]).to(torch.int32)
-        ctime = cl["c_time"][:nc].cpu().numpy()
-        torch.cuda.synchronize()
-        add("clustering", time.perf_counter() - t)
-        levels.append({"M": M, "w": w, "tf": tf, "pix": pix, "sizes": np.diff(off), "ctime": ctime,
-                       "Eo": Eo, "selected": np_})
-        log(f"e2e level M={M}: Eo={Eo:.3f}, {np_} pixels selected, {nc} clusters")
-    # one likelihood batch: every cluster of every level
-    t = time.perf_counter()
-    sizes = np.concatenate([lv["sizes"] for lv in levels]).astype(np.int64)
+    data_s = time.perf_counter() - t0
+    seg = segment.run(xd, ifos, rate, stream, order=order, log=log)
+    stages = dict(seg["stage_s"])
+    stages["data"] = data_s
+    net, nt, batch, scan, res = seg["net"], seg["nets"], seg["batch"], seg["scan"], \
+        seg["results"]
+    sizes, K, td_rate = seg["sizes"], seg["K"], seg["td_rate"]
     C_ = sizes.size
-    nd = 2 * K + 1
-    rms_off = np.concatenate([[0], np.cumsum(sizes * n)[:-1]]).astype(np.int64)
-    td_off = np.concatenate([[0], np.cumsum(sizes * n * nd)[:-1]]).astype(np.int64)
-    td = torch.empty(2 * int(sizes.sum()) * n * nd, dtype=torch.float32, device=dev)
-    rms = torch.full((int(sizes.sum()) * n,), 1.0 / math.sqrt(n), dtype=torch.float32,
-                     device=dev)
-    c0 = 0
-    for lv in levels:
-        bank = lv["w"].td_filters
-        if lv["sizes"].size:
-            for i in range(n):
-                inp = TDBatchInputs.from_device(lv["tf"][i], bank)
-                inp.extract_td_into(lv["pix"], K, td, int(td_off[c0]), i, n, stream=stream)
-        c0 += lv["sizes"].size
-        lv["tf"] = None
-    batch = ClusterBatch(n_pix=sizes.astype(np.int32), net=np.zeros(C_, np.int32),
-                         K=np.full(C_, K, np.int32), nifo=np.full(C_, n, np.int32),
-                         rms_off=rms_off, td_off=td_off, rms=rms, td=td)
-    torch.cuda.synchronize()
-    add("td_amplitudes", time.perf_counter() - t)
-    valid = torch.arange(net.n_sky, dtype=torch.int64, device=dev)
-    scan = BatchedSkyScan(batch, nt, valid, 0.5 * 0.5 * 2 / 3, 0.5, 0.5, block=64)
-    scan.launch(stream)                                            # warm (graph build)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    scan.launch(stream)
-    torch.cuda.synchronize()
-    add("likelihood_batch", time.perf_counter() - t)
-    res = scan.results()
+    S = net.n_sky
+    evals = int(sizes.sum()) * S
+    # per-stage kernel times of the likelihood batch
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     scan.launch_stage(0, stream)
     ev[0].record(stream)
@@ -872,62 +779,110 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, up_tdf=4, t
     ev[2].record(stream)
     scan.launch_stage(3, stream)
     torch.cuda.synchronize()
-    stage_ms = {"regulator": ev[0].elapsed_time(ev[1]), "scan": ev[1].elapsed_time(ev[2])}
+    reg_ms, scan_ms = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
     active = scan.active_pairs()
-    active_fraction = float(active.sum()) / float(batch.evals(net.n_sky))
-    parity = None
+    active_fraction = float(active.sum()) / float(evals)
+    # the plain GPU path on the same clusters
+    pl = PlainLoop(batch, nt, seg["valid"], 0.5 * 0.5 * 2 / 3, 0.5, 0.5)
+    pl.launch(stream)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    pl.launch(stream)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - tp
+    rp = pl.results()
+    plain_same = bool(np.array_equal(rp["l_max"], res["l_max"]))
+    # gradient refinement of every cluster from its grid argmax
+    eng = SkyGradient(batch, nt, [net], res["REG1"], scan.desc.reg0)
+    th0 = torch.from_numpy(np.pi / 2 - net.lat[res["l_max"]]).to(dev)
+    ph0 = torch.from_numpy(net.phi[res["l_max"]]).to(dev)
+    torch.cuda.synchronize()
+    tr = time.perf_counter()
+    ref = refine_sky(eng, th0, ph0, steps=refine_steps)
+    torch.cuda.synchronize()
+    stages["refinement"] = time.perf_counter() - tr
+    v0, v1 = ref["value0"].cpu().numpy(), ref["value"].cpu().numpy()
+    lm = lm_criterion(batch, nt, [net], scan, res, th0, ph0, ref)
+    # parity sample and the CPU path on it
+    parity, cpu_rate = None, None
     if cpu:
         from oracle import oracle as ora
 
         picks = [int(np.argmin(np.abs(sizes - 10))), int(np.argmin(np.abs(sizes - 40)))]
-        vh = np.arange(net.n_sky, dtype=np.int64)
+        vh = np.arange(S, dtype=np.int64)
         En = nt.En[0]
-        parity = []
+        parity, cpu_evals, cpu_s = [], 0, 0.0
         for c in picks:
             P = int(sizes[c])
             rms_c = batch.cluster_rms(c).cpu().numpy()
             td_c = batch.cluster_td(c).cpu().numpy()                  # (P, n, 2K+1, 2)
             t00 = np.ascontiguousarray(td_c[..., 0].transpose(2, 1, 0))
             t90 = np.ascontiguousarray(td_c[..., 1].transpose(2, 1, 0))
-            reg_o = ora.calculate_dpf(net.FP_t, net.FX_t, rms_c, net.n_sky, n,
-                                      0.5 * 0.5 * 2 / 3, En, vh, threads=16)
+            tc = time.perf_counter()
+            reg_o = ora.calculate_dpf(net.FP_t, net.FX_t, rms_c, S, n, 0.5 * 0.5 * 2 / 3, En,
+                                      vh, threads=16)
             REG = np.array([np.float32(0.5 * np.sqrt(2)), res["REG1"][c], 0.0], np.float32)
-            l_o, _, _ = ora.scan_sky(n, P, net.n_sky, net.FP_t, net.FX_t, rms_c, t00, t90,
-                                     net.ml, REG, 0.5, 0.5, En, vh, threads=16)
+            l_o, _, _ = ora.scan_sky(n, P, S, net.FP_t, net.FX_t, rms_c, t00, t90, net.ml, REG,
+                                     0.5, 0.5, En, vh, threads=16)
+            cpu_s += time.perf_counter() - tc
+            cpu_evals += P * S
             parity.append({"cluster": c, "pixels": P,
                            "regulator_bit_exact": bool(reg_o == res["regulator"][c]),
                            "l_max_bit_exact": bool(int(l_o) == int(res["l_max"][c]))})
+        cpu_rate = cpu_evals / cpu_s
         log(f"e2e parity sample: {parity}")
     scan.close()
-    # recovery: injections with a cluster within 0.1 s whose argmax delays match
-    ctimes = np.concatenate([lv["ctime"] for lv in levels])
-    ml = net.ml
-    found = 0
-    for k in range(n_inj):
-        near = np.nonzero(np.abs(ctimes - inj_t[k]) < 0.1)[0]
-        if near.size == 0:
-            continue
-        phi, lat = inj_dir[k]
-        nh = np.array([np.cos(lat) * np.cos(phi), np.cos(lat) * np.sin(phi), np.sin(lat)])
-        ref = sky.Detector(ifos[0]).vertex
-        dl = np.array([nh @ (sky.Detector(d).vertex - ref) / sky.C_LIGHT for d in ifos])
-        rec = (ml[:, res["l_max"][near]] - ml[0, res["l_max"][near]]) / td_rate
-        if np.any(np.all(np.abs(rec - dl[:, None]) < 1e-3, axis=0)):
-            found += 1
+    found = segment.recovered(inj, seg["ctime"], res["l_max"], net, ifos, td_rate)
+    found_sg = segment.recovered([d for d in inj if d["kind"] == "sg"], seg["ctime"],
+                                 res["l_max"], net, ifos, td_rate)
+    # roofline of the stages with an algorithmic count
+    levels = seg["levels"]
+    L = x.shape[1]
+    t2w_bytes = float(sum(n * (8 * L + 16 * (lv["M"] + 1) * lv["n_time"]) for lv in levels))
+    td_flops = float(sizes.sum()) * n * (2 * K + 1) * 300.0
+    nd_frac = float(active.sum()) / float(evals)
+    req = float(np.sum(sizes.astype(np.float64) * S * (4.0 * n + 4.0))) + \
+        float(active.sum()) * (74.0 * n + 125.0)
+    roof = {
+        "t2w": {"bound": "hbm", "achieved_gbs": t2w_bytes / stages["t2w"] / 1e9,
+                "frac": t2w_bytes / stages["t2w"] / 1e9 / PEAK_HBM_GBS,
+                "note": "wall time including launches"},
+        "td_amplitudes": {"bound": "fp64", "achieved_tflops": td_flops / stages["td_amplitudes"]
+                          / 1e12, "frac": td_flops / stages["td_amplitudes"] / 1e12 / 78.6},
+        "likelihood_scan": {"bound": "valu", "achieved_tflops": req / (scan_ms * 1e-3) / 1e12,
+                            "frac": req / (scan_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS},
+    }
+    cpu_path = None
+    if cpu_rate:
+        cpu_path = {"likelihood_evals_per_s": cpu_rate, "threads": 16,
+                    "likelihood_batch_s_estimated": evals / cpu_rate,
+                    "gpu_over_cpu": (evals / stages["likelihood_batch"]) / cpu_rate}
+        if cpu_rates:
+            cpu_path["other_stage_rates_1_thread"] = cpu_rates
     total = sum(v for k, v in stages.items() if k not in ("data", "setup"))
-    evals = int(sizes.sum()) * net.n_sky
     out = {"detectors": n, "seconds": seconds, "rate": rate, "injections": n_inj,
+           "injection_kinds": {"sg": sum(d["kind"] == "sg" for d in inj),
+                               "wnb": sum(d["kind"] == "wnb" for d in inj)},
+           "noise": "coloured (PSD (40/f)^4 + 1 + (f/300)^2), whitened to unit variance",
            "levels": [{"M": lv["M"], "Eo": round(lv["Eo"], 4), "selected": lv["selected"],
                        "clusters": int(lv["sizes"].size)} for lv in levels],
            "clusters": C_, "pixels": int(sizes.sum()), "healpix_order": order, "K": K,
            "likelihood_evals": evals,
            "likelihood_evals_per_s": evals / stages["likelihood_batch"],
+           "plain_gpu": {"s": round(plain_s, 4), "evals_per_s": evals / plain_s,
+                         "batched_over_plain": plain_s / stages["likelihood_batch"],
+                         "l_max_equal": plain_same},
+           "cpu_path": cpu_path,
            "stage_s": {k: round(v, 4) for k, v in stages.items()},
-           "likelihood_stage_ms": {k: round(v, 2) for k, v in stage_ms.items()},
-           "likelihood_active_fraction": round(active_fraction, 4),
+           "likelihood_stage_ms": {"regulator": round(reg_ms, 2), "scan": round(scan_ms, 2)},
+           "likelihood_active_fraction": round(nd_frac, 4),
+           "roofline": roof,
+           "refinement": {"steps": refine_steps, "frac_improved": float(np.mean(v1 > v0)),
+                          "Lm": lm},
            "parity_sample": parity,
            "wall_s_without_data_and_setup": round(total, 3),
-           "injections_found_with_delays": found}
+           "injections_found_with_delays": found,
+           "sine_gaussians_found_with_delays": found_sg}
     log(f"e2e: {C_} clusters ({int(sizes.sum())} px) from 7 levels in {total:.2f} s "
         f"(stages {out['stage_s']}); {found}/{n_inj} injections recovered with their delays")
     return out
@@ -1196,7 +1151,10 @@ def main():
     maxen = None if args.no_maxenergy else maxenergy_leg(stream, log, cpu=not args.no_cpu)
     clus = None if args.no_cluster else cluster_leg(stream, log, cpu=not args.no_cpu)
     thr = None if args.no_threshold else threshold_leg(stream, log, cpu=not args.no_cpu)
-    e2e = None if args.no_e2e else e2e_leg(stream, log, cpu=not args.no_cpu)
+    rates = {k: v["cpu_baseline"] for k, v in (("selection", select), ("max_energy", maxen),
+                                                 ("clustering", clus), ("threshold", thr))
+             if v and v.get("cpu_baseline")}
+    e2e = None if args.no_e2e else e2e_leg(stream, log, cpu=not args.no_cpu, cpu_rates=rates)
 
     traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
     out = {
