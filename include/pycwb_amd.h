@@ -174,6 +174,22 @@ int pycwb_batch_launch(pycwb_batch* batch, void* stream);
 int pycwb_batch_launch_stage(pycwb_batch* batch, int stage, void* stream);
 int pycwb_batch_destroy(pycwb_batch* batch);
 
+/* The batched regulator + scan + argmax of pycwb_batch_create/launch as one
+ * call that allocates nothing: no hipMalloc, no hipGraph, no host heap.  No
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * stream capture, serving loops, harnesses that forbid allocation).  The
+ * kernels and results are those of pycwb_batch_launch on the same description;
+ * the description's tickets, counts, best and weight_stats are ignored (may
+ * be NULL): the work queues, regulator counts, argmax keys and noise-weight
+ * statistics live in a device array of the library (about 2.3 MB per GPU,
+ * held until the process ends), cleared by a fill kernel at the start of
+ * every call.  At most pycwb_batch_noalloc_max_clusters() (16,384) clusters;
+ * more return PYCWB_ERR_ARG with the reason in pycwb_last_error().  The
+ * kernels are enqueued on `stream` directly (use_graph is ignored).  Calls
+ * that overlap on one GPU share that array and race: serialise them. */
+int pycwb_batch_run_noalloc(const pycwb_batch_desc* desc, void* stream);
+size_t pycwb_batch_noalloc_max_clusters(void);
+
 /* Best-sky statistics of a batch of clusters at their selected directions,
  * one workgroup per cluster.  Replaces compute_statistics_at_sky_position
  * (pycwb/modules/likelihoodWP/sky_statistics.py:26-290; called at

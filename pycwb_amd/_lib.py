@@ -170,6 +170,8 @@ SIGNATURES = {
     "pycwb_batch_launch": (i32, [vp, vp]),
     "pycwb_batch_launch_stage": (i32, [vp, i32, vp]),
     "pycwb_batch_destroy": (i32, [vp]),
+    "pycwb_batch_run_noalloc": (i32, [C.POINTER(BatchDesc), vp]),
+    "pycwb_batch_noalloc_max_clusters": (sz, []),
     "pycwb_sky_stats_scratch32": (sz, [i32, i32]),
     "pycwb_sky_stats_scratch64": (sz, [i32, i32]),
     "pycwb_sky_statistics": (i32, [C.POINTER(StatsDesc), vp]),
@@ -227,7 +229,10 @@ def check(rc: int, what: str) -> None:
     if rc == PYCWB_OK:
         return
     if rc == PYCWB_ERR_ARG:
-        raise ValueError(f"{what}: arguments rejected by the C ABI")
+        reason = _lib.pycwb_last_error().decode(errors="replace") if _lib is not None else ""
+        reason = reason[len(what) + 2:] if reason.startswith(what + ": ") else ""
+        raise ValueError(f"{what}: arguments rejected by the C ABI" +
+                         (f" ({reason})" if reason else ""))
     detail = ""
     if _lib is not None:
         detail = _lib.pycwb_last_error().decode(errors="replace")

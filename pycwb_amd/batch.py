@@ -352,6 +352,12 @@ class BatchedSkyScan:
     def launch(self, stream=None) -> None:
         _lib.check(lib().pycwb_batch_launch(self.handle, stream_ptr(stream)), "pycwb_batch_launch")
 
+    def launch_noalloc(self, stream=None) -> None:
+        """The same kernels through pycwb_batch_run_noalloc (library-held
+        scratch, no allocation, no graph)."""
+        _lib.check(lib().pycwb_batch_run_noalloc(C.byref(self.desc), stream_ptr(stream)),
+                   "pycwb_batch_run_noalloc")
+
     def launch_stage(self, stage: int, stream=None) -> None:
         _lib.check(lib().pycwb_batch_launch_stage(self.handle, int(stage), stream_ptr(stream)),
                    "pycwb_batch_launch_stage")

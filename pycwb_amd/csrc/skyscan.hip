@@ -18,6 +18,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "skyscan_lane.hh"
 #include "pycwb_amd.h"
 
@@ -1242,9 +1244,15 @@ static int batch_enqueue(const pycwb_batch* b, hipStream_t st) {
   return PYCWB_OK;
 }
 
-int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
-  if (!d || !out) return PYCWB_ERR_ARG;
-  *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Checks a batch description and fills `b` from it (graph left empty).  With
+// own_scratch the description's tickets/counts/best/weight_stats must be set;
+// without, the caller supplies them.
+int batch_from_desc(const pycwb_batch_desc* d, bool own_scratch, pycwb_batch* b) {
+  if (!d) return PYCWB_ERR_ARG;
   if (d->n_clusters < 1 || d->n_sky < 1 || d->n_valid < 1 || d->n_valid > d->n_sky ||
       d->n_sky >= (int64_t)0x7fffffff)
     return PYCWB_ERR_ARG;
@@ -1262,10 +1270,9 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
   if (covered != d->n_clusters) return PYCWB_ERR_ARG;
   if (!d->sky_valid_indices || !d->order || !d->n_pix || !d->net || !d->K || !d->rms_off ||
       !d->td_off || !d->rms || !d->td || !d->net_nifo || !d->net_off || !d->net_En || !d->FP ||
-      !d->FX || !d->ml || !d->tickets || !d->counts || !d->best || !d->REG1 || !d->l_max ||
-      !d->sky_stat_max)
+      !d->FX || !d->ml || !d->REG1 || !d->l_max || !d->sky_stat_max)
     return PYCWB_ERR_ARG;
-  pycwb_batch* b = new pycwb_batch();
+  if (own_scratch && (!d->tickets || !d->counts || !d->best)) return PYCWB_ERR_ARG;
   BatchDev& v = b->dev;
   v.n_clusters = d->n_clusters;
   v.tiles_per_cluster = (int)tiles;
@@ -1317,6 +1324,70 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
   }
   b->graph = nullptr;
   b->exec = nullptr;
+  return PYCWB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of pycwb_batch_run_noalloc: one array per GPU, held until the
+// process ends.  Layout: tickets (kTicketsPerGroup x 8 groups uint32, padded
+// to 1 KiB), counts (uint32), best (uint64), weight statistics (kWStats
+// doubles), each for kNoAllocMaxClusters clusters.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxClusters = 16384;
+constexpr size_t kNoAllocTicketBytes = 1024;
+constexpr size_t kNoAllocCountsOff = kNoAllocTicketBytes;
+constexpr size_t kNoAllocBestOff = kNoAllocCountsOff + sizeof(unsigned) * kNoAllocMaxClusters;
+constexpr size_t kNoAllocWStatsOff =
+    kNoAllocBestOff + sizeof(unsigned long long) * kNoAllocMaxClusters;
+constexpr size_t kNoAllocScratchBytes =
+    kNoAllocWStatsOff + sizeof(double) * kWStats * kNoAllocMaxClusters;
+static_assert(sizeof(unsigned) * kTicketsPerGroup * kMaxIfo <= kNoAllocTicketBytes,
+              "ticket area of the no-alloc scratch");
+static_assert(kNoAllocBestOff % 8 == 0 && kNoAllocWStatsOff % 8 == 0, "scratch alignment");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch array (looked up once per device, with
+// that device current).  The array's name is a host-side stand-in and must
+// never be passed to a kernel.
+int scratch_address(unsigned char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    pycwb::record_error("pycwb_batch_run_noalloc", "device index beyond 63");
+    return PYCWB_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      pycwb::record_error("pycwb_batch_run_noalloc", "no address for the scratch array");
+      return PYCWB_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PYCWB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
+  if (!d || !out) return PYCWB_ERR_ARG;
+  *out = nullptr;
+  pycwb_batch* b = new pycwb_batch();
+  int frc = batch_from_desc(d, true, b);
+  if (frc) {
+    delete b;
+    return frc;
+  }
   if (d->use_graph) {
     hipStream_t cs;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
@@ -1344,6 +1415,34 @@ int pycwb_batch_create(const pycwb_batch_desc* d, pycwb_batch** out) {
   *out = b;
   return PYCWB_OK;
 }
+
+int pycwb_batch_run_noalloc(const pycwb_batch_desc* d, void* stream) {
+  clear_stale_error();
+  pycwb::g_last_error[0] = '\0';
+  if (!d) return PYCWB_ERR_ARG;
+  if (d->n_clusters > kNoAllocMaxClusters) {
+    char why[128];
+    snprintf(why, sizeof(why), "%d clusters exceed the maximum of %d", d->n_clusters,
+             kNoAllocMaxClusters);
+    pycwb::record_error("pycwb_batch_run_noalloc", why);
+    return PYCWB_ERR_ARG;
+  }
+  pycwb_batch b{};
+  int rc = batch_from_desc(d, false, &b);
+  if (rc) return rc;
+  unsigned char* base = nullptr;
+  rc = scratch_address(&base);
+  if (rc) return rc;
+  b.dev.tickets = (unsigned*)base;
+  b.dev.counts = (unsigned*)(base + kNoAllocCountsOff);
+  b.dev.best = (unsigned long long*)(base + kNoAllocBestOff);
+  b.dev.wstats = (double*)(base + kNoAllocWStatsOff);
+  // stage 0 clears the tickets, counts and argmax keys with a fill kernel and
+  // writes every weight statistic it later reads
+  return batch_enqueue(&b, (hipStream_t)stream);
+}
+
+size_t pycwb_batch_noalloc_max_clusters(void) { return kNoAllocMaxClusters; }
 
 int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
   if (!b || stage < 0 || stage > 4) return PYCWB_ERR_ARG;
