@@ -56,13 +56,15 @@ def scan_flops(batch, n_valid):
     return float(np.sum(batch.n_pix.astype(np.float64) * (78.0 * batch.nifo + 129.0))) * n_valid
 
 
-def required_scan_flops(batch, n_valid, active):
-    """Flops the scan needs on this data: the energy test (4n + 4) on every
-    evaluation and the rest of the chain (74n + 125) on the evaluations above
-    the energy threshold; the others contribute exactly nothing."""
+def required_scan_flops(batch, tested, active):
+    """Flops the scan needs on this data: the energy test (4n + 4) on the
+    evaluations whose energy it computes (``tested``: those its tile energy
+    bound does not rule out) and the rest of the chain (74n + 125) on the
+    evaluations above the energy threshold (``active``); the others contribute
+    exactly nothing."""
     n = batch.nifo.astype(np.float64)
-    every = float(np.sum(batch.n_pix.astype(np.float64) * (4.0 * n + 4.0))) * n_valid
-    return every + float(np.sum(active.astype(np.float64) * (74.0 * n + 125.0)))
+    return float(np.sum(tested.astype(np.float64) * (4.0 * n + 4.0))) + \
+        float(np.sum(active.astype(np.float64) * (74.0 * n + 125.0)))
 
 
 def regulator_flops(batch, n_valid):
@@ -78,25 +80,77 @@ def scan_bytes(batch, nets, n_valid):
     return td + rms + sky
 
 
-def cpu_baseline(threads, order=7, P=1024, repeats=2):
-    """Oracle (C/OpenMP restatement of PycWB's Numba scan) on one plain-config
-    cluster: n=3, P=1,024, K=64, all order-7 directions."""
+def cpu_info() -> dict:
+    """CPU model, physical cores of the machine and the cores this process may use."""
+    model, phys = None, set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    if cur:
+                        phys.add((cur.get("physical id"), cur.get("core id")))
+                    cur = {}
+                    continue
+                k, v = (t.strip() for t in line.split(":", 1))
+                cur[k] = v
+                if k == "model name" and model is None:
+                    model = v
+            if cur:
+                phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "physical_cores": len(phys) or None, "usable_cpus": usable}
+
+
+def _time_oracle(o, net, cl, P, threads, repeats):
+    """Median over ``repeats`` runs (after one warm-up) of the oracle's regulator +
+    scan of one cluster over every direction; returns (seconds, evals)."""
+    valid = np.arange(net.n_sky, dtype=np.int64)
+    gamma = 0.5 * 0.5 * 2 / 3
+    times = []
+    for k in range(repeats + 1):
+        t0 = time.perf_counter()
+        reg = o.calculate_dpf(net.FP_t, net.FX_t, cl["rms"], net.n_sky, 3, gamma, 12.0, valid,
+                              threads=threads)
+        REG = np.array([np.float32(0.5 * np.sqrt(2)), reg, 0.0], np.float32)
+        o.scan_sky(3, P, net.n_sky, net.FP_t, net.FX_t, cl["rms"], cl["td00"], cl["td90"],
+                   net.ml, REG, 0.5, 0.5, 12.0, valid, threads=threads)
+        if k:
+            times.append(time.perf_counter() - t0)
+    return float(np.median(times)), float(net.n_sky) * P
+
+
+def cpu_baseline(threads, order=7, P=1024, repeats=5):
+    """The oracle (C/OpenMP restatement of PycWB's Numba regulator + sky scan,
+    not PycWB itself) on the plain configuration's cluster: three detectors,
+    P = 1,024, K = 64, every direction of HEALPix order ``order`` -- the same
+    per-evaluation work as the GPU step.  Median of ``repeats`` runs after a
+    warm-up, on ``threads`` threads, and on one thread over order 5 (the
+    one-thread rate does not depend on the number of directions)."""
     from oracle import oracle as o
     from pycwb_amd import synth
 
     net = synth.make_network(synth.NETWORK_3, order=order, K=64)
     cl = synth.make_cluster(net, P, seed=1001 * 100000)
-    valid = np.arange(net.n_sky, dtype=np.int64)
-    REG = np.array([0.5 * np.sqrt(2), 1.0, 0.0], np.float32)
-    best = math.inf
-    for _ in range(repeats):
-        t0 = time.perf_counter()
-        o.scan_sky(3, P, net.n_sky, net.FP_t, net.FX_t, cl["rms"], cl["td00"], cl["td90"],
-                   net.ml, REG, 0.5, 0.5, 12.0, valid, threads=threads)
-        best = min(best, time.perf_counter() - t0)
-    evals = float(net.n_sky) * P
-    return evals / best, f"one n=3 cluster, P={P}, K=64, order {order} ({net.n_sky} dirs), " \
-                         f"best of {repeats}, {best:.2f} s"
+    med, evals = _time_oracle(o, net, cl, P, threads, repeats)
+    net1 = synth.make_network(synth.NETWORK_3, order=min(order, 5), K=64)
+    cl1 = synth.make_cluster(net1, P, seed=1001 * 100000)
+    med1, evals1 = _time_oracle(o, net1, cl1, P, 1, repeats)
+    info = cpu_info()
+    sample = (f"regulator + scan of one three-detector cluster, P={P}, K=64, HEALPix order "
+              f"{order} ({net.n_sky} directions): median of {repeats} runs after a warm-up, "
+              f"{med:.2f} s on {threads} threads; one thread: order {min(order, 5)}, median "
+              f"{med1:.2f} s")
+    return {"value": evals / med, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": sample, "one_thread_evals_per_s": evals1 / med1,
+            "cpu_model": info["model"], "physical_cores": info["physical_cores"],
+            "usable_cpus": info["usable_cpus"],
+            "note": "C restatement of PycWB's Numba likelihood (oracle/), gcc -O2, OpenMP"}
 
 
 def oracle_parity(batch, nets, nt, rb, lm_gpu, gamma, delta, netCC, reg0, n_check=6):
@@ -129,21 +183,45 @@ def oracle_parity(batch, nets, nt, rb, lm_gpu, gamma, delta, netCC, reg0, n_chec
             "oracle_Lm_max_rel_err": lm_err}
 
 
-def profiled_traffic(clusters, order, K):
-    """HBM bytes per scan-stage launch of the default workload, from the
-    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs, each
-    scaled by a 1 GiB copy); None for other workloads or without the file."""
+def profiled_value(key, clusters, order, K):
+    """A figure of the default workload from the committed rocprofv3 summary
+    (``key: value`` line): the scan stage's HBM bytes per launch
+    (``traffic_bytes_per_scan_stage``, FETCH_SIZE and WRITE_SIZE runs each
+    scaled by a 1 GiB copy) or its VALU wave-instructions per launch
+    (``valu_insts_per_scan_stage``, SQ_INSTS_VALU); None for other workloads
+    or without the file."""
     path = os.path.join(ROOT, "profiles", "batched_step_4096clusters_order7.md")
     if (clusters, order, K) != (4096, 7, 64):
         return None, None
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_scan_stage:"):
+                if line.startswith(key + ":"):
                     return int(line.split(":")[1]), os.path.relpath(path, ROOT)
     except OSError:
         pass
     return None, None
+
+
+def valu_ceiling(stream, blocks=2048, iters=20000, reps=3):
+    """Wave-instructions per second of the VALU probe kernel (eight waves per
+    SIMD of independent float32 FMA chains), best of ``reps``."""
+    import torch
+
+    from pycwb_amd import _lib
+    from pycwb_amd.device import lib
+
+    out = torch.zeros(blocks, dtype=torch.float32, device="cuda")
+    best = 0.0
+    for _ in range(reps + 1):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        _lib.check(lib().pycwb_valu_probe(out.data_ptr(), blocks, iters, stream.cuda_stream),
+                   "pycwb_valu_probe")
+        b.record(stream)
+        torch.cuda.synchronize()
+        best = max(best, blocks * 4 * 64.0 * iters / (a.elapsed_time(b) * 1e-3))
+    return best
 
 
 def refinement_leg(n_clusters, order, K, steps, stream, log):
@@ -781,7 +859,7 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, cpu=True,
     torch.cuda.synchronize()
     reg_ms, scan_ms = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
     active = scan.active_pairs()
-    active_fraction = float(active.sum()) / float(evals)
+    tested = scan.tested_pairs()
     # the plain GPU path on the same clusters
     pl = PlainLoop(batch, nt, seg["valid"], 0.5 * 0.5 * 2 / 3, 0.5, 0.5)
     pl.launch(stream)
@@ -841,8 +919,7 @@ def e2e_leg(stream, log, seconds=600, rate=4096, n_inj=200, order=8, cpu=True,
     t2w_bytes = float(sum(n * (8 * L + 16 * (lv["M"] + 1) * lv["n_time"]) for lv in levels))
     td_flops = float(sizes.sum()) * n * (2 * K + 1) * 300.0
     nd_frac = float(active.sum()) / float(evals)
-    req = float(np.sum(sizes.astype(np.float64) * S * (4.0 * n + 4.0))) + \
-        float(active.sum()) * (74.0 * n + 125.0)
+    req = float(tested.sum()) * (4.0 * n + 4.0) + float(active.sum()) * (74.0 * n + 125.0)
     roof = {
         "t2w": {"bound": "hbm", "achieved_gbs": t2w_bytes / stages["t2w"] / 1e9,
                 "frac": t2w_bytes / stages["t2w"] / 1e9 / PEAK_HBM_GBS,
@@ -1097,13 +1174,20 @@ def main():
     scan_ms = e[1].elapsed_time(e[2])
     rs = bs.results()
     active = bs.active_pairs()
+    tested = bs.tested_pairs()
     sf = scan_flops(batch, n_valid)
-    rf = required_scan_flops(batch, n_valid, active)
+    rf = required_scan_flops(batch, tested, active)
     achieved = rf / (scan_ms * 1e-3) / 1e12
     act_frac = float(active.sum()) / float(batch.evals(n_valid))
+    tested_frac = float(tested.sum()) / float(batch.evals(n_valid))
+    ceiling = valu_ceiling(stream)
+    valu_insts, valu_src = profiled_value("valu_insts_per_scan_stage", args.clusters, args.order,
+                                          args.K)
+    issue = None if valu_insts is None else valu_insts / (scan_ms * 1e-3)
     log(f"scan stage {scan_ms:.1f} ms ({achieved:.2f} TFLOP/s required work, "
-        f"{sf / (scan_ms * 1e-3) / 1e12:.2f} nominal), active fraction {act_frac:.4f}, "
-        f"regulator {reg_ms:.1f} ms")
+        f"{sf / (scan_ms * 1e-3) / 1e12:.2f} effective), active fraction {act_frac:.4f}, "
+        f"energy-tested fraction {tested_frac:.4f}, regulator {reg_ms:.1f} ms, VALU ceiling "
+        f"{ceiling:.3e} wave-instr/s")
 
     plain = None
     parity = {"batched_vs_plain_l_max": None}
@@ -1134,10 +1218,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        v, sample = cpu_baseline(threads, order=args.order)
-        cpu = {"value": v, "unit": "evals/s", "cores": threads, "kind": "port", "sample": sample}
-        log(f"cpu baseline {v:.4e} evals/s on {threads} threads")
+        threads = args.cpu_threads or min(16, cpu_info()["usable_cpus"])
+        cpu = cpu_baseline(threads, order=args.order)
+        log(f"cpu baseline {cpu['value']:.4e} evals/s on {threads} threads "
+            f"({cpu['one_thread_evals_per_s']:.4e} on one), {cpu['cpu_model']}")
 
     refinement = None
     if not args.no_refine:
@@ -1156,7 +1240,8 @@ def main():
              if v and v.get("cpu_baseline")}
     e2e = None if args.no_e2e else e2e_leg(stream, log, cpu=not args.no_cpu, cpu_rates=rates)
 
-    traffic, traffic_src = profiled_traffic(args.clusters, args.order, args.K)
+    traffic, traffic_src = profiled_value("traffic_bytes_per_scan_stage", args.clusters,
+                                          args.order, args.K)
     out = {
         "metric": "sky x cluster-pixel evals/s, batched vs plain GPU at Lm rel.err<=1e-5",
         "value": evals / ms * 1e3,
@@ -1183,13 +1268,26 @@ def main():
                      "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_batch_persistent<N,BLOCK,1> (scan stage)",
                      "kernel_ms": round(scan_ms, 3), "required_flops": rf,
+                     "required_flops_rule": "(4n+4) per energy-tested evaluation + (74n+125) "
+                                            "per evaluation above the energy threshold",
                      "nominal_flops": sf,
-                     "nominal_tflops": sf / (scan_ms * 1e-3) / 1e12,
-                     "active_fraction": act_frac,
+                     "effective_tflops": sf / (scan_ms * 1e-3) / 1e12,
+                     "effective_note": "(78n+129) flops on every evaluation / scan time: a "
+                                       "throughput, not a roofline (exact skips)",
+                     "active_fraction": act_frac, "energy_tested_fraction": tested_frac,
+                     "valu_issue": {"insts_per_launch": valu_insts, "source": valu_src,
+                                    "wave_insts_per_s": issue,
+                                    "ceiling_wave_insts_per_s": ceiling,
+                                    "frac": None if issue is None else issue / ceiling,
+                                    "ceiling_kernel": "pycwb_valu_probe (float32 FMA chains, "
+                                                      "8 waves per SIMD)"},
                      "algorithmic_bytes": scan_bytes(batch, nt, n_valid),
                      "hbm_gbs": scan_bytes(batch, nt, n_valid) / (scan_ms * 1e-3) / 1e9},
         "regulator": {"ms": round(reg_ms, 3),
-                      "tflops": regulator_flops(batch, n_valid) / (reg_ms * 1e-3) / 1e12},
+                      "nominal_tflops": regulator_flops(batch, n_valid) / (reg_ms * 1e-3) / 1e12,
+                      "nominal_note": "(24n+25) flops per pixel of every direction / regulator "
+                                      "time: a throughput, not a roofline (the term bounds "
+                                      "decide most directions before their last pixel)"},
         "cpu_baseline": cpu,
         "refinement": refinement,
         "wdm": wdm,

@@ -170,7 +170,10 @@ int pycwb_batch_launch(pycwb_batch* batch, void* stream);
  * 3 finish), for per-stage timing; stages must run in order.  Stage 4, run
  * right after stage 0 instead of 1-3, writes into the per-cluster counts the
  * number of (direction, pixel) pairs above the energy threshold (the pairs
- * the scan evaluates in full); it is a measurement aid. */
+ * the scan evaluates in full); stage 5, likewise after stage 0 and once a
+ * launch has finished, the number of pairs whose energy the scan evaluates
+ * (the others are ruled out by its tile energy bound).  Both are measurement
+ * aids. */
 int pycwb_batch_launch_stage(pycwb_batch* batch, int stage, void* stream);
 int pycwb_batch_destroy(pycwb_batch* batch);
 
@@ -189,6 +192,13 @@ int pycwb_batch_destroy(pycwb_batch* batch);
  * that overlap on one GPU share that array and race: serialise them. */
 int pycwb_batch_run_noalloc(const pycwb_batch_desc* desc, void* stream);
 size_t pycwb_batch_noalloc_max_clusters(void);
+
+/* Measurement aid, no counterpart in the reference: a kernel of known VALU
+ * instruction count (blocks x 4 wavefronts, each 64 * iters float32 FMAs in
+ * eight independent chains, no memory traffic) whose run time gives the
+ * device's VALU issue ceiling in wave-instructions per second.  out needs
+ * `blocks` floats (never written in practice). */
+int pycwb_valu_probe(float* out, int blocks, int iters, void* stream);
 
 /* Best-sky statistics of a batch of clusters at their selected directions,
  * one workgroup per cluster.  Replaces compute_statistics_at_sky_position

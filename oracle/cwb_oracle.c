@@ -189,6 +189,36 @@ double oracle_calculate_dpf(int n_ifo, int n_pix, int n_sky, const float *FP,
     return (FF * FF / (ff * ff + 1.e-9) - 1) * En;
 }
 
+/* Per-pixel float32 DPF terms of the listed directions (dpf_np_loops_vec,
+ * dpf.py:342-453): fp, fx and ni as the mixed-precision oracle computes them,
+ * each (n_dirs, P). */
+void oracle_dpf_terms(int n_ifo, int n_pix, const float *FP, const float *FX, const float *rms,
+                      const int64_t *dirs, int64_t n_dirs, float *fp_out, float *fx_out,
+                      float *ni_out, int n_threads)
+{
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel
+#endif
+    {
+        float *f = malloc(sizeof(float) * n_pix * n_ifo * 2 + sizeof(float) * n_pix * 5);
+        float *F = f + n_pix * n_ifo, *fp = F + n_pix * n_ifo, *fx = fp + n_pix,
+              *si = fx + n_pix, *co = si + n_pix, *ni = co + n_pix;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int64_t k = 0; k < n_dirs; ++k) {
+            int64_t s = dirs[k];
+            dpf_mixed(FP + s * n_ifo, FX + s * n_ifo, rms, n_pix, n_ifo, f, F, fp, fx, si, co,
+                      ni);
+            memcpy(fp_out + k * n_pix, fp, sizeof(float) * n_pix);
+            memcpy(fx_out + k * n_pix, fx, sizeof(float) * n_pix);
+            memcpy(ni_out + k * n_pix, ni, sizeof(float) * n_pix);
+        }
+        free(f);
+    }
+}
+
 double oracle_calculate_dpf_f64(int n_ifo, int n_pix, int n_sky, const double *FP,
                                 const double *FX, const double *rms,
                                 double gamma_regulator, double En,

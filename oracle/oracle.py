@@ -78,6 +78,21 @@ def calculate_dpf(FP, FX, rms, n_sky, n_ifo, gamma_regulator, En, valid, threads
     return (r, q) if return_quality else r
 
 
+def dpf_terms(FP, FX, rms, dirs, threads=0):
+    """Per-pixel float32 (fp, fx, ni) of dpf_np_loops_vec for the listed
+    directions, each (len(dirs), P)."""
+    FP, FX, rms = _c(FP, np.float32), _c(FX, np.float32), _c(rms, np.float32)
+    dirs = _c(dirs, np.int64)
+    P, n = rms.shape
+    out = [np.zeros((dirs.size, P), np.float32) for _ in range(3)]
+    fn = load().oracle_dpf_terms
+    fn.restype = None
+    fn.argtypes = [C.c_int, C.c_int] + [vp] * 4 + [C.c_int64] + [vp] * 3 + [C.c_int]
+    fn(n, P, _p(FP), _p(FX), _p(rms), _p(dirs), dirs.size, _p(out[0]), _p(out[1]), _p(out[2]),
+       threads)
+    return tuple(out)
+
+
 def calculate_dpf_f64(FP, FX, rms, n_sky, n_ifo, gamma_regulator, En, valid, threads=0):
     FP, FX, rms = _c(FP, np.float64), _c(FX, np.float64), _c(rms, np.float64)
     valid = _c(valid, np.int64)

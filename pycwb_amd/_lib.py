@@ -171,6 +171,7 @@ SIGNATURES = {
     "pycwb_batch_launch_stage": (i32, [vp, i32, vp]),
     "pycwb_batch_destroy": (i32, [vp]),
     "pycwb_batch_run_noalloc": (i32, [C.POINTER(BatchDesc), vp]),
+    "pycwb_valu_probe": (i32, [vp, i32, i32, vp]),
     "pycwb_batch_noalloc_max_clusters": (sz, []),
     "pycwb_sky_stats_scratch32": (sz, [i32, i32]),
     "pycwb_sky_stats_scratch64": (sz, [i32, i32]),

@@ -376,6 +376,16 @@ class BatchedSkyScan:
         torch.cuda.current_stream().synchronize()
         return self.counts.cpu().numpy().astype(np.uint32).astype(np.int64)
 
+    def tested_pairs(self) -> np.ndarray:
+        """Per cluster, the (direction, pixel) pairs whose energy the scan
+        evaluates (the rest are ruled out by the tile energy bound without
+        loading their amplitudes).  Overwrites the regulator counts and runs
+        the scan once more with the regulator of the last launch."""
+        self.launch_stage(0)
+        self.launch_stage(5)
+        torch.cuda.current_stream().synchronize()
+        return self.counts.cpu().numpy().astype(np.uint32).astype(np.int64)
+
     def close(self) -> None:
         if self.handle is not None and self.handle.value:
             lib().pycwb_batch_destroy(self.handle)

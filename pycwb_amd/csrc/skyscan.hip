@@ -458,12 +458,16 @@ __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, in
 // bound and every lane then tests only those, in increasing order.  With
 // skip_masked the pairs passing the energy test go through the wavefront's
 // queue (queue_flush); otherwise every lane evaluates every pixel itself.
-template <int N, int WAVES>
+// COUNT_TESTED (a measurement pass) adds to *tested the (direction, pixel)
+// pairs whose energy the wavefront evaluates: candidate pixels times its
+// directions.
+template <int N, int WAVES, bool COUNT_TESTED = false>
 __device__ __forceinline__ unsigned long long scan_lane(
     const float* __restrict__ FP, const float* __restrict__ FX, const int32_t* __restrict__ ml,
     int64_t S, const float* __restrict__ rms, const float2* __restrict__ td, int P, int K,
     float rr, float RR, double En, double netCC, int64_t s, int64_t k, bool active,
-    bool skip_masked, bool tile_bound, float* __restrict__ maps) {
+    bool skip_masked, bool tile_bound, float* __restrict__ maps,
+    unsigned* __restrict__ tested = nullptr) {
   __shared__ WaveQueue<N> queues[WAVES];
   const int nd = 2 * K + 1;
   const int lane = (int)(threadIdx.x & 63);
@@ -490,6 +494,7 @@ __device__ __forceinline__ unsigned long long scan_lane(
   wave_sync();
   SkySums acc;
   sums_init(acc);
+  unsigned n_tested = 0;  // candidate pixels (COUNT_TESTED)
   const int64_t pstride = (int64_t)N * nd;
   for (int j0 = 0; j0 < P; j0 += 64) {
     const int left = P - j0;
@@ -499,6 +504,7 @@ __device__ __forceinline__ unsigned long long scan_lane(
       const bool c = j < P && pixel_candidate<N>(td + (int64_t)j * pstride, nd, lo, hi, En);
       cand = __ballot(c);
     }
+    if constexpr (COUNT_TESTED) n_tested += (unsigned)__popcll(cand);
     if (!cand) continue;
     // software pipeline: the next candidate's amplitudes load while this one computes
     int b = __ffsll((long long)cand) - 1;
@@ -557,6 +563,10 @@ __device__ __forceinline__ unsigned long long scan_lane(
   if (T > 0) {
     wave_sync();
     queue_flush<N>(q, T, T, lane, rms, P, rr, RR, En, acc);
+  }
+  if constexpr (COUNT_TESTED) {
+    const unsigned lanes = (unsigned)__popcll(__ballot(active));
+    if (lane == 0 && tested) atomicAdd(tested, n_tested * lanes);
   }
   SkyStat st;
   sky_epilogue(acc, N, netCC, st);
@@ -758,7 +768,9 @@ template <int PHASE, int N>
 struct PhaseOccupancy {
   static constexpr int waves =
       PHASE == 0 ? (N <= 3 ? PYCWB_OCC_REG_SMALL : PYCWB_OCC_REG_LARGE)
-                 : (PHASE == 1 ? (N <= 3 ? PYCWB_OCC_SCAN_SMALL : PYCWB_OCC_SCAN_LARGE) : 8);
+                 : ((PHASE == 1 || PHASE == 3) ? (N <= 3 ? PYCWB_OCC_SCAN_SMALL
+                                                         : PYCWB_OCC_SCAN_LARGE)
+                                               : 8);
 };
 
 // Regulator work item: reg_pool (at most kRegPool) tiles of BLOCK directions,
@@ -943,6 +955,17 @@ __device__ __forceinline__ void batch_item(const BatchDev& b, int c, int tile) {
       }
     }
     block_count<BLOCK>(cnt, &b.counts[c]);
+  } else if (PHASE == 3) {
+    // the scan with its regulator from the last finished launch, counting the
+    // pairs whose energy it evaluates into counts[] (no argmax)
+    const double En = b.net_En[net];
+    const int32_t* ml = b.ml + b.net_off[net];
+    const float2* td = b.td + b.td_off[c];
+    (void)scan_lane<N, BLOCK / 64, true>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, b.REG1[c],
+                                         En, b.netCC, s, k, active,
+                                         (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
+                                         (b.flags & PYCWB_TILE_ENERGY_BOUND) != 0, nullptr,
+                                         &b.counts[c]);
   } else {
     const double En = b.net_En[net];
     const float RR = (float)regulator_value(b.n_valid, b.counts[c], En);
@@ -1445,7 +1468,7 @@ int pycwb_batch_run_noalloc(const pycwb_batch_desc* d, void* stream) {
 size_t pycwb_batch_noalloc_max_clusters(void) { return kNoAllocMaxClusters; }
 
 int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
-  if (!b || stage < 0 || stage > 4) return PYCWB_ERR_ARG;
+  if (!b || stage < 0 || stage > 5) return PYCWB_ERR_ARG;
   clear_stale_error();
   hipStream_t st = (hipStream_t)stream;
   const BatchDev& d = b->dev;
@@ -1467,15 +1490,20 @@ int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
   }
   // stage 4: count pass (energy test only) into counts[]; uses the regulator
   // tickets, so it runs after stage 0 and before anything that reads counts.
-  const int phase = stage == 4 ? 2 : stage - 1;
+  // stage 5: the scan counting the pairs whose energy it evaluates into
+  // counts[] (measurement pass; runs after stage 0, reads REG1 of the last
+  // finished launch)
+  const int phase = stage == 4 ? 2 : (stage == 5 ? 3 : stage - 1);
   for (int g = 0; g < b->n_groups; ++g) {
-    unsigned* ticket = d.tickets + kTicketsPerGroup * g + (phase == 1 ? kTicketsPerGroup / 2 : 0);
+    unsigned* ticket =
+        d.tickets + kTicketsPerGroup * g + ((phase == 1 || phase == 3) ? kTicketsPerGroup / 2 : 0);
     int rc = PYCWB_ERR_ARG;
     const int n = b->group_nifo[g], first = b->group_first[g], cnt = b->group_count[g];
 #define PHASES(BLK)                                                                \
-  rc = phase == 2 ? launch_group<BLK, 2>(n, d, b->grid, first, cnt, ticket, st)   \
-       : phase ? launch_group<BLK, 1>(n, d, b->grid, first, cnt, ticket, st)      \
-               : launch_group<BLK, 0>(n, d, b->grid, first, cnt, ticket, st);
+  rc = phase == 3   ? launch_group<BLK, 3>(n, d, b->grid, first, cnt, ticket, st) \
+       : phase == 2 ? launch_group<BLK, 2>(n, d, b->grid, first, cnt, ticket, st) \
+       : phase      ? launch_group<BLK, 1>(n, d, b->grid, first, cnt, ticket, st) \
+                    : launch_group<BLK, 0>(n, d, b->grid, first, cnt, ticket, st);
     if (b->block == 64) {
       PHASES(64)
     } else if (b->block == 128) {
