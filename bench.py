@@ -433,6 +433,7 @@ def subnet_leg(stream, log, n_clusters=4096, loud=300, order=4, K=64, cpu=True):
     from pycwb_amd import synth
     from pycwb_amd.batch import synth_batch
     from pycwb_amd.skystats import self_xtalk
+    from pycwb_amd.xtalk import self_xtalk_batch
     from pycwb_amd.super_cluster_native.sub_net_cut import (SubnetBatch,
                                                             calculate_e2or_from_acore, needs_mra,
                                                             sky_tuple, subnet_sky_tables)
@@ -473,19 +474,34 @@ def subnet_leg(stream, log, n_clusters=4096, loud=300, order=4, K=64, cpu=True):
     sb.launch_mra(h, stream)
     e[3].record(stream)
     torch.cuda.synchronize()
-    del h
     mra_ms = e[2].elapsed_time(e[3])
+    mra_host = sb.mra_d.view(-1, 8)[need].cpu().numpy()
+    del h
+    # the same from the batch's device CSRs (as the cross-talk builder returns them)
+    sb.mra_d.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dcsr = self_xtalk_batch(sizes, dev)
+    h = sb.prepare_mra(need, dcsr)
+    sb.launch_mra(h, stream)
+    torch.cuda.synchronize()
+    dev_s = time.perf_counter() - t0
+    mra_dev_equal = bool(np.array_equal(sb.mra_d.view(-1, 8)[need].cpu().numpy(), mra_host))
+    del h, dcsr
     nd = 2 * K + 1
     td_bytes = float(np.sum(sizes.astype(np.float64) * batch.nifo * nd * 8))
     res = {"clusters": n_clusters, "loud": loud, "healpix_order": order, "n_sky": S,
            "pixels": int(sizes.sum()), "evals": evals, "sky_ms": round(sky_ms, 3),
            "evals_per_s": evals / (sky_ms * 1e-3), "mra_clusters": len(need),
            "mra_ms": round(mra_ms, 3), "mra_host_prep_ms": round(prep_s * 1e3, 3),
+           "mra_device_csr_wall_ms": round(dev_s * 1e3, 3),
+           "mra_device_csr_equal": mra_dev_equal,
            "td_bytes": td_bytes, "td_gbs": td_bytes / (sky_ms * 1e-3) / 1e9,
            "frac_hbm": td_bytes / (sky_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
     log(f"subnet: sky loop {sky_ms:.2f} ms ({evals / (sky_ms * 1e-3):.3e} evals/s), "
         f"MRA of {len(need)} clusters {mra_ms:.2f} ms (host CSR preparation "
-        f"{prep_s * 1e3:.1f} ms)")
+        f"{prep_s * 1e3:.1f} ms; from device CSRs {dev_s * 1e3:.1f} ms wall, equal "
+        f"{mra_dev_equal})")
     if cpu:
         from oracle import oracle as ora
 
@@ -541,14 +557,36 @@ def xtalk_leg(stream, log, n_clusters=4096, cpu=True):
         clusters.append(((L + 1).astype(np.int32), (ts * (L + 1) + fs).astype(np.int64)))
     cat.build_batch(clusters[:8], stream)             # warm (catalog upload)
     torch.cuda.synchronize()
+    # the batch's pixels laid end to end, as a clustering stage hands them over
+    flat_l = np.concatenate([c[0] for c in clusters])
+    flat_t = np.concatenate([c[1] for c in clusters])
     t = time.perf_counter()
-    csr = cat.build_batch(clusters, stream)
+    csr = cat.build_batch_flat(flat_l, flat_t, sizes, stream)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    csr_list = cat.build_batch(clusters, stream)
+    torch.cuda.synchronize()
+    ms_list = (time.perf_counter() - t) * 1e3
+    same = bool(np.array_equal(csr.xt_off, csr_list.xt_off) and
+                torch.equal(csr.lookup, csr_list.lookup) and torch.equal(csr.xt, csr_list.xt))
+    ld, td = torch.from_numpy(flat_l).to(stream.device), torch.from_numpy(flat_t).to(stream.device)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    cat.build_batch_flat(ld, td, sizes, stream)
+    torch.cuda.synchronize()
+    ms_dev = (time.perf_counter() - t) * 1e3
     n_pix = int(sizes.sum())
     res = {"clusters": n_clusters, "pixels": n_pix, "entries": int(csr.xt_off[-1]),
-           "ms": round(ms, 3), "pixels_per_s": n_pix / (ms * 1e-3)}
-    log(f"xtalk: CSR of {n_pix} pixels ({res['entries']} entries) in {ms:.1f} ms")
+           "ms": round(ms, 3), "pixels_per_s": n_pix / (ms * 1e-3),
+           "ms_device_inputs": round(ms_dev, 3), "ms_cluster_list": round(ms_list, 3),
+           "list_equals_flat": same,
+           "note": "ms: build_batch_flat from host arrays (uploads, hash, catalog walk, scan, "
+                   "host read of the total, ordered fill); ms_device_inputs: the same from "
+                   "device tensors; ms_cluster_list: build_batch on a list of 4096 per-cluster "
+                   "arrays"}
+    log(f"xtalk: CSR of {n_pix} pixels ({res['entries']} entries) in {ms:.1f} ms "
+        f"(device inputs {ms_dev:.1f} ms, cluster list {ms_list:.1f} ms, equal {same})")
     if cpu:
         from oracle import oracle as ora
 
@@ -618,9 +656,20 @@ def select_leg(stream, log, M=64, seconds=600, rate=4096, n_det=5, n_lag=128, cp
     ms = (time.perf_counter() - t) * 1e3
     pix = n_freq * n_time * n_lag
     sel = int(sum(o["time"].size for o in out))
+    from pycwb_amd.coherence_native.selection import select_network_pixels_device
+
+    t = time.perf_counter()
+    dsel = select_network_pixels_device(cache, eo, shifts, stream=stream)
+    stream.synchronize()
+    ms_dev = (time.perf_counter() - t) * 1e3
+    del dsel
     res = {"M": M, "detectors": n_det, "lags": n_lag, "pixels_per_lag": n_freq * n_time,
-           "selected": sel, "ms": round(ms, 3), "lag_pixels_per_s": pix / (ms * 1e-3)}
-    log(f"select: {n_lag} lags x {n_freq * n_time} pixels in {ms:.1f} ms, {sel} selected")
+           "selected": sel, "ms": round(ms, 3), "lag_pixels_per_s": pix / (ms * 1e-3),
+           "ms_device_payload": round(ms_dev, 3),
+           "note": "ms: select_network_pixels_lags with the payload brought to the host; "
+                   "ms_device_payload: the same pass with the payload left on the device"}
+    log(f"select: {n_lag} lags x {n_freq * n_time} pixels in {ms:.1f} ms ({ms_dev:.1f} ms with "
+        f"the payload on the device), {sel} selected")
     if cpu:
         from oracle import oracle as ora
 

@@ -165,6 +165,19 @@ def select_network_pixels_device(cache: dict, energy_threshold: float, shift_bin
             "n_time": n_time}
 
 
+def _to_host(tensors, stream):
+    """Device tensors to host arrays through page-locked buffers, every copy
+    queued on ``stream`` behind the work that wrote them, one wait at the end."""
+    out = []
+    with torch.cuda.stream(stream):
+        for a in tensors:
+            h = torch.empty(a.shape, dtype=a.dtype, pin_memory=True)
+            h.copy_(a, non_blocking=True)
+            out.append(h)
+    stream.synchronize()
+    return [h.numpy() for h in out]
+
+
 def select_network_pixels_lags(cache: dict, energy_threshold: float, shift_bins: np.ndarray,
                                veto=None, stream=None, with_mask: bool = True) -> list[dict]:
     """Every lag of ``shift_bins`` ((n_lag, n_ifo) bins) in one GPU pass; one
@@ -174,12 +187,14 @@ def select_network_pixels_lags(cache: dict, energy_threshold: float, shift_bins:
     the bytes brought back to the host."""
     stream = stream or torch.cuda.current_stream()
     sel = select_network_pixels_device(cache, energy_threshold, shift_bins, veto, stream)
-    stream.synchronize()
     n_lag, lo = sel["n_lag"], sel["lag_off"]
-    fr_h, tm_h, val_h = (sel[k].cpu().numpy() for k in ("frequency", "time", "energy"))
-    de_h, di_h = sel["pix_det_energy"].cpu().numpy(), sel["pix_det_index"].cpu().numpy()
-    flags_h = sel["flags"].cpu().numpy() if with_mask else None
-    live_h = sel["live"].cpu().numpy()
+    keys = ["frequency", "time", "energy", "pix_det_energy", "pix_det_index", "live"] + \
+        (["flags"] if with_mask else [])
+    host = dict(zip(keys, _to_host([sel[k] for k in keys], stream)))
+    fr_h, tm_h, val_h = host["frequency"], host["time"], host["energy"]
+    de_h, di_h = host["pix_det_energy"], host["pix_det_index"]
+    flags_h = host["flags"] if with_mask else None
+    live_h = host["live"]
     out = []
     for l in range(n_lag):
         a, b = int(lo[l]), int(lo[l + 1])

@@ -259,31 +259,54 @@ class SubnetBatch:
 
     def prepare_mra(self, clusters, csrs) -> dict:
         """Validate and upload the cross-talk CSRs of the given clusters
-        (csrs[k] = (lookup, xtalk) of clusters[k]); the returned handle feeds
-        launch_mra."""
+        (csrs[k] = (lookup, xtalk) of clusters[k]), or take the device CSRs of
+        the whole batch as pycwb_amd.xtalk.XTalk.build_batch returns them
+        (an XTalkCSR, used in place: its rows were built valid); the returned
+        handle feeds launch_mra."""
         clusters = np.asarray(clusters, np.int64).reshape(-1)
         b, d = self.batch, self.device
-        lk_parts, xt_parts = [], []
+        if clusters.size and (clusters.min() < 0 or clusters.max() >= b.n_clusters):
+            raise ValueError("cluster index out of range")
         lk_off = np.zeros(b.n_clusters, np.int64)
         xt_off = np.zeros(b.n_clusters, np.int64)
         scr_off = np.zeros(b.n_clusters, np.int64)
-        lo = xo = so = 0
-        for c, (lookup, xt) in zip(clusters, csrs):
-            P, n = int(b.n_pix[c]), int(b.nifo[c])
-            lookup, xt = _check_csr(lookup, xt, P)
-            lk_parts.append(lookup.ravel())
-            xt_parts.append(xt.ravel())
-            lk_off[c], xt_off[c], scr_off[c] = lo, xo, so
-            lo += P
-            xo += xt.shape[0]
-            so += int(lib().pycwb_subnet_mra_scratch(n, P))
         t = lambda a: torch.from_numpy(a).to(d)
+        scratch_of = {}
+
+        def scratch(n, P):
+            if (n, P) not in scratch_of:
+                scratch_of[(n, P)] = int(lib().pycwb_subnet_mra_scratch(n, P))
+            return scratch_of[(n, P)]
+
+        if hasattr(csrs, "pix_off") and hasattr(csrs, "xt_off"):
+            if csrs.pix_off.size != b.n_clusters + 1 or \
+                    np.any(np.diff(csrs.pix_off) != b.n_pix):
+                raise ValueError("the device CSRs must cover the batch's clusters in order")
+            lk_off[:] = csrs.pix_off[:-1]
+            xt_off[:] = csrs.xt_off[:-1]
+            lk_d, xt_d = csrs.lookup, csrs.xt
+            so = 0
+            for c in clusters:
+                scr_off[c] = so
+                so += scratch(int(b.nifo[c]), int(b.n_pix[c]))
+        else:
+            lk_parts, xt_parts = [], []
+            lo = xo = so = 0
+            for c, (lookup, xt) in zip(clusters, csrs):
+                P, n = int(b.n_pix[c]), int(b.nifo[c])
+                lookup, xt = _check_csr(lookup, xt, P)
+                lk_parts.append(lookup.ravel())
+                xt_parts.append(xt.ravel())
+                lk_off[c], xt_off[c], scr_off[c] = lo, xo, so
+                lo += P
+                xo += xt.shape[0]
+                so += scratch(n, P)
+            lk_d = t(np.concatenate(lk_parts + [np.zeros(2, np.int32)]))
+            xt_d = t(np.concatenate(xt_parts + [np.zeros(8, np.float32)]))
         nifo = b.nifo[clusters]
         groups = {n: t(clusters[nifo == n].astype(np.int32))
                   for n in sorted(set(int(x) for x in nifo))}
-        return {"clusters": clusters, "groups": groups,
-                "lk": t(np.concatenate(lk_parts + [np.zeros(2, np.int32)])),
-                "xt": t(np.concatenate(xt_parts + [np.zeros(8, np.float32)])),
+        return {"clusters": clusters, "groups": groups, "lk": lk_d, "xt": xt_d,
                 "lk_off": t(lk_off), "xt_off": t(xt_off), "scr_off": t(scr_off),
                 "scratch": torch.empty(max(so, 1), dtype=torch.float32, device=d)}
 

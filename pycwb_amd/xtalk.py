@@ -189,6 +189,28 @@ class XTalkCSR:
         return self.lookup[a:b].cpu().numpy(), self.xt[u:v].cpu().numpy()
 
 
+def self_xtalk_batch(n_pix, device=None) -> XTalkCSR:
+    """Self-only cross-talk of a whole batch as device CSRs: each pixel lists
+    itself with coefficients (1, 0, 0, 1) in columns 4-7, the CSRs of
+    pycwb_amd.skystats.self_xtalk for every cluster laid end to end."""
+    device = device or require_gpu()
+    n_pix = np.asarray(n_pix, dtype=np.int64).reshape(-1)
+    if (n_pix < 0).any():
+        raise ValueError("cluster sizes must be >= 0")
+    pix_off = np.concatenate([[0], np.cumsum(n_pix)]).astype(np.int64)
+    n = int(pix_off[-1])
+    first = torch.from_numpy(np.repeat(pix_off[:-1], n_pix)).to(device)
+    j = torch.arange(n, dtype=torch.int64, device=device) - first   # index inside the cluster
+    lookup = torch.zeros((max(n, 1), 2), dtype=torch.int32, device=device)
+    lookup[:n, 0] = j.to(torch.int32)
+    lookup[:n, 1] = (j + 1).to(torch.int32)
+    xt = torch.zeros((max(n, 1), 8), dtype=torch.float32, device=device)
+    xt[:n, 0] = j.to(torch.float32)
+    xt[:n, 4] = 1.0
+    xt[:n, 7] = 1.0
+    return XTalkCSR(pix_off=pix_off, xt_off=pix_off.copy(), lookup=lookup, xt=xt)
+
+
 @dataclass
 class XTalk:
     """The WDM cross-talk catalog (type.py:6-74)."""
@@ -229,31 +251,58 @@ class XTalk:
     def build_batch(self, clusters, stream=None, device=None) -> XTalkCSR:
         """CSRs of many clusters; clusters[c] = (layers, times) of its pixels
         (PixelArrays.layers, PixelArrays.time)."""
-        device = device or require_gpu()
-        nl, tm, sizes = [], [], []
+        lay, tim, sizes = [], [], []
         for layers, times in clusters:
-            layers = np.asarray(layers, dtype=np.int64).reshape(-1)
-            times = np.asarray(times, dtype=np.int64).reshape(-1)
+            layers = np.asarray(layers).reshape(-1)
+            times = np.asarray(times).reshape(-1)
             if layers.shape != times.shape:
                 raise ValueError("layers and times must have the same length")
-            if layers.size and (times.min() < 0 or layers.min() < 1 or
-                                layers.max() >= 1 << 20 or times.max() >= 1 << 32):
-                raise ValueError("pixel layers must be in [1, 2^20) and times in [0, 2^32)")
-            nl.append(layers.astype(np.int32))
-            tm.append(times)
+            lay.append(layers)
+            tim.append(times)
             sizes.append(layers.size)
-        n_cl = len(sizes)
+        if not sizes:
+            raise ValueError("no clusters")
+        return self.build_batch_flat(np.concatenate(lay), np.concatenate(tim), sizes, stream,
+                                     device)
+
+    def build_batch_flat(self, layers, times, sizes, stream=None, device=None) -> XTalkCSR:
+        """build_batch on the batch's pixels laid end to end: ``layers`` and
+        ``times`` (host arrays or device tensors) hold cluster 0's pixels, then
+        cluster 1's, ...; ``sizes[c]`` is cluster c's pixel count."""
+        device = device or require_gpu()
+        d = device
+        sizes = np.asarray(sizes, dtype=np.int64).reshape(-1)
+        n_cl = sizes.size
         if n_cl == 0:
             raise ValueError("no clusters")
+        if (sizes < 0).any():
+            raise ValueError("cluster sizes must be >= 0")
+        n = int(sizes.sum())
+
+        def dev(a, dtype):
+            if isinstance(a, torch.Tensor):
+                return a.reshape(-1).to(device=d, dtype=dtype).contiguous()
+            return torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1),
+                                                         dtype=np.dtype(str(dtype)[6:]))).to(d)
+
+        nl_d, tm_d = dev(layers, torch.int64), dev(times, torch.int64)
+        if nl_d.numel() != n or tm_d.numel() != n:
+            raise ValueError("layers and times must hold sum(sizes) pixels")
+        if n:
+            lo_l, hi_l = torch.aminmax(nl_d)
+            lo_t, hi_t = torch.aminmax(tm_d)
+            lim = torch.stack([lo_l, hi_l, lo_t, hi_t]).cpu().numpy()
+            if lim[2] < 0 or lim[0] < 1 or lim[1] >= 1 << 20 or lim[3] >= 1 << 32:
+                raise ValueError("pixel layers must be in [1, 2^20) and times in [0, 2^32)")
+        nl_d = torch.cat([nl_d.to(torch.int32), torch.zeros(1, dtype=torch.int32, device=d)])
+        tm_d = torch.cat([tm_d, torch.zeros(1, dtype=torch.int64, device=d)])
         pix_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-        slots = [1 << max(1, int(2 * s - 1).bit_length()) for s in sizes]
+        # open-addressing table of each cluster: the power of two >= 2 P - 1 (at least 2)
+        _, e = np.frexp(np.maximum(2 * sizes - 1, 1).astype(np.float64))
+        slots = np.left_shift(1, np.maximum(1, e.astype(np.int64)))
         hash_off = np.concatenate([[0], np.cumsum(slots)]).astype(np.int64)
-        n = int(pix_off[-1])
-        d = device
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(d)
         layers_d, table_d, coeff_d = self._device(d)
-        nl_d = t(np.concatenate(nl + [np.zeros(1, np.int32)]))
-        tm_d = t(np.concatenate(tm + [np.zeros(1, np.int64)]))
         pix_off_d, hash_off_d = t(pix_off), t(hash_off)
         n_slots = int(hash_off[-1])
         hkey = torch.empty(n_slots, dtype=torch.int64, device=d)
@@ -276,6 +325,8 @@ class XTalk:
         desc.lookup = ptr(lookup)
         stream = stream or torch.cuda.current_stream()
         st = stream_ptr(stream)
+        # the uploads above ran on torch's current stream
+        stream.wait_stream(torch.cuda.current_stream(d))
         _lib.check(lib().pycwb_xtalk_count(C.byref(desc), st), "pycwb_xtalk_count")
         stream.synchronize()
         xt_off_h = xt_off.cpu().numpy()

@@ -169,6 +169,13 @@ def test_batch_mixed_networks_bit_exact(oracle):
                                cl["rms"], cl["td00"], cl["td90"], nt.ml, En[ids[c]],
                                float(Es[ids[c]]) / 2, xt, lk, int(sky[c][0]))
         assert mra_tuple(row) == o, c
+    # the batch's self-only cross-talk as device CSRs gives the same rows
+    from pycwb_amd.xtalk import self_xtalk_batch
+
+    dcsr = self_xtalk_batch([cl["rms"].shape[0] for cl in cls], dev)
+    sb.mra_d.zero_()
+    mra_d = sb.mra_results(need, dcsr)
+    np.testing.assert_array_equal(mra_d, mra)
 
 
 class _EmptyXTalk:

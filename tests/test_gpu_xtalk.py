@@ -63,6 +63,36 @@ def test_batch_matches_oracle(oracle):
     assert csr.xt_off[-1] == total > 0
 
 
+def test_flat_batch_equals_cluster_list():
+    """build_batch_flat on the pixels laid end to end, from host arrays and
+    from device tensors, gives build_batch's CSR; bad inputs raise."""
+    import torch
+
+    print("case xtalk flat batch", flush=True)
+    rng = np.random.default_rng(21)
+    cat = X.synthetic_catalog([4, 8, 16, 32], seed=21)
+    clusters = [_cluster(rng, cat.layers, (12, 9, 5, 3)),
+                (np.zeros(0, np.int32), np.zeros(0, np.int64)),
+                _cluster(rng, cat.layers, (40, 20, 10, 5), dup=4, foreign=2),
+                _cluster(rng, cat.layers, (2, 0, 1, 0))]
+    sizes = [c[0].size for c in clusters]
+    ref = cat.build_batch(clusters)
+    fl = np.concatenate([c[0] for c in clusters])
+    ft = np.concatenate([c[1] for c in clusters])
+    dev = torch.device("cuda", 0)
+    for lay, tim in ((fl, ft), (torch.from_numpy(fl).to(dev), torch.from_numpy(ft).to(dev))):
+        got = cat.build_batch_flat(lay, tim, sizes)
+        np.testing.assert_array_equal(got.pix_off, ref.pix_off)
+        np.testing.assert_array_equal(got.xt_off, ref.xt_off)
+        assert torch.equal(got.lookup, ref.lookup) and torch.equal(got.xt, ref.xt)
+    with pytest.raises(ValueError):
+        cat.build_batch_flat(fl, ft[:-1], sizes)
+    with pytest.raises(ValueError):
+        cat.build_batch_flat(fl * 0, ft, sizes)          # layers must be >= 1
+    with pytest.raises(ValueError):
+        cat.build_batch_flat(fl, ft, [])
+
+
 def test_single_cluster_entry_points(oracle):
     from pycwb_amd.types import PixelArrays
 
