@@ -34,6 +34,60 @@ __device__ __forceinline__ unsigned bitrev(unsigned v, int bits) {
   return __brev(v) >> (32 - bits);
 }
 
+// In-place radix-2 DIT FFT, exp(-2 pi i f k / 2M), of `count` sequences of
+// length 2M = 2^lg stored back to back in LDS (input in bit-reversed order).
+// Stages run in pairs as radix-4 butterflies (one barrier per pair), the last
+// stage alone when lg is odd; power-of-two index arithmetic throughout.
+// tw[k] = exp(-2 pi i k / 2M), k < M.
+__device__ __forceinline__ void lds_fft(double2* __restrict__ buf, int count, int lg,
+                                        const double2* __restrict__ tw) {
+  const int M2 = 1 << lg;
+  const int qpl = lg - 2;  // log2 of the radix-4 butterflies per sequence
+  int lh = 0;
+  for (; lh + 1 < lg; lh += 2) {
+    const int h = 1 << lh;
+    const int n4 = count << qpl;
+    for (int e = threadIdx.x; e < n4; e += blockDim.x) {
+      const int p = e >> qpl, r = e & ((1 << qpl) - 1);
+      const int g = r >> lh, j = r & (h - 1);
+      const int i0 = (p << lg) + (g << (lh + 2)) + j;
+      const double2 w1 = tw[j << (lg - 1 - lh)];
+      const double2 w2a = tw[j << (lg - 2 - lh)], w2b = tw[(j + h) << (lg - 2 - lh)];
+      const double2 x0 = buf[i0], x1 = buf[i0 + h], x2 = buf[i0 + 2 * h], x3 = buf[i0 + 3 * h];
+      // stage h: (x0, x1) and (x2, x3) with w1
+      double vr = x1.x * w1.x - x1.y * w1.y, vi = x1.x * w1.y + x1.y * w1.x;
+      const double2 a0 = make_double2(x0.x + vr, x0.y + vi), a1 = make_double2(x0.x - vr, x0.y - vi);
+      vr = x3.x * w1.x - x3.y * w1.y;
+      vi = x3.x * w1.y + x3.y * w1.x;
+      const double2 a2 = make_double2(x2.x + vr, x2.y + vi), a3 = make_double2(x2.x - vr, x2.y - vi);
+      // stage 2h: (a0, a2) with w2a, (a1, a3) with w2b
+      vr = a2.x * w2a.x - a2.y * w2a.y;
+      vi = a2.x * w2a.y + a2.y * w2a.x;
+      buf[i0] = make_double2(a0.x + vr, a0.y + vi);
+      buf[i0 + 2 * h] = make_double2(a0.x - vr, a0.y - vi);
+      vr = a3.x * w2b.x - a3.y * w2b.y;
+      vi = a3.x * w2b.y + a3.y * w2b.x;
+      buf[i0 + h] = make_double2(a1.x + vr, a1.y + vi);
+      buf[i0 + 3 * h] = make_double2(a1.x - vr, a1.y - vi);
+    }
+    __syncthreads();
+  }
+  if (lh < lg) {  // the last stage, h = M2 / 2
+    const int h = 1 << lh;
+    const int n2 = count << (lg - 1);
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+      const int p = e >> (lg - 1), j = e & (h - 1);
+      const int i0 = (p << lg) + j, i1 = i0 + h;
+      const double2 w = tw[j];
+      const double2 u = buf[i0], v = buf[i1];
+      const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
+      buf[i0] = make_double2(u.x + vr, u.y + vi);
+      buf[i1] = make_double2(u.x - vr, u.y - vi);
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_t2w(
     const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int M, int stride,
     int64_t N, const double* __restrict__ filt, int m_H, const double2* __restrict__ tw,
@@ -49,7 +103,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w(
   const int nb = (int)((N - n0) < B ? (N - n0) : B);
   // fold: thread (b, k) sums the window taps t = k + 2M*j over |t| <= m_H - 1
   for (int e = threadIdx.x; e < nb * M2; e += kThreads) {
-    const int b = e / M2, k = e % M2;
+    const int b = e >> log2_2M, k = e & (M2 - 1);
     const int64_t base = (n0 + b) * stride;
     double acc = 0.0;
     // t = k - 2M*q (q >= 1) on the negative side, t = k + 2M*q (q >= 0) on the positive
@@ -58,21 +112,8 @@ __global__ __launch_bounds__(kThreads) void k_t2w(
     buf[b * M2 + bitrev((unsigned)k, log2_2M)] = make_double2(acc, 0.0);
   }
   __syncthreads();
-  // radix-2 DIT FFT of length 2M on each bin, exp(-2 pi i f k / 2M)
-  for (int h = 1; h < M2; h <<= 1) {
-    const int tstride = M2 / (2 * h);
-    for (int e = threadIdx.x; e < nb * M; e += kThreads) {
-      const int b = e / M, r = e % M;
-      const int g = r / h, j = r % h;
-      const int i0 = b * M2 + g * 2 * h + j, i1 = i0 + h;
-      const double2 w = tw[j * tstride];
-      const double2 u = buf[i0], v = buf[i1];
-      const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
-      buf[i0] = make_double2(u.x + vr, u.y + vi);
-      buf[i1] = make_double2(u.x - vr, u.y - vi);
-    }
-    __syncthreads();
-  }
+  // DIT FFT of length 2M on each bin, exp(-2 pi i f k / 2M)
+  lds_fft(buf, nb, log2_2M, tw);
   // edges and 00/90 assignment (WDM.cc:1494-1510)
   const double sqrt2 = 1.4142135623730951;
   for (int e = threadIdx.x; e < nb * (M + 1); e += kThreads) {
@@ -132,7 +173,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly(
   // fold
   const int items = (B / R) * M2;
   for (int it = threadIdx.x; it < items; it += kThreads) {
-    const int k = it % M2, r = it / M2;
+    const int k = it & (M2 - 1), r = it >> log2_2M;
     const int64_t nb0 = n0 + (int64_t)r * R;
     double acc[R], win[R];
 #pragma unroll
@@ -160,22 +201,8 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly(
     }
   }
   __syncthreads();
-  // radix-2 DIT FFT of length 2M on each bin pair, exp(-2 pi i f k / 2M)
-  const int pairs = B / 2;
-  for (int h = 1; h < M2; h <<= 1) {
-    const int tstride = M2 / (2 * h);
-    for (int e = threadIdx.x; e < pairs * M; e += kThreads) {
-      const int p = e / M, r = e % M;
-      const int g = r / h, j = r % h;
-      const int i0 = p * M2 + g * 2 * h + j, i1 = i0 + h;
-      const double2 w = tw[j * tstride];
-      const double2 u = buf[i0], v = buf[i1];
-      const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
-      buf[i0] = make_double2(u.x + vr, u.y + vi);
-      buf[i1] = make_double2(u.x - vr, u.y - vi);
-    }
-    __syncthreads();
-  }
+  // DIT FFT of length 2M on each bin pair, exp(-2 pi i f k / 2M)
+  lds_fft(buf, B / 2, log2_2M, tw);
   // split the pairs; edges and 00/90 assignment (WDM.cc:1494-1510)
   const double sqrt2 = 1.4142135623730951;
   for (int e = threadIdx.x; e < B * (M + 1); e += kThreads) {
@@ -238,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void k_w2t_bins(
   const double r2 = 1.4142135623730951;
   const int pairs = B / 2;
   for (int e = threadIdx.x; e < pairs * M2; e += kThreads) {
-    const int p = e / M2, k = e % M2;
+    const int p = e >> log2_2M, k = e & (M2 - 1);
     double wr[2], wi[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -271,22 +298,9 @@ __global__ __launch_bounds__(kThreads) void k_w2t_bins(
     buf[p * M2 + bitrev((unsigned)k, log2_2M)] = make_double2(wr[0] - wi[1], wi[0] + wr[1]);
   }
   __syncthreads();
-  for (int h = 1; h < M2; h <<= 1) {
-    const int tstride = M2 / (2 * h);
-    for (int e = threadIdx.x; e < pairs * M; e += kThreads) {
-      const int p = e / M, r = e % M;
-      const int g = r / h, j = r % h;
-      const int i0 = p * M2 + g * 2 * h + j, i1 = i0 + h;
-      const double2 w = tw[j * tstride];
-      const double2 u = buf[i0], v = buf[i1];
-      const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
-      buf[i0] = make_double2(u.x + vr, u.y + vi);
-      buf[i1] = make_double2(u.x - vr, u.y - vi);
-    }
-    __syncthreads();
-  }
+  lds_fft(buf, pairs, log2_2M, tw);
   for (int e = threadIdx.x; e < B * M2; e += kThreads) {
-    const int b = e / M2, k = e % M2;
+    const int b = e >> log2_2M, k = e & (M2 - 1);
     const int64_t n = n0 + b;
     if (n >= N) continue;
     const double2 z = buf[(b >> 1) * M2 + k];
