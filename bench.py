@@ -1329,7 +1329,11 @@ def main():
                                     "ceiling_wave_insts_per_s": ceiling,
                                     "frac": None if issue is None else issue / ceiling,
                                     "ceiling_kernel": "pycwb_valu_probe (float32 FMA chains, "
-                                                      "8 waves per SIMD)"},
+                                                      "8 waves per SIMD)",
+                                    "note": "frac above 1: the stage issues VALU instructions "
+                                            "faster than a stream of v_fma_f32 (one per SIMD "
+                                            "every 4 cycles): it is bound by instruction "
+                                            "issue, and its instruction count is the lever"},
                      "algorithmic_bytes": scan_bytes(batch, nt, n_valid),
                      "hbm_gbs": scan_bytes(batch, nt, n_valid) / (scan_ms * 1e-3) / 1e9},
         "regulator": {"ms": round(reg_ms, 3),

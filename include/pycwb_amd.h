@@ -86,6 +86,19 @@ int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, co
                         float* REG, double* reg_f64, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* Diagnostic without a counterpart in the reference: the term bounds the
+ * regulator passes decide with (PYCWB_REGULATOR_BOUNDS).  From the cluster's
+ * noise weights rms ((n_pix, n_ifo) float32) writes its weight statistics
+ * (24 doubles: min and max of R2 = sum_i r_i^2, then the per-detector minima
+ * and maxima of r_i^2 / R2 in slots 2.. and 10..) to weight_stats, and for
+ * each of n_dirs directions (rows of FP, FX: (n_dirs, n_ifo) float32) the
+ * lower bound t_lb and upper bound t_ub (< 0: none) on every pixel's term
+ * fx / (ni + 1e-9).  For tests that hold the kernel's bounds against the
+ * float32 terms. */
+int pycwb_regulator_term_bounds(int n_ifo, int n_pix, const float* rms, const float* FP,
+                                const float* FX, int64_t n_dirs, double* weight_stats,
+                                double* t_lb, double* t_ub, void* stream);
+
 /* Sky scan of one cluster.  Replaces scan_sky_for_best_fit
  * (pycwb/modules/likelihoodWP/sky_scan.py:17-192; called at likelihood.py:319).
  * Reads the amplitude pairs from td_packed ((n_pix, n_ifo, n_delay) float
@@ -159,7 +172,7 @@ typedef struct pycwb_batch_desc {
   int block;      /* sky directions per workgroup: 64, 128 or 256 */
   int grid;       /* persistent workgroups */
   int use_graph;  /* capture the launches in a hipGraph */
-  double* weight_stats; /* scratch, 16 doubles per cluster, or NULL (no regulator bounds) */
+  double* weight_stats; /* scratch, 24 doubles per cluster, or NULL (no regulator bounds) */
 } pycwb_batch_desc;
 
 typedef struct pycwb_batch pycwb_batch;

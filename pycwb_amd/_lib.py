@@ -22,7 +22,7 @@ REGULATOR_EARLY_EXIT = 2
 TILE_ENERGY_BOUND = 4
 REGULATOR_BOUNDS = 8
 DEFAULT_FLAGS = SKIP_MASKED_PIXELS | REGULATOR_EARLY_EXIT | TILE_ENERGY_BOUND | REGULATOR_BOUNDS
-WSTATS_PER_CLUSTER = 16
+WSTATS_PER_CLUSTER = 24
 TICKETS_PER_GROUP = 16   # uint32 ticket counters per detector group of a batch
 
 _lock = threading.Lock()
@@ -162,6 +162,7 @@ SIGNATURES = {
     "pycwb_last_error": (C.c_char_p, []),
     "pycwb_sky_scan_workspace_bytes": (sz, [i32, i32, i32]),
     "pycwb_td_relayout": (i32, [i32, i32, i32, vp, vp, vp, sz, vp]),
+    "pycwb_regulator_term_bounds": (i32, [i32, i32, vp, vp, vp, i64, vp, vp, vp, vp]),
     "pycwb_dpf_regulator": (i32, [i32, i32, i64, vp, vp, vp, f64, f64, f32, vp, i64, i32, vp,
                                   vp, vp, sz, vp]),
     "pycwb_sky_scan_plain": (i32, [i32, i32, i64, i32, vp, vp, vp, vp, vp, vp, f64, f64, vp,

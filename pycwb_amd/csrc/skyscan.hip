@@ -106,16 +106,21 @@ __device__ __forceinline__ void block_count(unsigned v, unsigned* dst) {
 
 // Noise-weight statistics of a cluster for the regulator bounds, float64:
 // [0] min_j R2_j, [1] max_j R2_j with R2_j = sum_i r_ji^2, [2 + i] min_j
-// r_ji^2 / R2_j (i < n).  kWStats doubles per cluster.
-constexpr int kWStats = 16;
+// r_ji^2 / R2_j and [10 + i] max_j r_ji^2 / R2_j (i < n).  kWStats doubles per
+// cluster.
+constexpr int kWStats = 24;
+constexpr int kWMin = 2, kWMax = 10;
 
 template <int BLK>
 __device__ void weight_stats_block(const float* __restrict__ rms, int P, int n,
                                    double* __restrict__ out) {
-  __shared__ double s_red[BLK / 64][2 + 8];
-  double lo = __builtin_inf(), hi = 0.0, wmin[8];
+  __shared__ double s_red[BLK / 64][2 + 16];
+  double lo = __builtin_inf(), hi = 0.0, wmin[8], wmax[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) wmin[i] = __builtin_inf();
+  for (int i = 0; i < 8; ++i) {
+    wmin[i] = __builtin_inf();
+    wmax[i] = 0.0;
+  }
   for (int j = threadIdx.x; j < P; j += BLK) {
     double r2[8], R2 = 0.0;
 #pragma unroll
@@ -133,6 +138,7 @@ __device__ void weight_stats_block(const float* __restrict__ rms, int P, int n,
       if (i < n) {
         const double w = r2[i] / R2;
         wmin[i] = (w < wmin[i] || w != w) ? w : wmin[i];  // NaN sticks
+        wmax[i] = (w > wmax[i] || w != w) ? w : wmax[i];
       }
     }
   }
@@ -146,91 +152,280 @@ __device__ void weight_stats_block(const float* __restrict__ rms, int P, int n,
     for (int i = 0; i < 8; ++i) {
       const double c = __shfl_xor(wmin[i], off, 64);
       wmin[i] = (c < wmin[i] || c != c) ? c : wmin[i];
+      const double d = __shfl_xor(wmax[i], off, 64);
+      wmax[i] = (d > wmax[i] || d != d) ? d : wmax[i];
     }
   }
   if (lane == 0) {
     s_red[wave][0] = lo;
     s_red[wave][1] = hi;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) s_red[wave][2 + i] = wmin[i];
+    for (int i = 0; i < 8; ++i) {
+      s_red[wave][kWMin + i] = wmin[i];
+      s_red[wave][kWMax + i] = wmax[i];
+    }
   }
   __syncthreads();
-  if (threadIdx.x < 2 + 8) {
+  if (threadIdx.x < 2 + 16) {
     const int f = threadIdx.x;
+    const bool take_max = f == 1 || f >= kWMax;
     double v = s_red[0][f];
     for (int w = 1; w < BLK / 64; ++w) {
       const double x = s_red[w][f];
-      if (f == 1) v = (x > v || x != x) ? x : v;
+      if (take_max) v = (x > v || x != x) ? x : v;
       else v = (x < v || x != x) ? x : v;
     }
     out[f] = v;
   }
-  for (int f = 2 + 8 + (int)threadIdx.x; f < kWStats; f += BLK) out[f] = 0.0;
+  for (int f = 2 + 16 + (int)threadIdx.x; f < kWStats; f += BLK) out[f] = 0.0;
   __syncthreads();
 }
 
 // Bounds on one direction's regulator terms t_j = fx_j / (ni_j + 1e-9).
-// With M_j = sum_i r_ji^2 v_i v_i^T (v_i = (F+_i, Fx_i)), exact arithmetic
-// gives fx_j = lambda_min(M_j) and 1/n <= ni_j <= 1 (Cauchy-Schwarz on the
-// dominant-frame vector); the float32 pipeline stays within 1e-4 * R2max *
-// max|v|^2 of lambda_min (its rounding is ~1e-6 of fp <= R2 max|v|^2).
-//   lower: lambda_min is concave in the weights, and every pixel's normalised
-//     weights lie in the simplex {w_i >= wmin_i}, so lambda_min(M_j) >= R2min
-//     times the minimum of lambda_min over that simplex's n vertices;
-//   upper: lambda_min(M(w)) <= u^T M(w) u <= max_i (u.v_i)^2 for any unit u,
-//     so lambda_min(M_j) <= R2max * min over sampled u of max_i (u.v_i)^2.
-//     Used only when fp is bounded away from 0 (fp^2 >= 1e-3), which also
-//     makes every pixel count in NN.
+// With M_j = sum_i r_ji^2 v_i v_i^T (v_i = (F+_i, Fx_i)) = R2_j M(w_j), w_j the
+// pixel's normalised weights (w_ji = r_ji^2 / R2_j), exact arithmetic gives
+// fx_j = lambda_min(M_j), fp_j = lambda_max(M_j) and ni_j = sum_i alpha_i^2 with
+// alpha_i = w_ji (u_j.v_i)^2 / lambda_max(M(w_j)) (u_j the dominant eigenvector,
+// sum_i alpha_i = 1), so 1/n <= ni_j <= 1; the float32 pipeline keeps fx within
+// 1e-4 R2max max|v|^2 of lambda_min (its rounding is ~1e-6 of fp <= R2 max|v|^2).
+// Every w_j lies in the polytope W = {wmin <= w <= wmax, sum w = 1}; for up to
+// four detectors its vertices are enumerated (n - 1 coordinates at a bound, the
+// last from the sum), for more the larger simplex {w >= wmin, sum w = 1} and
+// its n vertices stand in for it.  Over the vertices w_v:
+//   lower: lambda_min is concave in w, so lambda_min(M(w_j)) >= C = min_v
+//     lambda_min(M(w_v)).  ni_j <= N_hi: the point (x, y) = (g11 - g22, 2 g12)
+//     of M(w) (its dominant direction at angle 2 theta) is linear in w, so for
+//     w in W it lies in the convex hull of the vertices' points; when that hull
+//     sits in an open half-plane (every vertex projects positively on the
+//     reference direction, the point of the box centre), 2 theta_j lies between
+//     the hull's extreme rays e_lo, e_hi, and (u_j.v_i)^2 = |v_i|^2 (1 +
+//     cos(2 theta_j - 2 phi_i)) / 2 with cos at most 1 if 2 phi_i lies between
+//     them, else max(e_lo.q_i, e_hi.q_i) (q_i the unit vector at 2 phi_i),
+//     widened by delta = 1e-5 (tr_max / rho_min + 1) for the float32 rotation's
+//     angle; lambda_max >= (tr_min + rho_min) / 2 (rho_min the smallest
+//     projection); hence alpha_i <= cap_i = wmax_i |v_i|^2 (1 + cos_max) / 2 /
+//     lambda_max_min, and sum alpha_i^2 is at most the greedy fill of the
+//     largest caps up to a total of 1.  t_lb = (R2min C - slack) / (N_hi (1 +
+//     1e-3) + 1e-9), or / (1 + 2e-5) without N_hi (ni <= 1).
+//   upper: lambda_min(M(w)) <= u^T M(w) u for any unit u, linear in w, so
+//     lambda_min(M_j) <= R2max min(B1, B2) with B1 = min over 32 sampled u of
+//     max_i (u.v_i)^2 (any weights) and B2 = max_v u^T M(w_v) u for u the minor
+//     eigenvector of the box centre's Gram matrix; with ni >= 1/n, t_ub = n (1 +
+//     2e-5) (R2max min(B1, B2) + slack).  Used only when fp is bounded away from
+//     0 (fp^2 >= 1e-3), which also makes every pixel count in NN.
 struct RegBounds {
   double t_lb;  // >= 0
   double t_ub;  // < 0: no upper bound
 };
 
+// cos and sin of q pi / 32, q < 32: the sampled unit vectors of the upper bound
+__constant__ double kUnitCS[32][2] = {
+    {1.0, 0.0},
+    {0.9951847266721969, 0.0980171403295606},
+    {0.9807852804032304, 0.19509032201612825},
+    {0.9569403357322088, 0.29028467725446233},
+    {0.9238795325112867, 0.3826834323650898},
+    {0.881921264348355, 0.47139673682599764},
+    {0.8314696123025452, 0.5555702330196022},
+    {0.773010453362737, 0.6343932841636455},
+    {0.7071067811865476, 0.7071067811865475},
+    {0.6343932841636455, 0.773010453362737},
+    {0.5555702330196023, 0.8314696123025452},
+    {0.4713967368259978, 0.8819212643483549},
+    {0.38268343236508984, 0.9238795325112867},
+    {0.29028467725446233, 0.9569403357322089},
+    {0.19509032201612833, 0.9807852804032304},
+    {0.09801714032956077, 0.9951847266721968},
+    {6.123233995736766e-17, 1.0},
+    {-0.09801714032956065, 0.9951847266721969},
+    {-0.1950903220161282, 0.9807852804032304},
+    {-0.29028467725446216, 0.9569403357322089},
+    {-0.3826834323650897, 0.9238795325112867},
+    {-0.4713967368259977, 0.881921264348355},
+    {-0.555570233019602, 0.8314696123025455},
+    {-0.6343932841636454, 0.7730104533627371},
+    {-0.7071067811865475, 0.7071067811865476},
+    {-0.773010453362737, 0.6343932841636455},
+    {-0.8314696123025453, 0.5555702330196022},
+    {-0.8819212643483549, 0.47139673682599786},
+    {-0.9238795325112867, 0.3826834323650899},
+    {-0.9569403357322088, 0.2902846772544624},
+    {-0.9807852804032304, 0.1950903220161286},
+    {-0.9951847266721968, 0.09801714032956083},
+};
+
 template <int N>
 __device__ __forceinline__ RegBounds regulator_bounds(const float (&fp0)[N], const float (&fx0)[N],
                                                       const double* __restrict__ ws) {
+  // ws is the same for every lane of the wavefront: its loads are scalar
   const double R2min = ws[0], R2max = ws[1];
-  double a[N], b[N], v2max = 0.0, sw = 0.0, fpw = 0.0;
+  const double* lo = ws + kWMin;
+  const double* hi = ws + kWMax;
+  double v2max = 0.0, sw = 0.0, fpw = 0.0, cs = 0.0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    a[i] = fp0[i];
-    b[i] = fx0[i];
-    const double v2 = a[i] * a[i] + b[i] * b[i];
+    const double a = fp0[i], b = fx0[i];
+    const double v2 = a * a + b * b;
     v2max = v2 > v2max ? v2 : v2max;
-    sw += ws[2 + i];
-    fpw += ws[2 + i] * v2;
+    sw += lo[i];
+    fpw += lo[i] * v2;
+    cs += lo[i] + hi[i];
   }
   const double slack = 1e-4 * R2max * v2max;
-  double C = __builtin_inf();
+  // reference point: the box centre scaled to the simplex; its Gram matrix
+  double m11 = 0.0, m12 = 0.0, m22 = 0.0;
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    double g11 = 0.0, g12 = 0.0, g22 = 0.0;
+  for (int i = 0; i < N; ++i) {
+    const double a = fp0[i], b = fx0[i];
+    const double w = (lo[i] + hi[i]) / cs;
+    m11 += w * (a * a);
+    m12 += w * (a * b);
+    m22 += w * (b * b);
+  }
+  double dx = m11 - m22, dy = 2.0 * m12;
+  const double dn = __builtin_sqrt(dx * dx + dy * dy);
+  dx = dn > 0.0 ? dx / dn : 1.0;
+  dy = dn > 0.0 ? dy / dn : 0.0;
+  // minor eigenvector of the centre's Gram matrix (any unit vector is valid)
+  double u1 = 1.0, u2 = 0.0;
+  {
+    const double h = 0.5 * (m11 + m22), disc = h * h - (m11 * m22 - m12 * m12);
+    const double lmin = h - __builtin_sqrt(disc > 0.0 ? disc : 0.0);
+    const double p1 = m12, p2 = lmin - m11, q1 = lmin - m22, q2 = m12;
+    const double np = p1 * p1 + p2 * p2, nq = q1 * q1 + q2 * q2;
+    if (np >= nq && np > 0.0) {
+      const double r = __builtin_sqrt(np);
+      u1 = p1 / r;
+      u2 = p2 / r;
+    } else if (nq > 0.0) {
+      const double r = __builtin_sqrt(nq);
+      u1 = q1 / r;
+      u2 = q2 / r;
+    }
+  }
+  // vertices: v < N 2^(N-1) for up to four detectors (coordinate k = v >>
+  // (N - 1) from the sum, the others at lo or hi by the bits of v), then, if
+  // none was feasible or for more detectors, the N simplex vertices (v >= nbox)
+  const int nbox = N <= 4 ? N << (N - 1) : 0;
+  double C = __builtin_inf(), Bq = 0.0, rho = __builtin_inf();
+  double trmin = __builtin_inf(), trmax = 0.0;
+  double tlo = __builtin_inf(), thi = -__builtin_inf();
+  double xlo = 0.0, ylo = 0.0, xhi = 0.0, yhi = 0.0;
+  int count = 0;
+#pragma unroll 1
+  for (int v = 0; v < nbox + N; ++v) {
+    if (v == nbox && count > 0) break;
+    const bool box = v < nbox;
+    const int k = box ? v >> (N - 1) : v - nbox;
+    const int bits = v & ((1 << (N - 1)) - 1);
+    double rest = 0.0, s11 = 0.0, s12 = 0.0, s22 = 0.0, k11 = 0.0, k12 = 0.0, k22 = 0.0;
+    double lok = 0.0, hik = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      const double w = i == k ? 1.0 - (sw - ws[2 + k]) : ws[2 + i];
-      g11 += w * a[i] * a[i];
-      g12 += w * a[i] * b[i];
-      g22 += w * b[i] * b[i];
+      const double a = fp0[i], b = fx0[i];
+      if (i == k) {
+        k11 = a * a;
+        k12 = a * b;
+        k22 = b * b;
+        lok = lo[i];
+        hik = hi[i];
+      } else {
+        const int bit = i < k ? i : i - 1;
+        const double w = box && ((bits >> bit) & 1) ? hi[i] : lo[i];
+        rest += w;
+        s11 += w * (a * a);
+        s12 += w * (a * b);
+        s22 += w * (b * b);
+      }
     }
+    const double wk = 1.0 - rest;
+    if (box && !(wk >= lok - 1e-12 && wk <= hik + 1e-12)) continue;
+    const double g11 = s11 + wk * k11, g12 = s12 + wk * k12, g22 = s22 + wk * k22;
     const double h = 0.5 * (g11 + g22), det = g11 * g22 - g12 * g12;
     const double disc = h * h - det;
     const double lm = h - __builtin_sqrt(disc > 0.0 ? disc : 0.0);
     C = lm < C ? lm : C;
+    const double q = u1 * u1 * g11 + 2.0 * u1 * u2 * g12 + u2 * u2 * g22;
+    Bq = q > Bq ? q : Bq;
+    const double x = g11 - g22, y = 2.0 * g12, tr = g11 + g22;
+    trmin = tr < trmin ? tr : trmin;
+    trmax = tr > trmax ? tr : trmax;
+    const double pr = dx * x + dy * y;
+    rho = pr < rho ? pr : rho;
+    if (pr > 0.0) {
+      const double t = (dx * y - dy * x) / pr;
+      if (t < tlo) {
+        tlo = t;
+        xlo = x;
+        ylo = y;
+      }
+      if (t > thi) {
+        thi = t;
+        xhi = x;
+        yhi = y;
+      }
+    }
+    ++count;
+  }
+  // upper bound on ni from the cone of dominant directions
+  double ni_hi = 1.0;
+  const bool cone = rho > 1e-3 * trmax && thi >= tlo;
+  const double delta = cone ? 1e-5 * (trmax / rho + 1.0) : 1.0;
+  if (cone && delta < 0.1) {
+    const double nlo = __builtin_sqrt(xlo * xlo + ylo * ylo);
+    const double nhi = __builtin_sqrt(xhi * xhi + yhi * yhi);
+    const double elx = xlo / nlo, ely = ylo / nlo, ehx = xhi / nhi, ehy = yhi / nhi;
+    const double lam = 0.5 * (trmin + rho);
+    double cap[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double a = fp0[i], b = fx0[i];
+      const double v2 = a * a + b * b;
+      double mc = 1.0;
+      if (v2 > 0.0) {
+        const double qx = (a * a - b * b) / v2, qy = 2.0 * (a * b) / v2;
+        const bool inside = elx * qy - ely * qx >= 0.0 && qx * ehy - qy * ehx >= 0.0;
+        if (!inside) {
+          const double c1 = elx * qx + ely * qy, c2 = ehx * qx + ehy * qy;
+          mc = (c1 > c2 ? c1 : c2) + delta;
+          mc = mc < 1.0 ? mc : 1.0;
+        }
+      }
+      cap[i] = hi[i] * v2 * 0.5 * (1.0 + mc) / lam * (1.0 + 1e-5);
+    }
+    // fill the largest caps first (ties in index order) up to a total of 1:
+    // cap i takes min(cap_i, 1 - the caps ahead of it), at least 0
+    double tot = 0.0, all = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double ahead = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        if (k != i && (cap[k] > cap[i] || (cap[k] == cap[i] && k < i))) ahead += cap[k];
+      double take = 1.0 - ahead;
+      take = take < cap[i] ? take : cap[i];
+      take = take > 0.0 ? take : 0.0;
+      tot += take * take;
+      all += cap[i];
+    }
+    if (tot == tot && all >= 1.0) ni_hi = tot < 1.0 ? tot : 1.0;
   }
   RegBounds r;
-  r.t_lb = (R2min * C - slack) / (1.0 + 2e-5);
+  double den = ni_hi * (1.0 + 1e-3) + 1e-9;
+  den = den < 1.0 + 2e-5 ? den : 1.0 + 2e-5;
+  r.t_lb = (R2min * C - slack) / den;
   if (!(r.t_lb > 0.0)) r.t_lb = 0.0;
   r.t_ub = -1.0;
   const double fp_lb = 0.5 * R2min * fpw * (1.0 - 1e-5);
   if (fp_lb * fp_lb >= 1e-3) {
-    double B = __builtin_inf();
+    double B = Bq;
+#pragma unroll 1
     for (int q = 0; q < 32; ++q) {
-      double su, cu;
-      sincos(q * (3.14159265358979323846 / 32.0), &su, &cu);
+      const double cu = kUnitCS[q][0], su = kUnitCS[q][1];
       double mx = 0.0;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const double p = cu * a[i] + su * b[i];
+        const double p = cu * (double)fp0[i] + su * (double)fx0[i];
         mx = p * p > mx ? p * p : mx;
       }
       B = mx < B ? mx : B;
@@ -625,6 +820,25 @@ __global__ __launch_bounds__(BLOCK) void k_regulator_plain(
   block_count<BLOCK>(v, count);
 }
 
+template <int N>
+__global__ __launch_bounds__(256) void k_term_bounds(const float* __restrict__ FP,
+                                                     const float* __restrict__ FX, int64_t n_dirs,
+                                                     const double* __restrict__ ws,
+                                                     double* __restrict__ t_lb,
+                                                     double* __restrict__ t_ub) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n_dirs) return;
+  float fp0[N], fx0[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    fp0[i] = FP[s * N + i];
+    fx0[i] = FX[s * N + i];
+  }
+  const RegBounds bd = regulator_bounds<N>(fp0, fx0, ws);
+  t_lb[s] = bd.t_lb;
+  t_ub[s] = bd.t_ub;
+}
+
 __global__ __launch_bounds__(256) void k_weight_stats_plain(const float* __restrict__ rms, int P,
                                                             int n, double* __restrict__ out) {
   weight_stats_block<256>(rms, P, n, out);
@@ -753,10 +967,10 @@ __global__ __launch_bounds__(256) void k_batch_weights(BatchDev b) {
 constexpr int kTicketsPerGroup = PYCWB_XCD_QUEUES ? 16 : 2;  // per phase: 8 queues or 1
 
 #ifndef PYCWB_OCC_REG_SMALL
-#define PYCWB_OCC_REG_SMALL 5
+#define PYCWB_OCC_REG_SMALL 4
 #endif
 #ifndef PYCWB_OCC_REG_LARGE
-#define PYCWB_OCC_REG_LARGE 5
+#define PYCWB_OCC_REG_LARGE 4
 #endif
 #ifndef PYCWB_OCC_SCAN_SMALL
 #define PYCWB_OCC_SCAN_SMALL 4
@@ -1017,7 +1231,8 @@ __global__ __launch_bounds__(BLOCK, (PhaseOccupancy<PHASE, N>::waves)) void k_ba
     for (;;) {
       if (threadIdx.x == 0) s_item = atomicAdd(ticket + q, 1u);
       __syncthreads();
-      const unsigned item = s_item;
+      // the same on every lane: scalar from here on
+      const unsigned item = __builtin_amdgcn_readfirstlane(s_item);
       __syncthreads();
       if (item >= n_items) break;
       const unsigned tile = (unsigned)q * per + item % per;
@@ -1031,7 +1246,7 @@ __global__ __launch_bounds__(BLOCK, (PhaseOccupancy<PHASE, N>::waves)) void k_ba
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
     __syncthreads();
-    const unsigned item = s_item;
+    const unsigned item = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
     if (item >= n_items) break;  // every block reaches this once the tickets run out
     run_item<N, BLOCK, PHASE>(b, b.order[first + (int)(item / tiles)], (int)(item % tiles));
@@ -1189,6 +1404,31 @@ int pycwb_td_relayout(int n_ifo, int n_pix, int n_delay, const float* td00, cons
   const int64_t total = (int64_t)n_ifo * n_pix * n_delay;
   hipLaunchKernelGGL(k_relayout_td, dim3(grid_for(total, 65536)), dim3(256), 0, st, td00, td90,
                      n_ifo, n_pix, n_delay, w.td, (unsigned*)w.best, 4);
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
+int pycwb_regulator_term_bounds(int n_ifo, int n_pix, const float* rms, const float* FP,
+                                const float* FX, int64_t n_dirs, double* weight_stats,
+                                double* t_lb, double* t_ub, void* stream) {
+  clear_stale_error();
+  if (n_ifo < 1 || n_ifo > kMaxIfo || n_pix < 1 || n_dirs < 1) return PYCWB_ERR_ARG;
+  if (!rms || !FP || !FX || !weight_stats || !t_lb || !t_ub) return PYCWB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_weight_stats_plain, dim3(1), dim3(256), 0, st, rms, n_pix, n_ifo,
+                     weight_stats);
+  HIP_TRY(hipGetLastError());
+  const unsigned grid = (unsigned)((n_dirs + 255) / 256);
+  switch (n_ifo) {
+#define CASE(NN)                                                                         \
+  case NN:                                                                               \
+    hipLaunchKernelGGL((k_term_bounds<NN>), dim3(grid), dim3(256), 0, st, FP, FX, n_dirs, \
+                       weight_stats, t_lb, t_ub);                                        \
+    break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: return PYCWB_ERR_ARG;
+  }
   HIP_TRY(hipGetLastError());
   return PYCWB_OK;
 }

@@ -2,18 +2,23 @@
 
 The batched and plain regulator kernels decide a direction's DPF-quality count
 early from per-direction bounds on every pixel's term t_j = fx_j / (ni_j + 1e-9)
-(skyscan.hip: ``regulator_bounds``): with the cluster's noise-weight statistics
-(R2 = sum_i r_i^2 per pixel, its range, the per-detector minimum of
-r_i^2 / R2), lower bound R2min times the smallest lambda_min over the weight
-simplex's vertices, upper bound n R2max min_u max_i (u.v_i)^2 over 32 sampled
-unit vectors (used only when fp is bounded away from zero), each widened by
-the margin 1e-4 R2max max|v|^2 for the float32 pipeline's deviation from the
-exact eigenvalue.  The count is exact only if every float32 term the
-reference computes (dpf.py:342-453, restated by the oracle) lies inside those
-bounds.  Here that is checked for every direction of HEALPix order 7 and every
-pixel of clusters built to strain it: noise weights spread over four decades,
-one detector dominating (a near rank-one Gram matrix, fx near 0), noise rms
-spanning thirty decades across pixels, and three to five detectors.
+(skyscan.hip: ``regulator_bounds``), computed from the cluster's noise-weight
+statistics (R2 = sum_i r_i^2 per pixel, its range, the per-detector range of
+r_i^2 / R2): the lower bound is R2min times the smallest lambda_min over the
+vertices of the weight polytope, divided by an upper bound on ni from the cone
+of dominant-frame directions those vertices span; the upper bound is n R2max
+times the smaller of min_u max_i (u.v_i)^2 over 32 sampled unit vectors and
+the largest u^T M u over the vertices for the box centre's minor eigenvector u
+(used only when fp is bounded away from zero); both widened by the margin
+1e-4 R2max max|v|^2 for the float32 pipeline's deviation from the exact
+eigenvalue.  ``bounds`` below follows the kernel step by step.  The count is
+exact only if every float32 term the reference computes (dpf.py:342-453,
+restated by the oracle) lies inside those bounds, and the ni bound holds for
+the float32 ni.  Here that is checked for every direction of HEALPix order 7
+and every pixel of clusters built to strain it: noise weights spread over four
+decades, one detector dominating (a near rank-one Gram matrix, fx near 0),
+noise rms spanning thirty decades across pixels, weights nearly equal (a
+narrow cone, where the ni bound is tightest), and three to five detectors.
 """
 
 import numpy as np
@@ -23,39 +28,113 @@ from pycwb_amd import synth
 
 
 def weight_stats(rms):
-    """[R2min, R2max, min_j r_ji^2 / R2_j per detector] in float64."""
+    """[R2min, R2max, min_j r_ji^2 / R2_j, max_j r_ji^2 / R2_j per detector] in float64."""
     r2 = rms.astype(np.float64) ** 2
     R2 = r2.sum(1)
-    return R2.min(), R2.max(), (r2 / R2[:, None]).min(0)
+    W = r2 / R2[:, None]
+    return R2.min(), R2.max(), W.min(0), W.max(0)
+
+
+def _vertices(lo, hi):
+    """The kernel's vertex list: the vertices of {lo <= w <= hi, sum w = 1} for up
+    to four detectors (n - 1 coordinates at a bound, the last from the sum),
+    else (or if none) the n vertices of {w >= lo, sum w = 1}."""
+    n = lo.size
+    out = []
+    if n <= 4:
+        for k in range(n):
+            others = [i for i in range(n) if i != k]
+            for bits in range(1 << (n - 1)):
+                w = np.empty(n)
+                rest = 0.0
+                for bit, i in enumerate(others):
+                    w[i] = hi[i] if (bits >> bit) & 1 else lo[i]
+                    rest += w[i]
+                w[k] = 1.0 - rest
+                if lo[k] - 1e-12 <= w[k] <= hi[k] + 1e-12:
+                    out.append(w)
+    if not out:
+        sw = lo.sum()
+        for k in range(n):
+            w = lo.copy()
+            w[k] = 1.0 - (sw - lo[k])
+            out.append(w)
+    return np.array(out)
 
 
 def bounds(FP_t, FX_t, rms):
     """The kernel's (t_lb, t_ub) per direction, float64 (t_ub < 0: none)."""
-    R2min, R2max, wmin = weight_stats(rms)
+    R2min, R2max, lo, hi = weight_stats(rms)
     a, b = FP_t.astype(np.float64), FX_t.astype(np.float64)
-    n = a.shape[1]
-    v2 = a * a + b * b
+    S, n = a.shape
+    A2, B2, AB = a * a, b * b, a * b
+    v2 = A2 + B2
     v2max = v2.max(1)
-    sw = wmin.sum()
-    fpw = (wmin[None, :] * v2).sum(1)
     slack = 1e-4 * R2max * v2max
-    C = np.full(a.shape[0], np.inf)
-    for k in range(n):
-        w = wmin.copy()
-        w[k] = 1.0 - (sw - wmin[k])
-        g11 = (w * a * a).sum(1)
-        g12 = (w * a * b).sum(1)
-        g22 = (w * b * b).sum(1)
-        h = 0.5 * (g11 + g22)
-        disc = h * h - (g11 * g22 - g12 * g12)
-        C = np.minimum(C, h - np.sqrt(np.maximum(disc, 0.0)))
-    t_lb = np.maximum((R2min * C - slack) / (1.0 + 2e-5), 0.0)
+    # reference: the box centre scaled to the simplex
+    wc = (lo + hi) / (lo + hi).sum()
+    m11, m12, m22 = A2 @ wc, AB @ wc, B2 @ wc
+    dx, dy = m11 - m22, 2 * m12
+    dn = np.hypot(dx, dy)
+    dx, dy = np.where(dn > 0, dx / np.where(dn > 0, dn, 1), 1.0), np.where(dn > 0, dy / np.where(dn > 0, dn, 1), 0.0)
+    h = 0.5 * (m11 + m22)
+    lmin = h - np.sqrt(np.maximum(h * h - (m11 * m22 - m12 * m12), 0))
+    p1, p2, q1, q2 = m12, lmin - m11, lmin - m22, m12
+    np_, nq = p1 * p1 + p2 * p2, q1 * q1 + q2 * q2
+    usep = (np_ >= nq) & (np_ > 0)
+    useq = ~usep & (nq > 0)
+    u1 = np.where(usep, p1 / np.sqrt(np.where(usep, np_, 1)), np.where(useq, q1 / np.sqrt(np.where(useq, nq, 1)), 1.0))
+    u2 = np.where(usep, p2 / np.sqrt(np.where(usep, np_, 1)), np.where(useq, q2 / np.sqrt(np.where(useq, nq, 1)), 0.0))
+    V = _vertices(lo, hi)
+    g11, g12, g22 = A2 @ V.T, AB @ V.T, B2 @ V.T          # (S, nV)
+    hh = 0.5 * (g11 + g22)
+    C = (hh - np.sqrt(np.maximum(hh * hh - (g11 * g22 - g12 * g12), 0))).min(1)
+    Bq = np.maximum((u1 * u1)[:, None] * g11 + (2 * u1 * u2)[:, None] * g12 +
+                    (u2 * u2)[:, None] * g22, 0).max(1)
+    x, y, tr = g11 - g22, 2 * g12, g11 + g22
+    pr = dx[:, None] * x + dy[:, None] * y
+    rho = pr.min(1)
+    trmin, trmax = tr.min(1), tr.max(1)
+    tan = np.where(pr > 0, (dx[:, None] * y - dy[:, None] * x) / np.where(pr > 0, pr, 1), np.nan)
+    cone = (rho > 1e-3 * trmax) & np.all(pr > 0, 1)
+    kl = np.nanargmin(np.where(cone[:, None], tan, 0), 1)
+    kh = np.nanargmax(np.where(cone[:, None], tan, 0), 1)
+    idx = np.arange(S)
+    el = np.stack([x[idx, kl], y[idx, kl]], 1)
+    eh = np.stack([x[idx, kh], y[idx, kh]], 1)
+    el /= np.maximum(np.linalg.norm(el, axis=1, keepdims=True), 1e-300)
+    eh /= np.maximum(np.linalg.norm(eh, axis=1, keepdims=True), 1e-300)
+    delta = np.where(cone, 1e-5 * (trmax / np.where(cone, rho, 1) + 1), 1.0)
+    ok = cone & (delta < 0.1)
+    sv = np.where(v2 > 0, v2, 1)
+    qx, qy = (A2 - B2) / sv, 2 * AB / sv
+    inside = (el[:, 0:1] * qy - el[:, 1:2] * qx >= 0) & (qx * eh[:, 1:2] - qy * eh[:, 0:1] >= 0)
+    mc = np.minimum(np.maximum(el[:, 0:1] * qx + el[:, 1:2] * qy,
+                               eh[:, 0:1] * qx + eh[:, 1:2] * qy) + delta[:, None], 1.0)
+    mc = np.where(inside | (v2 <= 0), 1.0, mc)
+    lam = 0.5 * (trmin + rho)
+    cap = hi[None, :] * v2 * 0.5 * (1 + mc) / np.where(ok, lam, 1)[:, None] * (1 + 1e-5)
+    # greedy: largest caps first (ties in index order)
+    tot = np.zeros(S)
+    for i in range(n):
+        ahead = np.zeros(S)
+        for k in range(n):
+            if k == i:
+                continue
+            sel = (cap[:, k] > cap[:, i]) | ((cap[:, k] == cap[:, i]) & (k < i))
+            ahead += np.where(sel, cap[:, k], 0.0)
+        take = np.clip(np.minimum(1.0 - ahead, cap[:, i]), 0, None)
+        tot += take * take
+    ni_hi = np.where(ok & (cap.sum(1) >= 1.0), np.minimum(tot, 1.0), 1.0)
+    den = np.minimum(ni_hi * (1 + 1e-3) + 1e-9, 1 + 2e-5)
+    t_lb = np.maximum((R2min * C - slack) / den, 0.0)
+    fpw = (lo[None, :] * v2).sum(1)
     fp_lb = 0.5 * R2min * fpw * (1.0 - 1e-5)
     q = np.arange(32) * (np.pi / 32.0)
     proj = np.cos(q)[None, :, None] * a[:, None, :] + np.sin(q)[None, :, None] * b[:, None, :]
-    B = (proj ** 2).max(2).min(1)
+    B = np.minimum((proj ** 2).max(2).min(1), Bq)
     t_ub = np.where(fp_lb * fp_lb >= 1e-3, n * (1.0 + 2e-5) * (R2max * B + slack), -1.0)
-    return t_lb, t_ub
+    return t_lb, t_ub, ni_hi
 
 
 def _rms(kind, P, n, seed):
@@ -69,6 +148,10 @@ def _rms(kind, P, n, seed):
     elif kind == "extreme":        # noise rms over thirty decades across pixels
         sigma = np.exp(rng.uniform(np.log(1e-15), np.log(1e15), size=(P, 1))) * \
             np.exp(rng.uniform(-0.3, 0.3, (P, n)))
+    elif kind == "bench":          # the benchmark's noise: |N(1, 0.1)| per detector
+        sigma = np.abs(rng.normal(1.0, 0.1, size=(P, n)))
+    elif kind == "narrow":         # nearly equal weights: the narrowest cone of directions
+        sigma = 1.0 + 1e-4 * rng.standard_normal((P, n))
     else:
         raise ValueError(kind)
     inv = 1.0 / sigma
@@ -79,6 +162,9 @@ def _rms(kind, P, n, seed):
     (synth.NETWORK_3, "spread", 1), (synth.NETWORK_4, "spread", 2),
     (synth.NETWORK_3, "dominant", 3), (synth.NETWORK_5, "dominant", 4),
     (synth.NETWORK_4, "extreme", 5), (synth.NETWORK_5, "spread", 6),
+    (synth.NETWORK_3, "bench", 7), (synth.NETWORK_4, "bench", 8),
+    (synth.NETWORK_3, "narrow", 9), (synth.NETWORK_4, "narrow", 10),
+    (synth.NETWORK_5, "narrow", 11),
 ])
 def test_float32_terms_lie_inside_the_bounds(oracle, ifos, kind, seed):
     net = synth.make_network(ifos, order=7, K=8)
@@ -88,7 +174,11 @@ def test_float32_terms_lie_inside_the_bounds(oracle, ifos, kind, seed):
     fp, fx, ni = oracle.dpf_terms(net.FP_t, net.FX_t, rms, dirs)
     t = fx / (ni + np.float32(1e-9))                     # the float32 term of dpf.py:450
     assert t.dtype == np.float32 and np.all(np.isfinite(t))
-    t_lb, t_ub = bounds(net.FP_t, net.FX_t, rms)
+    t_lb, t_ub, ni_hi = bounds(net.FP_t, net.FX_t, rms)
+    ni64 = ni.astype(np.float64)
+    assert np.all(ni64 <= ni_hi[:, None] * (1 + 1e-3)), float((ni64 / ni_hi[:, None]).max())
+    if kind in ("bench", "narrow"):
+        assert np.mean(ni_hi < 1.0) > 0.1     # the ni bound takes part
     t64 = t.astype(np.float64)
     low = t64 < t_lb[:, None]
     assert not low.any(), (kind, int(low.sum()), float((t_lb[:, None] - t64)[low].max()))
@@ -107,3 +197,51 @@ def test_float32_terms_lie_inside_the_bounds(oracle, ifos, kind, seed):
     lam = np.linalg.eigvalsh(G)                          # (S, P, 2)
     dev = np.abs(fx.astype(np.float64) - lam[..., 0])
     assert np.all(dev <= slack[:, None]), float((dev / slack[:, None]).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ifos,kind,seed", [
+    (synth.NETWORK_3, "bench", 21), (synth.NETWORK_4, "bench", 22),
+    (synth.NETWORK_3, "narrow", 23), (synth.NETWORK_5, "spread", 24),
+    (synth.NETWORK_4, "dominant", 25),
+])
+def test_kernel_bounds_follow_the_restatement(oracle, ifos, kind, seed):
+    """The HIP kernel's weight statistics and term bounds (through the
+    diagnostic entry point) against ``bounds`` above, and against the float32
+    terms themselves."""
+    import ctypes as C
+
+    import torch
+
+    from pycwb_amd import _lib
+    from pycwb_amd.device import lib, ptr
+
+    print(f"case kernel term bounds {len(ifos)} detectors, {kind}", flush=True)
+    dev = torch.device("cuda", 0)
+    net = synth.make_network(ifos, order=5, K=8)
+    n, S, P = net.n_ifo, net.n_sky, 96
+    rms = _rms(kind, P, n, seed)
+    d_rms = torch.from_numpy(rms).to(dev)
+    d_fp = torch.from_numpy(np.ascontiguousarray(net.FP_t, np.float32)).to(dev)
+    d_fx = torch.from_numpy(np.ascontiguousarray(net.FX_t, np.float32)).to(dev)
+    ws = torch.zeros(24, dtype=torch.float64, device=dev)
+    lb = torch.empty(S, dtype=torch.float64, device=dev)
+    ub = torch.empty(S, dtype=torch.float64, device=dev)
+    _lib.check(lib().pycwb_regulator_term_bounds(n, P, ptr(d_rms), ptr(d_fp), ptr(d_fx), S,
+                                                  ptr(ws), ptr(lb), ptr(ub), None),
+               "pycwb_regulator_term_bounds")
+    torch.cuda.synchronize()
+    ws, lb, ub = ws.cpu().numpy(), lb.cpu().numpy(), ub.cpu().numpy()
+    R2min, R2max, lo, hi = weight_stats(rms)
+    np.testing.assert_allclose(ws[:2], [R2min, R2max], rtol=1e-15)
+    np.testing.assert_allclose(ws[2:2 + n], lo, rtol=1e-15)
+    np.testing.assert_allclose(ws[10:10 + n], hi, rtol=1e-15)
+    t_lb, t_ub, _ = bounds(net.FP_t, net.FX_t, rms)
+    scale = (net.FP_t.astype(np.float64) ** 2 + net.FX_t.astype(np.float64) ** 2).max(1)
+    np.testing.assert_allclose(lb, t_lb, rtol=1e-9, atol=1e-12 * scale.max())
+    assert np.array_equal(ub >= 0, t_ub >= 0)
+    np.testing.assert_allclose(ub[ub >= 0], t_ub[t_ub >= 0], rtol=1e-9)
+    fp, fx, ni = oracle.dpf_terms(net.FP_t, net.FX_t, rms, np.arange(S, dtype=np.int64))
+    t = (fx / (ni + np.float32(1e-9))).astype(np.float64)
+    assert np.all(t >= lb[:, None])
+    assert np.all(t[ub >= 0] <= ub[ub >= 0, None])
