@@ -1094,6 +1094,8 @@ def main():
     ap.add_argument("--no-cluster", action="store_true")
     ap.add_argument("--no-threshold", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
+    ap.add_argument("--no-pixel-classes", action="store_true",
+                    help="regulator sweep with cluster-wide term bounds only")
     ap.add_argument("--only", choices=("refine", "wdm", "tdamp", "subnet", "xtalk", "select",
                                        "maxenergy", "cluster", "threshold", "e2e"), default=None,
                     help="run one side leg alone and print its JSON (for profiling)")
@@ -1185,7 +1187,8 @@ def main():
     else:
         tuned = autotune.tune_batched(nets, nt, valid, gamma, delta, netCC, log=log)
         block, grid = tuned["block"], tuned["grid"]
-    bs = BatchedSkyScan(batch, nt, valid, gamma, delta, netCC, block=block, grid=grid)
+    bs = BatchedSkyScan(batch, nt, valid, gamma, delta, netCC, block=block, grid=grid,
+                        pixel_classes=not args.no_pixel_classes)
     stats = SkyStatistics(batch, nt)   # self-only cross-talk
     reg0 = float(np.float32(delta * np.sqrt(2)))
     log(f"batched: block={bs.block} grid={bs.grid} ({tuned.get('source')})")

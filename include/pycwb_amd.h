@@ -99,6 +99,19 @@ int pycwb_regulator_term_bounds(int n_ifo, int n_pix, const float* rms, const fl
                                 const float* FX, int64_t n_dirs, double* weight_stats,
                                 double* t_lb, double* t_ub, void* stream);
 
+/* Diagnostic without a counterpart in the reference: the pixel classes of
+ * one cluster (pixel_class, n_pix bytes; class_stats, 16 x 24 doubles: per
+ * class the weight statistics in the layout of weight_stats, the pixel count
+ * in slot 18 and, in class 0, the number of classes in slot 19; class_rem,
+ * pycwb_class_rem_entries(n_pix) uint16: row q, class g = pixels j >= 16 q
+ * of class g) and, for each of n_dirs directions, every class's term bounds
+ * (t_lb, t_ub: n_dirs x 16; 0 and -1 for unused classes), as the batched
+ * regulator's sweep uses them. */
+int pycwb_regulator_class_bounds(int n_ifo, int n_pix, const float* rms, const float* FP,
+                                 const float* FX, int64_t n_dirs, double* weight_stats,
+                                 uint8_t* pixel_class, double* class_stats, uint16_t* class_rem,
+                                 double* t_lb, double* t_ub, void* stream);
+
 /* Sky scan of one cluster.  Replaces scan_sky_for_best_fit
  * (pycwb/modules/likelihoodWP/sky_scan.py:17-192; called at likelihood.py:319).
  * Reads the amplitude pairs from td_packed ((n_pix, n_ifo, n_delay) float
@@ -173,7 +186,28 @@ typedef struct pycwb_batch_desc {
   int grid;       /* persistent workgroups */
   int use_graph;  /* capture the launches in a hipGraph */
   double* weight_stats; /* scratch, 24 doubles per cluster, or NULL (no regulator bounds) */
+  /* Pixel classes for the regulator sweep (all five, with weight_stats and
+   * PYCWB_REGULATOR_EARLY_EXIT | PYCWB_REGULATOR_BOUNDS, or NULL: none).
+   * Scratch: pixel_class one byte per noise weight (sum of n_pix[c] times
+   * the cluster's detector count; cluster c's n_pix[c] class bytes at
+   * rms_off[c]), class_stats 384 doubles per cluster, class_rem
+   * pycwb_class_rem_entries(n_pix[c]) uint16 per cluster from
+   * class_rem_off[c] (a multiple of 16; the buffer 32-byte aligned). */
+  uint8_t* pixel_class;
+  double* class_stats;
+  uint16_t* class_rem;
+  const int64_t* class_rem_off;
+  /* class-bound scratch of at least pycwb_class_bound_scratch_bytes(block,
+   * grid) bytes (class_bounds_bytes), 16-byte aligned */
+  uint32_t* class_bounds;
+  size_t class_bounds_bytes;
 } pycwb_batch_desc;
+
+/* uint16 entries of class_rem one cluster of n_pix pixels needs. */
+int64_t pycwb_class_rem_entries(int n_pix);
+
+/* Bytes of class-bound scratch a batch of this launch shape needs. */
+size_t pycwb_class_bound_scratch_bytes(int block, int grid);
 
 typedef struct pycwb_batch pycwb_batch;
 

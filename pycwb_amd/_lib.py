@@ -23,6 +23,7 @@ TILE_ENERGY_BOUND = 4
 REGULATOR_BOUNDS = 8
 DEFAULT_FLAGS = SKIP_MASKED_PIXELS | REGULATOR_EARLY_EXIT | TILE_ENERGY_BOUND | REGULATOR_BOUNDS
 WSTATS_PER_CLUSTER = 24
+CLASS_STATS_PER_CLUSTER = 16 * 24
 TICKETS_PER_GROUP = 16   # uint32 ticket counters per detector group of a batch
 
 _lock = threading.Lock()
@@ -75,6 +76,8 @@ class BatchDesc(C.Structure):
         ("block", i32),
         ("grid", i32),
         ("use_graph", i32), ("weight_stats", vp),
+        ("pixel_class", vp), ("class_stats", vp), ("class_rem", vp), ("class_rem_off", vp),
+        ("class_bounds", vp), ("class_bounds_bytes", sz),
     ]
 
 
@@ -162,6 +165,9 @@ SIGNATURES = {
     "pycwb_last_error": (C.c_char_p, []),
     "pycwb_sky_scan_workspace_bytes": (sz, [i32, i32, i32]),
     "pycwb_td_relayout": (i32, [i32, i32, i32, vp, vp, vp, sz, vp]),
+    "pycwb_class_rem_entries": (i64, [i32]),
+    "pycwb_class_bound_scratch_bytes": (sz, [i32, i32]),
+    "pycwb_regulator_class_bounds": (i32, [i32, i32, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp]),
     "pycwb_regulator_term_bounds": (i32, [i32, i32, vp, vp, vp, i64, vp, vp, vp, vp]),
     "pycwb_dpf_regulator": (i32, [i32, i32, i64, vp, vp, vp, f64, f64, f32, vp, i64, i32, vp,
                                   vp, vp, sz, vp]),

@@ -292,7 +292,7 @@ class BatchedSkyScan:
     def __init__(self, batch: ClusterBatch, nets: NetTablesDev, valid: torch.Tensor,
                  gamma_regulator: float, delta_regulator: float, netCC: float,
                  block: int = 256, grid: int | None = None, use_graph: bool = True,
-                 flags: int | None = None, ordered: bool = True):
+                 flags: int | None = None, ordered: bool = True, pixel_classes: bool = True):
         dev = batch.td.device
         self.batch, self.nets, self.valid = batch, nets, valid
         C_ = batch.n_clusters
@@ -343,6 +343,25 @@ class BatchedSkyScan:
         d.grid = int(grid or default_grid(block))
         d.use_graph = 1 if use_graph else 0
         d.weight_stats = ptr(self.wstats)
+        if pixel_classes:
+            # regulator sweep bounds per class of pixels with similar noise weights
+            ent = np.array([int(lib().pycwb_class_rem_entries(int(p))) for p in
+                            np.unique(batch.n_pix)], np.int64)
+            per = dict(zip(np.unique(batch.n_pix).tolist(), ent.tolist()))
+            n_ent = np.array([per[int(p)] for p in batch.n_pix], np.int64)
+            rem_off = np.concatenate([[0], np.cumsum(n_ent)[:-1]]).astype(np.int64)
+            # one byte per noise weight: cluster c's class bytes start at rms_off[c]
+            self.pclass = torch.empty(max(int(batch.rms.numel()), 1), dtype=torch.uint8,
+                                      device=dev)
+            self.cstats = torch.empty(C_ * _lib.CLASS_STATS_PER_CLUSTER, dtype=torch.float64,
+                                      device=dev)
+            self.crem = torch.empty(max(int(n_ent.sum()), 16), dtype=torch.int16, device=dev)
+            self.crem_off = torch.from_numpy(rem_off).to(dev)
+            d.pixel_class, d.class_stats = ptr(self.pclass), ptr(self.cstats)
+            d.class_rem, d.class_rem_off = ptr(self.crem), ptr(self.crem_off)
+            nb = int(lib().pycwb_class_bound_scratch_bytes(d.block, d.grid))
+            self.cbound = torch.empty(max(nb // 4, 1), dtype=torch.int32, device=dev)
+            d.class_bounds, d.class_bounds_bytes = ptr(self.cbound), nb
         self.desc = d
         self.block, self.grid = d.block, d.grid
         h = C.c_void_p()

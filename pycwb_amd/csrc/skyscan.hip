@@ -254,9 +254,30 @@ __constant__ double kUnitCS[32][2] = {
     {-0.9951847266721968, 0.09801714032956083},
 };
 
+// B1 = min over the 32 sampled unit vectors u of max_i (u.v_i)^2 for one direction
+template <int N>
+__device__ __forceinline__ double sampled_projection_bound(const float (&fp0)[N],
+                                                           const float (&fx0)[N]) {
+  double B = __builtin_inf();
+#pragma unroll 1
+  for (int q = 0; q < 32; ++q) {
+    const double cu = kUnitCS[q][0], su = kUnitCS[q][1];
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double p = cu * (double)fp0[i] + su * (double)fx0[i];
+      mx = p * p > mx ? p * p : mx;
+    }
+    B = mx < B ? mx : B;
+  }
+  return B;
+}
+
+// B1 < 0: computed here (sampled_projection_bound), else the caller's value.
 template <int N>
 __device__ __forceinline__ RegBounds regulator_bounds(const float (&fp0)[N], const float (&fx0)[N],
-                                                      const double* __restrict__ ws) {
+                                                      const double* __restrict__ ws,
+                                                      double B1 = -1.0) {
   // ws is the same for every lane of the wavefront: its loads are scalar
   const double R2min = ws[0], R2max = ws[1];
   const double* lo = ws + kWMin;
@@ -418,22 +439,416 @@ __device__ __forceinline__ RegBounds regulator_bounds(const float (&fp0)[N], con
   r.t_ub = -1.0;
   const double fp_lb = 0.5 * R2min * fpw * (1.0 - 1e-5);
   if (fp_lb * fp_lb >= 1e-3) {
-    double B = Bq;
-#pragma unroll 1
-    for (int q = 0; q < 32; ++q) {
-      const double cu = kUnitCS[q][0], su = kUnitCS[q][1];
-      double mx = 0.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double p = cu * (double)fp0[i] + su * (double)fx0[i];
-        mx = p * p > mx ? p * p : mx;
-      }
-      B = mx < B ? mx : B;
-    }
+    const double b1 = B1 < 0.0 ? sampled_projection_bound<N>(fp0, fx0) : B1;
+    const double B = b1 < Bq ? b1 : Bq;
     r.t_ub = (double)N * (1.0 + 2e-5) * (R2max * B + slack);
   }
   if (!(r.t_ub == r.t_ub)) r.t_ub = -1.0;  // NaN statistics: no upper bound
   return r;
+}
+
+// ---------------------------------------------------------------------------
+// Pixel classes of a cluster for the regulator sweep.  The pixels are split
+// by their normalised noise weights w_ji = r_ji^2 / R2_j into 2^L classes (L =
+// 0 below 64 pixels, 2 below 256, 3 below 1,024, else 4; 0 with NaN or
+// infinite statistics or 65,536 pixels or more): L rounds in which every
+// class splits at the mean of its widest weight coordinate (fixed-point sums,
+// so the classes do not depend on the order of the atomics).  Per class, the
+// weight statistics in the layout of the cluster's (R2 range, per-detector
+// weight minima and maxima, each rounded outwards to float32), the pixel
+// count in slot 18 and, in class 0's slot 19, the number of classes; and a
+// table rem[q][g] (q <= ceil(P / 16), 16 classes per row) of the pixels j >=
+// 16 q in class g.  The sweep bounds the terms of its remaining pixels class
+// by class: a class's weight box is much narrower than the cluster's.
+// ---------------------------------------------------------------------------
+constexpr int kMaxClasses = 16;
+constexpr int kClassStats = kMaxClasses * kWStats;  // doubles per cluster
+constexpr int kClassSlotCount = 18, kClassSlotClasses = 19;
+
+__host__ __device__ inline int class_levels(int P) {
+  return P < 64 ? 0 : P < 256 ? 2 : P < 1024 ? 3 : P < 65536 ? 4 : 0;
+}
+
+__host__ __device__ inline int64_t class_rem_entries(int P) {
+  return ((int64_t)(P + 15) / 16 + 1) * kMaxClasses;
+}
+
+__device__ __forceinline__ double pixel_weight(const float* __restrict__ rms, int64_t j, int n,
+                                               int i) {
+  double R2 = 0.0, ri = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double r = (double)rms[j * n + k];
+    R2 += r * r;
+    if (k == i) ri = r * r;
+  }
+  return ri / R2;
+}
+
+template <int BLK>
+__device__ void pixel_classes_block(const float* __restrict__ rms, int P, int n,
+                                    const double* __restrict__ ws, uint8_t* __restrict__ pclass,
+                                    double* __restrict__ cstats, uint16_t* __restrict__ crem) {
+  __shared__ unsigned s_min[kMaxClasses][8], s_max[kMaxClasses][8];
+  __shared__ unsigned s_r2min[kMaxClasses], s_r2max[kMaxClasses], s_cnt[kMaxClasses];
+  __shared__ unsigned long long s_sum[kMaxClasses];
+  __shared__ int s_axis[kMaxClasses];
+  __shared__ double s_split[kMaxClasses];
+  __shared__ unsigned s_tot[BLK][kMaxClasses];
+  bool finite = __builtin_isfinite(ws[0]) && __builtin_isfinite(ws[1]);
+  for (int i = 0; i < n; ++i)
+    finite = finite && __builtin_isfinite(ws[kWMin + i]) && __builtin_isfinite(ws[kWMax + i]);
+  const int L = finite ? class_levels(P) : 0;
+  const int G = 1 << L;
+  const int t = threadIdx.x;
+  for (int j = t; j < P; j += BLK) pclass[j] = 0;
+  __syncthreads();
+  for (int level = 0; level < L; ++level) {
+    const int cells = 1 << level;
+    for (int e = t; e < cells * 8; e += BLK) {
+      s_min[e >> 3][e & 7] = 0x7f800000u;  // +inf
+      s_max[e >> 3][e & 7] = 0u;
+    }
+    __syncthreads();
+    for (int j = t; j < P; j += BLK) {
+      const int c = pclass[j];
+      for (int i = 0; i < n; ++i) {
+        const unsigned w = __float_as_uint((float)pixel_weight(rms, j, n, i));
+        atomicMin(&s_min[c][i], w);
+        atomicMax(&s_max[c][i], w);
+      }
+    }
+    __syncthreads();
+    if (t < cells) {
+      int ax = 0;
+      float wid = -1.f;
+      for (int i = 0; i < n; ++i) {
+        const float d = __uint_as_float(s_max[t][i]) - __uint_as_float(s_min[t][i]);
+        if (d > wid) {
+          wid = d;
+          ax = i;
+        }
+      }
+      s_axis[t] = ax;
+      s_sum[t] = 0ull;
+      s_cnt[t] = 0u;
+    }
+    __syncthreads();
+    for (int j = t; j < P; j += BLK) {
+      const int c = pclass[j];
+      const double w = pixel_weight(rms, j, n, s_axis[c]);
+      atomicAdd(&s_sum[c], (unsigned long long)(w * 4294967296.0));
+      atomicAdd(&s_cnt[c], 1u);
+    }
+    __syncthreads();
+    if (t < cells)
+      s_split[t] = s_cnt[t] ? (double)s_sum[t] / 4294967296.0 / (double)s_cnt[t] : 0.0;
+    __syncthreads();
+    for (int j = t; j < P; j += BLK) {
+      const int c = pclass[j];
+      const double w = pixel_weight(rms, j, n, s_axis[c]);
+      pclass[j] = (uint8_t)(2 * c + (w > s_split[c] ? 1 : 0));
+    }
+    __syncthreads();
+  }
+  // class statistics, rounded outwards
+  for (int e = t; e < kMaxClasses * 8; e += BLK) {
+    s_min[e >> 3][e & 7] = 0x7f800000u;
+    s_max[e >> 3][e & 7] = 0u;
+  }
+  if (t < kMaxClasses) {
+    s_r2min[t] = 0x7f800000u;
+    s_r2max[t] = 0u;
+    s_cnt[t] = 0u;
+  }
+  __syncthreads();
+  for (int j = t; j < P; j += BLK) {
+    const int c = pclass[j];
+    double r2[8], R2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double r = (double)rms[(int64_t)j * n + i];
+      r2[i] = r * r;
+      R2 += r2[i];
+    }
+    atomicMin(&s_r2min[c], __float_as_uint(__double2float_rd(R2)));
+    atomicMax(&s_r2max[c], __float_as_uint(__double2float_ru(R2)));
+    for (int i = 0; i < n; ++i) {
+      const double w = r2[i] / R2;
+      atomicMin(&s_min[c][i], __float_as_uint(__double2float_rd(w)));
+      atomicMax(&s_max[c][i], __float_as_uint(__double2float_ru(w)));
+    }
+    atomicAdd(&s_cnt[c], 1u);
+  }
+  __syncthreads();
+  for (int e = t; e < kClassStats; e += BLK) {
+    const int g = e / kWStats, f = e % kWStats;
+    double v = 0.0;
+    if (g < G && s_cnt[g]) {
+      if (f == 0) v = __uint_as_float(s_r2min[g]);
+      else if (f == 1) v = __uint_as_float(s_r2max[g]);
+      else if (f >= kWMin && f < kWMin + n) v = __uint_as_float(s_min[g][f - kWMin]);
+      else if (f >= kWMax && f < kWMax + n) v = __uint_as_float(s_max[g][f - kWMax]);
+    }
+    if (f == kClassSlotCount) v = g < G ? (double)s_cnt[g] : 0.0;
+    if (f == kClassSlotClasses && g == 0) v = (double)G;
+    cstats[e] = v;
+  }
+  // rem[q][g]: thread t owns the checkpoint rows [q0, q1)
+  const int nq = (P + 15) / 16;
+  const int per = (nq + BLK - 1) / BLK;
+  const int q0 = t * per < nq ? t * per : nq, q1 = (t + 1) * per < nq ? (t + 1) * per : nq;
+  for (int g = 0; g < kMaxClasses; ++g) s_tot[t][g] = 0u;
+  for (int j = q0 * 16; j < q1 * 16 && j < P; ++j) s_tot[t][pclass[j]] += 1u;
+  __syncthreads();
+  if (t < kMaxClasses) {  // suffix totals over the threads after each
+    unsigned run = 0u;
+    for (int u = BLK - 1; u >= 0; --u) {
+      const unsigned x = s_tot[u][t];
+      s_tot[u][t] = run;
+      run += x;
+    }
+  }
+  __syncthreads();
+  if (G > 1) {
+    if (t == 0)
+      for (int g = 0; g < kMaxClasses; ++g) crem[(int64_t)nq * kMaxClasses + g] = 0;
+    for (int q = q1 - 1; q >= q0; --q) {
+      for (int j = q * 16 + 15; j >= q * 16; --j)
+        if (j < P) s_tot[t][pclass[j]] += 1u;
+      for (int g = 0; g < kMaxClasses; ++g)
+        crem[(int64_t)q * kMaxClasses + g] = (uint16_t)s_tot[t][g];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float half_bits_value(unsigned short b) {
+  return (float)__builtin_bit_cast(_Float16, b);
+}
+
+// ---- class bounds in float32 -------------------------------------------
+// The class bounds follow regulator_bounds step for step in float32 with the
+// hardware's reciprocal and square root (about 1 ulp): every margin of those
+// bounds (1e-4 R2max max|v|^2 on lambda_min, the angle widening 1e-5 (tr /
+// rho + 1), 1e-5 on the caps, 1e-3 on ni, 2e-5 on the upper bound) is at
+// least a hundred times the float32 error of the quantity it covers, so the
+// bounds stay valid; the extreme rays of the cone are found by cross-product
+// comparisons instead of angles.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ unsigned half_down_f(float f) {  // largest half <= f, f >= 0
+  _Float16 h = (_Float16)f;
+  unsigned b = __builtin_bit_cast(unsigned short, h);
+  if ((float)h > f) b -= 1u;
+  return b;
+}
+
+__device__ __forceinline__ unsigned half_up_f(float f) {  // smallest half >= f, f >= 0
+  _Float16 h = (_Float16)f;
+  unsigned b = __builtin_bit_cast(unsigned short, h);
+  if ((float)h < f) b += 1u;
+  return b;
+}
+
+// Per-direction quantities every class's bounds share.
+template <int N>
+struct DirPre {
+  float A2[N], B2[N], AB[N], v2[N], qx[N], qy[N];
+  float v2max, B1;
+};
+
+template <int N>
+__device__ __forceinline__ void dir_pre(const float (&fp0)[N], const float (&fx0)[N],
+                                        DirPre<N>& d) {
+  d.v2max = 0.f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const float a = fp0[i], b = fx0[i];
+    d.A2[i] = a * a;
+    d.B2[i] = b * b;
+    d.AB[i] = a * b;
+    d.v2[i] = d.A2[i] + d.B2[i];
+    d.v2max = d.v2[i] > d.v2max ? d.v2[i] : d.v2max;
+    const float r = d.v2[i] > 0.f ? fast_rcp(d.v2[i]) : 0.f;
+    d.qx[i] = (d.A2[i] - d.B2[i]) * r;
+    d.qy[i] = 2.f * d.AB[i] * r;
+  }
+  float B = __builtin_inff();
+#pragma unroll 1
+  for (int q = 0; q < 32; ++q) {
+    const float cu = (float)kUnitCS[q][0], su = (float)kUnitCS[q][1];
+    float mx = 0.f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const float p = cu * fp0[i] + su * fx0[i];
+      mx = p * p > mx ? p * p : mx;
+    }
+    B = mx < B ? mx : B;
+  }
+  d.B1 = B;
+}
+
+// The (lower, upper) term bounds of one direction over one class (weight
+// statistics cg, the layout of weight_stats) as a half-precision pair: lower
+// rounded down in the low 16 bits, upper rounded up (or +inf: none) above.
+template <int N>
+__device__ __forceinline__ unsigned class_bound_pair(const DirPre<N>& d,
+                                                     const double* __restrict__ cg) {
+  const float R2min = (float)cg[0], R2max = (float)cg[1];
+  float lo[N], hi[N];
+  float sw = 0.f, fpw = 0.f, cs = 0.f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    lo[i] = (float)cg[kWMin + i];
+    hi[i] = (float)cg[kWMax + i];
+    sw += lo[i];
+    fpw += lo[i] * d.v2[i];
+    cs += lo[i] + hi[i];
+  }
+  const float slack = 1e-4f * R2max * d.v2max;
+  const float rcs = fast_rcp(cs);
+  float m11 = 0.f, m12 = 0.f, m22 = 0.f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const float w = (lo[i] + hi[i]) * rcs;
+    m11 += w * d.A2[i];
+    m12 += w * d.AB[i];
+    m22 += w * d.B2[i];
+  }
+  float dx = m11 - m22, dy = 2.f * m12;
+  const float dn2 = dx * dx + dy * dy;
+  if (dn2 > 0.f) {
+    const float r = fast_rcp(fast_sqrt(dn2));
+    dx *= r;
+    dy *= r;
+  } else {
+    dx = 1.f;
+    dy = 0.f;
+  }
+  float u1 = 1.f, u2 = 0.f;
+  {
+    const float h = 0.5f * (m11 + m22), disc = h * h - (m11 * m22 - m12 * m12);
+    const float lmin = h - fast_sqrt(disc > 0.f ? disc : 0.f);
+    const float p1 = m12, p2 = lmin - m11, q1 = lmin - m22, q2 = m12;
+    const float np = p1 * p1 + p2 * p2, nq = q1 * q1 + q2 * q2;
+    if (np >= nq && np > 0.f) {
+      const float r = fast_rcp(fast_sqrt(np));
+      u1 = p1 * r;
+      u2 = p2 * r;
+    } else if (nq > 0.f) {
+      const float r = fast_rcp(fast_sqrt(nq));
+      u1 = q1 * r;
+      u2 = q2 * r;
+    }
+  }
+  const int nbox = N <= 4 ? N << (N - 1) : 0;
+  float C = __builtin_inff(), Bq = 0.f, rho = __builtin_inff();
+  float trmin = __builtin_inff(), trmax = 0.f;
+  float xlo = 0.f, ylo = 0.f, xhi = 0.f, yhi = 0.f;
+  int count = 0;
+#pragma unroll 1
+  for (int v = 0; v < nbox + N; ++v) {
+    if (v == nbox && count > 0) break;
+    const bool box = v < nbox;
+    const int k = box ? v >> (N - 1) : v - nbox;
+    const int bits = v & ((1 << (N - 1)) - 1);
+    float rest = 0.f, s11 = 0.f, s12 = 0.f, s22 = 0.f, k11 = 0.f, k12 = 0.f, k22 = 0.f;
+    float lok = 0.f, hik = 0.f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (i == k) {
+        k11 = d.A2[i];
+        k12 = d.AB[i];
+        k22 = d.B2[i];
+        lok = lo[i];
+        hik = hi[i];
+      } else {
+        const int bit = i < k ? i : i - 1;
+        const float w = box && ((bits >> bit) & 1) ? hi[i] : lo[i];
+        rest += w;
+        s11 += w * d.A2[i];
+        s12 += w * d.AB[i];
+        s22 += w * d.B2[i];
+      }
+    }
+    const float wk = 1.f - rest;
+    if (box && !(wk >= lok - 1e-6f && wk <= hik + 1e-6f)) continue;
+    const float g11 = s11 + wk * k11, g12 = s12 + wk * k12, g22 = s22 + wk * k22;
+    const float h = 0.5f * (g11 + g22), disc = h * h - (g11 * g22 - g12 * g12);
+    const float lm = h - fast_sqrt(disc > 0.f ? disc : 0.f);
+    C = lm < C ? lm : C;
+    const float q = u1 * u1 * g11 + 2.f * u1 * u2 * g12 + u2 * u2 * g22;
+    Bq = q > Bq ? q : Bq;
+    const float x = g11 - g22, y = 2.f * g12, tr = g11 + g22;
+    trmin = tr < trmin ? tr : trmin;
+    trmax = tr > trmax ? tr : trmax;
+    const float pr = dx * x + dy * y;
+    rho = pr < rho ? pr : rho;
+    if (count == 0) {
+      xlo = xhi = x;
+      ylo = yhi = y;
+    } else {
+      if (xlo * y - ylo * x < 0.f) {  // clockwise of the current lower ray
+        xlo = x;
+        ylo = y;
+      }
+      if (xhi * y - yhi * x > 0.f) {  // counter-clockwise of the upper ray
+        xhi = x;
+        yhi = y;
+      }
+    }
+    ++count;
+  }
+  float ni_hi = 1.f;
+  const bool cone = count > 0 && rho > 1e-3f * trmax;
+  const float delta = cone ? 1e-5f * (trmax * fast_rcp(rho) + 1.f) : 1.f;
+  if (cone && delta < 0.1f) {
+    const float rl = fast_rcp(fast_sqrt(xlo * xlo + ylo * ylo));
+    const float rh = fast_rcp(fast_sqrt(xhi * xhi + yhi * yhi));
+    const float elx = xlo * rl, ely = ylo * rl, ehx = xhi * rh, ehy = yhi * rh;
+    const float rlam = fast_rcp(0.5f * (trmin + rho));
+    float cap[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      float mc = 1.f;
+      if (d.v2[i] > 0.f) {
+        const float qx = d.qx[i], qy = d.qy[i];
+        const bool inside = elx * qy - ely * qx >= 0.f && qx * ehy - qy * ehx >= 0.f;
+        if (!inside) {
+          const float c1 = elx * qx + ely * qy, c2 = ehx * qx + ehy * qy;
+          mc = (c1 > c2 ? c1 : c2) + delta;
+          mc = mc < 1.f ? mc : 1.f;
+        }
+      }
+      cap[i] = hi[i] * d.v2[i] * 0.5f * (1.f + mc) * rlam * (1.f + 1e-5f);
+    }
+    float tot = 0.f, all = 0.f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      float ahead = 0.f;
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        if (k != i && (cap[k] > cap[i] || (cap[k] == cap[i] && k < i))) ahead += cap[k];
+      float take = 1.f - ahead;
+      take = take < cap[i] ? take : cap[i];
+      take = take > 0.f ? take : 0.f;
+      tot += take * take;
+      all += cap[i];
+    }
+    if (tot == tot && all >= 1.f) ni_hi = tot < 1.f ? tot : 1.f;
+  }
+  float den = ni_hi * (1.f + 1e-3f) + 1e-9f;
+  den = den < 1.f + 2e-5f ? den : 1.f + 2e-5f;
+  float t_lb = (R2min * C - slack) * fast_rcp(den);
+  t_lb = t_lb > 0.f ? t_lb : 0.f;  // NaN -> 0
+  unsigned hi16 = 0x7c00u;          // +inf: no upper bound
+  const float fp_lb = 0.5f * R2min * fpw * (1.f - 1e-5f);
+  if (fp_lb * fp_lb >= 1e-3f) {
+    const float B = d.B1 < Bq ? d.B1 : Bq;
+    const float t_ub = (float)N * (1.f + 2e-5f) * (R2max * B + slack);
+    if (t_ub == t_ub) hi16 = half_up_f(t_ub);
+  }
+  return half_down_f(t_lb) | (hi16 << 16);
 }
 
 // DPF quality of the direction s; returns 1 if quality > gamma.
@@ -839,6 +1254,45 @@ __global__ __launch_bounds__(256) void k_term_bounds(const float* __restrict__ F
   t_ub[s] = bd.t_ub;
 }
 
+__global__ __launch_bounds__(256) void k_classes_plain(const float* __restrict__ rms, int P,
+                                                       int n, const double* __restrict__ ws,
+                                                       uint8_t* __restrict__ pclass,
+                                                       double* __restrict__ cstats,
+                                                       uint16_t* __restrict__ crem) {
+  pixel_classes_block<256>(rms, P, n, ws, pclass, cstats, crem);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_class_term_bounds(const float* __restrict__ FP,
+                                                           const float* __restrict__ FX,
+                                                           int64_t n_dirs,
+                                                           const double* __restrict__ cstats,
+                                                           double* __restrict__ t_lb,
+                                                           double* __restrict__ t_ub) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= n_dirs) return;
+  float fp0[N], fx0[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    fp0[i] = FP[s * N + i];
+    fx0[i] = FX[s * N + i];
+  }
+  const int G = (int)cstats[kClassSlotClasses];
+  DirPre<N> dp;
+  dir_pre<N>(fp0, fx0, dp);
+  for (int g = 0; g < kMaxClasses; ++g) {
+    double lo = 0.0, hi = -1.0;
+    if (g < G && cstats[g * kWStats + kClassSlotCount] > 0.0) {
+      const unsigned v = class_bound_pair<N>(dp, cstats + g * kWStats);
+      lo = half_bits_value((unsigned short)(v & 0xffffu));
+      const float h = half_bits_value((unsigned short)(v >> 16));
+      hi = __builtin_isinf(h) ? -1.0 : (double)h;
+    }
+    t_lb[s * kMaxClasses + g] = lo;
+    t_ub[s * kMaxClasses + g] = hi;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_weight_stats_plain(const float* __restrict__ rms, int P,
                                                             int n, double* __restrict__ out) {
   weight_stats_block<256>(rms, P, n, out);
@@ -936,6 +1390,11 @@ struct BatchDev {
   double* stat_max;
   double* reg_f64;
   double* wstats;           // per cluster kWStats doubles, or NULL (no regulator bounds)
+  uint8_t* pclass;          // class bytes (cluster c's P at rms_off[c]), or NULL (no classes)
+  double* cstats;           // per cluster kClassStats doubles
+  uint16_t* crem;           // per cluster class_rem_entries(P) counts
+  const int64_t* crem_off;  // per cluster offset into crem
+  unsigned* cbound;         // class bounds scratch: per wavefront of the grid kList x 16 words
 };
 
 __global__ void k_batch_init(BatchDev b, int n_tickets) {
@@ -951,6 +1410,17 @@ __global__ __launch_bounds__(256) void k_batch_weights(BatchDev b) {
   const int c = blockIdx.x;
   weight_stats_block<256>(b.rms + b.rms_off[c], b.npix[c], b.net_nifo[b.net[c]],
                           b.wstats + (int64_t)c * kWStats);
+}
+
+__global__ __launch_bounds__(256) void k_batch_classes(BatchDev b) {
+  const int c = blockIdx.x;
+  const int n = b.net_nifo[b.net[c]];
+  // the class bytes of cluster c start at its first noise weight's index
+  // (rms_off[c]): a buffer of one byte per noise weight holds every cluster's
+  // P class bytes apart from the others whatever their detector counts
+  pixel_classes_block<256>(b.rms + b.rms_off[c], b.npix[c], n, b.wstats + (int64_t)c * kWStats,
+                           b.pclass + b.rms_off[c], b.cstats + (int64_t)c * kClassStats,
+                           b.crem + b.crem_off[c]);
 }
 
 // Waves per SIMD the compiler must leave room for (caps VGPRs), per phase and
@@ -1003,6 +1473,13 @@ struct PhaseOccupancy {
 #define PYCWB_REG_POOL 8
 #endif
 constexpr int kRegPool = PYCWB_REG_POOL;
+// Class-bound scratch of the batched regulator: per wavefront of the grid, a
+// half-precision bound pair per class for each of the kRegPool * 64 listed
+// directions.
+inline size_t class_bound_scratch_bytes(int block, int grid) {
+  return (size_t)grid * (size_t)(block / 64) * (size_t)(kRegPool * 64) * kMaxClasses *
+         sizeof(unsigned);
+}
 
 __device__ __forceinline__ int lane_rank(unsigned long long mask) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
@@ -1012,9 +1489,11 @@ __device__ __forceinline__ int lane_rank(unsigned long long mask) {
 template <int N, int BLOCK>
 __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int item) {
   constexpr int kList = kRegPool * 64;
+  static_assert(2 * kList >= kMaxClasses * 64, "class bounds fit the bound area");
   __shared__ int s_dir[BLOCK / 64][kList];
-  __shared__ float s_lb[BLOCK / 64][kList];
-  __shared__ float s_ub[BLOCK / 64][kList];
+  // per listed direction: float bits of its lower and upper term bound
+  // (without pixel classes)
+  __shared__ unsigned s_aux[BLOCK / 64][2 * kList];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int net = b.net[c];
   const float* FP = b.FP + b.net_off[net];
@@ -1024,6 +1503,14 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
   const bool early = (b.flags & PYCWB_REGULATOR_EARLY_EXIT) != 0;
   const double* ws = (b.wstats && (b.flags & PYCWB_REGULATOR_BOUNDS))
                          ? b.wstats + (int64_t)c * kWStats : nullptr;
+  const double* cls = (ws && early && b.cstats) ? b.cstats + (int64_t)c * kClassStats : nullptr;
+  const int G = cls ? (int)cls[kClassSlotClasses] : 1;
+  const bool classes = G > 1;
+  const uint16_t* crem = classes ? b.crem + b.crem_off[c] : nullptr;
+  // this wavefront's class bounds: kMaxClasses half pairs per listed direction
+  unsigned* cb = classes ? b.cbound + ((int64_t)blockIdx.x * (BLOCK / 64) + wave) * kList *
+                               kMaxClasses
+                         : nullptr;
   const double den = (double)P + 0.01;
   const double thr_hi = b.gamma * b.gamma * den * (1.0 + 1e-9);
   const double thr_lo = b.gamma * b.gamma * den * (1.0 - 1e-9);
@@ -1065,12 +1552,38 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
     if (open) {
       const int slot = n_open + lane_rank(m);
       s_dir[wave][slot] = (int)s;
-      s_lb[wave][slot] = lb;
-      s_ub[wave][slot] = ub;
+      if (!classes) {
+        s_aux[wave][slot] = __float_as_uint(lb);
+        s_aux[wave][kList + slot] = __float_as_uint(ub);
+      }
     }
     n_open += __popcll(m);
   }
   __syncthreads();
+
+  // pass 1b (pixel classes): each listed direction's bounds per class, one
+  // direction per lane, into the wavefront's area of the class-bound scratch
+  if (classes) {
+    for (int slot = lane; slot < n_open; slot += 64) {
+      const int64_t s = s_dir[wave][slot];
+      float fp1[N], fx1[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        fp1[i] = FP[s * N + i];
+        fx1[i] = FX[s * N + i];
+      }
+      DirPre<N> dp;
+      dir_pre<N>(fp1, fx1, dp);
+#pragma unroll 1
+      for (int g = 0; g < kMaxClasses; ++g) {
+        unsigned v = 0u;
+        if (g < G && cls[g * kWStats + kClassSlotCount] > 0.0)
+          v = class_bound_pair<N>(dp, cls + g * kWStats);
+        cb[(int64_t)slot * kMaxClasses + g] = v;
+      }
+    }
+    __syncthreads();  // the other lanes of the wavefront read these
+  }
 
   // pass 2: the sweep, a listed direction per idle lane
   int next = 0;
@@ -1078,6 +1591,7 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
   int j = 0;
   float NI = 0.f, lb = 0.f, ub = -1.f;
   unsigned NN = 0;
+  int cur = 0;  // the lane's list slot
   float fp0[N], fx0[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) fp0[i] = fx0[i] = 0.f;
@@ -1092,8 +1606,11 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
           fp0[i] = FP[s * N + i];
           fx0[i] = FX[s * N + i];
         }
-        lb = s_lb[wave][slot];
-        ub = s_ub[wave][slot];
+        cur = slot;
+        if (!classes) {
+          lb = __uint_as_float(s_aux[wave][slot]);
+          ub = __uint_as_float(s_aux[wave][kList + slot]);
+        }
         j = 0;
         NI = 0.f;
         NN = 0u;
@@ -1114,6 +1631,36 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
         const double q = __builtin_sqrt((double)NI / ((double)NN + 0.01));
         cnt += q > b.gamma ? 1u : 0u;
         busy = false;
+      } else if (early && classes) {
+        // remaining pixels of each class times its bounds (j is a multiple of 16);
+        // a count of 0 also needs every pixel so far in NN (fp > 0)
+        const uint4* rq = reinterpret_cast<const uint4*>(crem + (int64_t)(j >> 4) * kMaxClasses);
+        const uint4 r0 = rq[0], r1 = rq[1];
+        const unsigned rw[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+        const uint4* bq = reinterpret_cast<const uint4*>(cb + (int64_t)cur * kMaxClasses);
+        const uint4 b0 = bq[0], b1 = bq[1], b2 = bq[2], b3 = bq[3];
+        const unsigned bw[16] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w,
+                                 b2.x, b2.y, b2.z, b2.w, b3.x, b3.y, b3.z, b3.w};
+        double ls = 0.0, us = 0.0;
+        bool hub = NN == (unsigned)j;
+#pragma unroll
+        for (int g = 0; g < kMaxClasses; ++g) {
+          const unsigned rem = (rw[g >> 1] >> ((g & 1) * 16)) & 0xffffu;
+          if (g < G && rem) {
+            const unsigned v = bw[g];
+            const float lo = half_bits_value((unsigned short)(v & 0xffffu));
+            const float hi = half_bits_value((unsigned short)(v >> 16));
+            ls += (double)rem * (double)lo;
+            if (__builtin_isinf(hi)) hub = false;
+            else us += (double)rem * (double)hi;
+          }
+        }
+        if (((double)NI + ls) * sum_lo > thr_hi) {
+          cnt += 1u;
+          busy = false;
+        } else if (hub && ((double)NI + us) * sum_hi < thr_lo) {
+          busy = false;
+        }
       } else if (early) {
         const double rest = (double)(P - j);
         if (((double)NI + rest * (double)lb) * sum_lo > thr_hi) {
@@ -1433,6 +1980,37 @@ int pycwb_regulator_term_bounds(int n_ifo, int n_pix, const float* rms, const fl
   return PYCWB_OK;
 }
 
+int pycwb_regulator_class_bounds(int n_ifo, int n_pix, const float* rms, const float* FP,
+                                 const float* FX, int64_t n_dirs, double* weight_stats,
+                                 uint8_t* pixel_class, double* class_stats, uint16_t* class_rem,
+                                 double* t_lb, double* t_ub, void* stream) {
+  clear_stale_error();
+  if (n_ifo < 1 || n_ifo > kMaxIfo || n_pix < 1 || n_dirs < 1) return PYCWB_ERR_ARG;
+  if (!rms || !FP || !FX || !weight_stats || !pixel_class || !class_stats || !class_rem ||
+      !t_lb || !t_ub)
+    return PYCWB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_weight_stats_plain, dim3(1), dim3(256), 0, st, rms, n_pix, n_ifo,
+                     weight_stats);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_classes_plain, dim3(1), dim3(256), 0, st, rms, n_pix, n_ifo,
+                     (const double*)weight_stats, pixel_class, class_stats, class_rem);
+  HIP_TRY(hipGetLastError());
+  const unsigned grid = (unsigned)((n_dirs + 255) / 256);
+  switch (n_ifo) {
+#define CASE(NN)                                                                             \
+  case NN:                                                                                   \
+    hipLaunchKernelGGL((k_class_term_bounds<NN>), dim3(grid), dim3(256), 0, st, FP, FX, n_dirs, \
+                       (const double*)class_stats, t_lb, t_ub);                              \
+    break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default: return PYCWB_ERR_ARG;
+  }
+  HIP_TRY(hipGetLastError());
+  return PYCWB_OK;
+}
+
 int pycwb_dpf_regulator(int n_ifo, int n_pix, int64_t n_sky, const float* FP, const float* FX,
                         const float* rms, double gamma_regulator,
                         double network_energy_threshold, float reg0,
@@ -1577,6 +2155,16 @@ int batch_from_desc(const pycwb_batch_desc* d, bool own_scratch, pycwb_batch* b)
   v.stat_max = d->sky_stat_max;
   v.reg_f64 = d->reg_f64;
   v.wstats = d->weight_stats;
+  // pixel classes need all five buffers, the weight statistics and a
+  // class-bound scratch for every wavefront of the grid
+  const bool cls = d->class_stats && d->pixel_class && d->class_rem && d->class_rem_off &&
+                   d->class_bounds && d->weight_stats &&
+                   d->class_bounds_bytes >= class_bound_scratch_bytes(d->block, d->grid);
+  v.pclass = cls ? d->pixel_class : nullptr;
+  v.cstats = cls ? d->class_stats : nullptr;
+  v.crem = cls ? d->class_rem : nullptr;
+  v.crem_off = cls ? d->class_rem_off : nullptr;
+  v.cbound = cls ? d->class_bounds : nullptr;
   b->block = d->block;
   b->grid = d->grid;
   b->n_groups = d->n_groups;
@@ -1700,12 +2288,25 @@ int pycwb_batch_run_noalloc(const pycwb_batch_desc* d, void* stream) {
   b.dev.counts = (unsigned*)(base + kNoAllocCountsOff);
   b.dev.best = (unsigned long long*)(base + kNoAllocBestOff);
   b.dev.wstats = (double*)(base + kNoAllocWStatsOff);
+  b.dev.pclass = nullptr;  // no pixel classes: their buffers scale with the pixels
+  b.dev.cstats = nullptr;
+  b.dev.crem = nullptr;
+  b.dev.crem_off = nullptr;
+  b.dev.cbound = nullptr;
   // stage 0 clears the tickets, counts and argmax keys with a fill kernel and
   // writes every weight statistic it later reads
   return batch_enqueue(&b, (hipStream_t)stream);
 }
 
 size_t pycwb_batch_noalloc_max_clusters(void) { return kNoAllocMaxClusters; }
+
+int64_t pycwb_class_rem_entries(int n_pix) { return n_pix < 0 ? 0 : class_rem_entries(n_pix); }
+
+size_t pycwb_class_bound_scratch_bytes(int block, int grid) {
+  return (block == 64 || block == 128 || block == 256) && grid > 0
+             ? class_bound_scratch_bytes(block, grid)
+             : 0;
+}
 
 int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
   if (!b || stage < 0 || stage > 5) return PYCWB_ERR_ARG;
@@ -1719,6 +2320,10 @@ int pycwb_batch_launch_stage(pycwb_batch* b, int stage, void* stream) {
     if (d.wstats && (d.flags & PYCWB_REGULATOR_BOUNDS)) {
       hipLaunchKernelGGL(k_batch_weights, dim3((unsigned)d.n_clusters), dim3(256), 0, st, d);
       HIP_TRY(hipGetLastError());
+      if (d.cstats) {
+        hipLaunchKernelGGL(k_batch_classes, dim3((unsigned)d.n_clusters), dim3(256), 0, st, d);
+        HIP_TRY(hipGetLastError());
+      }
     }
     return PYCWB_OK;
   }

@@ -245,3 +245,112 @@ def test_kernel_bounds_follow_the_restatement(oracle, ifos, kind, seed):
     t = (fx / (ni + np.float32(1e-9))).astype(np.float64)
     assert np.all(t >= lb[:, None])
     assert np.all(t[ub >= 0] <= ub[ub >= 0, None])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ifos,kind,P,seed", [
+    (synth.NETWORK_3, "bench", 1500, 31), (synth.NETWORK_4, "bench", 700, 32),
+    (synth.NETWORK_3, "spread", 300, 33), (synth.NETWORK_4, "dominant", 1100, 34),
+    (synth.NETWORK_5, "narrow", 260, 35), (synth.NETWORK_3, "bench", 40, 36),
+])
+def test_pixel_class_bounds_hold(oracle, ifos, kind, P, seed):
+    """The batched regulator's pixel classes (through the diagnostic entry
+    point): every pixel's weights inside its class's box, the remaining-count
+    table equal to the suffix counts of the classes, and every float32 term
+    inside its class's bounds for every direction."""
+    import torch
+
+    from pycwb_amd import _lib
+    from pycwb_amd.device import lib, ptr
+
+    print(f"case pixel classes {len(ifos)} detectors, {kind}, P={P}", flush=True)
+    dev = torch.device("cuda", 0)
+    net = synth.make_network(ifos, order=5, K=8)
+    n, S = net.n_ifo, net.n_sky
+    rms = _rms(kind, P, n, seed)
+    ent = int(lib().pycwb_class_rem_entries(P))
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in
+         (("rms", rms), ("fp", net.FP_t.astype(np.float32)), ("fx", net.FX_t.astype(np.float32)))}
+    ws = torch.zeros(24, dtype=torch.float64, device=dev)
+    pcl = torch.zeros(P, dtype=torch.uint8, device=dev)
+    cst = torch.zeros(16 * 24, dtype=torch.float64, device=dev)
+    rem = torch.zeros(ent, dtype=torch.int16, device=dev)
+    lb = torch.empty(S * 16, dtype=torch.float64, device=dev)
+    ub = torch.empty(S * 16, dtype=torch.float64, device=dev)
+    _lib.check(lib().pycwb_regulator_class_bounds(n, P, ptr(d["rms"]), ptr(d["fp"]), ptr(d["fx"]),
+                                                   S, ptr(ws), ptr(pcl), ptr(cst), ptr(rem),
+                                                   ptr(lb), ptr(ub), None),
+               "pycwb_regulator_class_bounds")
+    torch.cuda.synchronize()
+    pcl = pcl.cpu().numpy().astype(np.int64)
+    cst = cst.cpu().numpy().reshape(16, 24)
+    G = int(cst[0, 19])
+    expect_G = 1 if P < 64 else 4 if P < 256 else 8 if P < 1024 else 16
+    assert G == expect_G
+    assert pcl.max() < G
+    counts = np.bincount(pcl, minlength=16)
+    np.testing.assert_array_equal(cst[:, 18], counts[:16] * (np.arange(16) < G))
+    r2 = rms.astype(np.float64) ** 2
+    R2 = r2.sum(1)
+    W = r2 / R2[:, None]
+    for g in range(G):
+        m = pcl == g
+        if not m.any():
+            continue
+        assert cst[g, 0] <= R2[m].min() and R2[m].max() <= cst[g, 1]
+        assert np.all(cst[g, 2:2 + n] <= W[m].min(0)) and np.all(W[m].max(0) <= cst[g, 10:10 + n])
+    if G == 1:
+        return
+    rem = rem.cpu().numpy().view(np.uint16).astype(np.int64).reshape(-1, 16)
+    nq = (P + 15) // 16
+    onehot = np.zeros((nq * 16 + 16, 16), np.int64)
+    onehot[np.arange(P), pcl] = 1
+    suffix = onehot[::-1].cumsum(0)[::-1]
+    np.testing.assert_array_equal(rem[:nq + 1], suffix[0:16 * (nq + 1):16])
+    lb = lb.cpu().numpy().reshape(S, 16)
+    ub = ub.cpu().numpy().reshape(S, 16)
+    fp, fx, ni = oracle.dpf_terms(net.FP_t, net.FX_t, rms, np.arange(S, dtype=np.int64))
+    t = (fx / (ni + np.float32(1e-9))).astype(np.float64)
+    assert np.all(t >= lb[:, pcl])
+    u = ub[:, pcl]
+    assert np.all((u < 0) | (t <= u))
+    assert np.all(fp[u >= 0] > 0)
+
+
+@pytest.mark.gpu
+def test_batched_regulator_with_pixel_classes_mixed_detectors(oracle):
+    """A batch mixing three, four and five detectors (so the clusters' noise
+    weights start at offsets that are not multiples of their own detector
+    counts): the regulator counts with pixel classes equal those without and
+    the oracle's."""
+    import torch
+
+    from pycwb_amd.batch import BatchedSkyScan, net_tables, synth_batch
+
+    print("case batched regulator, pixel classes, 3/4/5 detectors", flush=True)
+    dev = torch.device("cuda", 0)
+    nets = [synth.make_network(synth.NETWORK_3, order=4, K=8),
+            synth.make_network(synth.NETWORK_4, order=4, K=8),
+            synth.make_network(synth.NETWORK_5, order=4, K=8)]
+    sizes = np.array([1500, 37, 700, 260, 1100, 90, 64, 2100, 300, 1023], np.int64)
+    ids = np.array([1, 0, 2, 1, 0, 2, 0, 1, 2, 0], np.int64)
+    b = synth_batch(nets, sizes, ids, seed=515, device=dev)
+    nt = net_tables(nets, device=dev)
+    valid = torch.arange(nets[0].n_sky, dtype=torch.int64, device=dev)
+    gamma = 0.5 * 0.5 * 2 / 3
+    res = {}
+    for cls in (True, False):
+        bs = BatchedSkyScan(b, nt, valid, gamma, 0.5, 0.5, block=64, grid=64,
+                            pixel_classes=cls)
+        bs.launch()
+        res[cls] = bs.results()
+        bs.close()
+    np.testing.assert_array_equal(res[True]["regulator"], res[False]["regulator"])
+    np.testing.assert_array_equal(res[True]["l_max"], res[False]["l_max"])
+    vh = np.arange(nets[0].n_sky, dtype=np.int64)
+    for c in (0, 2, 7):
+        net = nets[ids[c]]
+        rms = b.cluster_rms(c).cpu().numpy()
+        reg = oracle.calculate_dpf(net.FP_t, net.FX_t, rms, net.n_sky, net.n_ifo, gamma,
+                                   nt.En[ids[c]], vh)
+        assert reg == res[True]["regulator"][c], c
