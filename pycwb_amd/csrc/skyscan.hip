@@ -465,8 +465,20 @@ constexpr int kMaxClasses = 16;
 constexpr int kClassStats = kMaxClasses * kWStats;  // doubles per cluster
 constexpr int kClassSlotCount = 18, kClassSlotClasses = 19;
 
-__host__ __device__ inline int class_levels(int P) {
-  return P < 64 ? 0 : P < 256 ? 2 : P < 1024 ? 3 : P < 65536 ? 4 : 0;
+#ifndef PYCWB_CLASS_P0
+#define PYCWB_CLASS_P0 128
+#endif
+#ifndef PYCWB_CLASS_MAX_LEVEL
+#define PYCWB_CLASS_MAX_LEVEL 4
+#endif
+#ifndef PYCWB_CLASS_MAX_LEVEL_WIDE
+#define PYCWB_CLASS_MAX_LEVEL_WIDE 4  // four or more detectors
+#endif
+__host__ __device__ inline int class_levels(int P, int n) {
+  const int p0 = PYCWB_CLASS_P0;
+  const int L = P < p0 ? 0 : P < 4 * p0 ? 2 : P < 16 * p0 ? 3 : P < 65536 ? 4 : 0;
+  const int cap = n >= 4 ? PYCWB_CLASS_MAX_LEVEL_WIDE : PYCWB_CLASS_MAX_LEVEL;
+  return L < cap ? L : cap;
 }
 
 __host__ __device__ inline int64_t class_rem_entries(int P) {
@@ -497,7 +509,7 @@ __device__ void pixel_classes_block(const float* __restrict__ rms, int P, int n,
   bool finite = __builtin_isfinite(ws[0]) && __builtin_isfinite(ws[1]);
   for (int i = 0; i < n; ++i)
     finite = finite && __builtin_isfinite(ws[kWMin + i]) && __builtin_isfinite(ws[kWMax + i]);
-  const int L = finite ? class_levels(P) : 0;
+  const int L = finite ? class_levels(P, n) : 0;
   const int G = 1 << L;
   const int t = threadIdx.x;
   for (int j = t; j < P; j += BLK) pclass[j] = 0;
@@ -694,13 +706,14 @@ __device__ __forceinline__ void dir_pre(const float (&fp0)[N], const float (&fx0
 template <int N>
 __device__ __forceinline__ unsigned class_bound_pair(const DirPre<N>& d,
                                                      const double* __restrict__ cg) {
-  const float R2min = (float)cg[0], R2max = (float)cg[1];
+  // statistics to float32 rounded outwards (exact for a class's, which are floats)
+  const float R2min = __double2float_rd(cg[0]), R2max = __double2float_ru(cg[1]);
   float lo[N], hi[N];
   float sw = 0.f, fpw = 0.f, cs = 0.f;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    lo[i] = (float)cg[kWMin + i];
-    hi[i] = (float)cg[kWMax + i];
+    lo[i] = __double2float_rd(cg[kWMin + i]);
+    hi[i] = __double2float_ru(cg[kWMax + i]);
     sw += lo[i];
     fpw += lo[i] * d.v2[i];
     cs += lo[i] + hi[i];
@@ -1536,17 +1549,22 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
         fp0[i] = FP[s * N + i];
         fx0[i] = FX[s * N + i];
       }
-      const RegBounds bd = regulator_bounds<N>(fp0, fx0, ws);
+      // the cluster-wide bounds, float32 as a class's (the cluster's statistics
+      // in place of a class's)
+      DirPre<N> dp;
+      dir_pre<N>(fp0, fx0, dp);
+      const unsigned pr = class_bound_pair<N>(dp, ws);
+      lb = half_bits_value((unsigned short)(pr & 0xffffu));
+      const float uh = half_bits_value((unsigned short)(pr >> 16));
+      ub = __builtin_isinf(uh) ? -1.f : uh;
       if (early) {
-        if ((double)P * bd.t_lb * sum_lo > thr_hi) {
+        if ((double)P * (double)lb * sum_lo > thr_hi) {
           cnt += 1u;
           open = false;
-        } else if (bd.t_ub >= 0.0 && (double)P * bd.t_ub * sum_hi < thr_lo) {
+        } else if (ub >= 0.f && (double)P * (double)ub * sum_hi < thr_lo) {
           open = false;
         }
       }
-      lb = __double2float_rd(bd.t_lb);
-      ub = bd.t_ub >= 0.0 ? __double2float_ru(bd.t_ub) : -1.f;
     }
     const unsigned long long m = __ballot(open);
     if (open) {
@@ -1619,6 +1637,38 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
     }
     next += __popcll(idle);
     if (__ballot(busy) == 0ull) break;  // every listed direction decided
+    if (busy && early && classes && j < P) {
+      // remaining pixels of each class times its bounds (j is a multiple of
+      // 16; at j = 0 the class bounds may decide what the cluster-wide bounds
+      // could not); a count of 0 also needs every pixel so far in NN (fp > 0)
+      const uint4* rq = reinterpret_cast<const uint4*>(crem + (int64_t)(j >> 4) * kMaxClasses);
+      const uint4 r0 = rq[0], r1 = rq[1];
+      const unsigned rw[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+      const uint4* bq = reinterpret_cast<const uint4*>(cb + (int64_t)cur * kMaxClasses);
+      const uint4 b0 = bq[0], b1 = bq[1], b2 = bq[2], b3 = bq[3];
+      const unsigned bw[16] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w,
+                               b2.x, b2.y, b2.z, b2.w, b3.x, b3.y, b3.z, b3.w};
+      double ls = 0.0, us = 0.0;
+      bool hub = NN == (unsigned)j;
+#pragma unroll
+      for (int g = 0; g < kMaxClasses; ++g) {
+        const unsigned rem = (rw[g >> 1] >> ((g & 1) * 16)) & 0xffffu;
+        if (g < G && rem) {
+          const unsigned v = bw[g];
+          const float lo = half_bits_value((unsigned short)(v & 0xffffu));
+          const float hi = half_bits_value((unsigned short)(v >> 16));
+          ls += (double)rem * (double)lo;
+          if (__builtin_isinf(hi)) hub = false;
+          else us += (double)rem * (double)hi;
+        }
+      }
+      if (((double)NI + ls) * sum_lo > thr_hi) {
+        cnt += 1u;
+        busy = false;
+      } else if (hub && ((double)NI + us) * sum_hi < thr_lo) {
+        busy = false;
+      }
+    }
     if (busy) {
       const int stop = j + 16 < P ? j + 16 : P;
       for (; j < stop; ++j) {
@@ -1631,37 +1681,7 @@ __device__ __forceinline__ void regulator_item(const BatchDev& b, int c, int ite
         const double q = __builtin_sqrt((double)NI / ((double)NN + 0.01));
         cnt += q > b.gamma ? 1u : 0u;
         busy = false;
-      } else if (early && classes) {
-        // remaining pixels of each class times its bounds (j is a multiple of 16);
-        // a count of 0 also needs every pixel so far in NN (fp > 0)
-        const uint4* rq = reinterpret_cast<const uint4*>(crem + (int64_t)(j >> 4) * kMaxClasses);
-        const uint4 r0 = rq[0], r1 = rq[1];
-        const unsigned rw[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
-        const uint4* bq = reinterpret_cast<const uint4*>(cb + (int64_t)cur * kMaxClasses);
-        const uint4 b0 = bq[0], b1 = bq[1], b2 = bq[2], b3 = bq[3];
-        const unsigned bw[16] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w,
-                                 b2.x, b2.y, b2.z, b2.w, b3.x, b3.y, b3.z, b3.w};
-        double ls = 0.0, us = 0.0;
-        bool hub = NN == (unsigned)j;
-#pragma unroll
-        for (int g = 0; g < kMaxClasses; ++g) {
-          const unsigned rem = (rw[g >> 1] >> ((g & 1) * 16)) & 0xffffu;
-          if (g < G && rem) {
-            const unsigned v = bw[g];
-            const float lo = half_bits_value((unsigned short)(v & 0xffffu));
-            const float hi = half_bits_value((unsigned short)(v >> 16));
-            ls += (double)rem * (double)lo;
-            if (__builtin_isinf(hi)) hub = false;
-            else us += (double)rem * (double)hi;
-          }
-        }
-        if (((double)NI + ls) * sum_lo > thr_hi) {
-          cnt += 1u;
-          busy = false;
-        } else if (hub && ((double)NI + us) * sum_hi < thr_lo) {
-          busy = false;
-        }
-      } else if (early) {
+      } else if (early && !classes) {
         const double rest = (double)(P - j);
         if (((double)NI + rest * (double)lb) * sum_lo > thr_hi) {
           cnt += 1u;

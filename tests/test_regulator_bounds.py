@@ -249,7 +249,7 @@ def test_kernel_bounds_follow_the_restatement(oracle, ifos, kind, seed):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("ifos,kind,P,seed", [
-    (synth.NETWORK_3, "bench", 1500, 31), (synth.NETWORK_4, "bench", 700, 32),
+    (synth.NETWORK_3, "bench", 2500, 31), (synth.NETWORK_4, "bench", 700, 32),
     (synth.NETWORK_3, "spread", 300, 33), (synth.NETWORK_4, "dominant", 1100, 34),
     (synth.NETWORK_5, "narrow", 260, 35), (synth.NETWORK_3, "bench", 40, 36),
 ])
@@ -285,7 +285,8 @@ def test_pixel_class_bounds_hold(oracle, ifos, kind, P, seed):
     pcl = pcl.cpu().numpy().astype(np.int64)
     cst = cst.cpu().numpy().reshape(16, 24)
     G = int(cst[0, 19])
-    expect_G = 1 if P < 64 else 4 if P < 256 else 8 if P < 1024 else 16
+    # 1 class below 128 pixels, then 4, 8 and 16 from 128, 512 and 2048 pixels
+    expect_G = 1 if P < 128 else 4 if P < 512 else 8 if P < 2048 else 16
     assert G == expect_G
     assert pcl.max() < G
     counts = np.bincount(pcl, minlength=16)
