@@ -254,199 +254,6 @@ __constant__ double kUnitCS[32][2] = {
     {-0.9951847266721968, 0.09801714032956083},
 };
 
-// B1 = min over the 32 sampled unit vectors u of max_i (u.v_i)^2 for one direction
-template <int N>
-__device__ __forceinline__ double sampled_projection_bound(const float (&fp0)[N],
-                                                           const float (&fx0)[N]) {
-  double B = __builtin_inf();
-#pragma unroll 1
-  for (int q = 0; q < 32; ++q) {
-    const double cu = kUnitCS[q][0], su = kUnitCS[q][1];
-    double mx = 0.0;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double p = cu * (double)fp0[i] + su * (double)fx0[i];
-      mx = p * p > mx ? p * p : mx;
-    }
-    B = mx < B ? mx : B;
-  }
-  return B;
-}
-
-// B1 < 0: computed here (sampled_projection_bound), else the caller's value.
-template <int N>
-__device__ __forceinline__ RegBounds regulator_bounds(const float (&fp0)[N], const float (&fx0)[N],
-                                                      const double* __restrict__ ws,
-                                                      double B1 = -1.0) {
-  // ws is the same for every lane of the wavefront: its loads are scalar
-  const double R2min = ws[0], R2max = ws[1];
-  const double* lo = ws + kWMin;
-  const double* hi = ws + kWMax;
-  double v2max = 0.0, sw = 0.0, fpw = 0.0, cs = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const double a = fp0[i], b = fx0[i];
-    const double v2 = a * a + b * b;
-    v2max = v2 > v2max ? v2 : v2max;
-    sw += lo[i];
-    fpw += lo[i] * v2;
-    cs += lo[i] + hi[i];
-  }
-  const double slack = 1e-4 * R2max * v2max;
-  // reference point: the box centre scaled to the simplex; its Gram matrix
-  double m11 = 0.0, m12 = 0.0, m22 = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const double a = fp0[i], b = fx0[i];
-    const double w = (lo[i] + hi[i]) / cs;
-    m11 += w * (a * a);
-    m12 += w * (a * b);
-    m22 += w * (b * b);
-  }
-  double dx = m11 - m22, dy = 2.0 * m12;
-  const double dn = __builtin_sqrt(dx * dx + dy * dy);
-  dx = dn > 0.0 ? dx / dn : 1.0;
-  dy = dn > 0.0 ? dy / dn : 0.0;
-  // minor eigenvector of the centre's Gram matrix (any unit vector is valid)
-  double u1 = 1.0, u2 = 0.0;
-  {
-    const double h = 0.5 * (m11 + m22), disc = h * h - (m11 * m22 - m12 * m12);
-    const double lmin = h - __builtin_sqrt(disc > 0.0 ? disc : 0.0);
-    const double p1 = m12, p2 = lmin - m11, q1 = lmin - m22, q2 = m12;
-    const double np = p1 * p1 + p2 * p2, nq = q1 * q1 + q2 * q2;
-    if (np >= nq && np > 0.0) {
-      const double r = __builtin_sqrt(np);
-      u1 = p1 / r;
-      u2 = p2 / r;
-    } else if (nq > 0.0) {
-      const double r = __builtin_sqrt(nq);
-      u1 = q1 / r;
-      u2 = q2 / r;
-    }
-  }
-  // vertices: v < N 2^(N-1) for up to four detectors (coordinate k = v >>
-  // (N - 1) from the sum, the others at lo or hi by the bits of v), then, if
-  // none was feasible or for more detectors, the N simplex vertices (v >= nbox)
-  const int nbox = N <= 4 ? N << (N - 1) : 0;
-  double C = __builtin_inf(), Bq = 0.0, rho = __builtin_inf();
-  double trmin = __builtin_inf(), trmax = 0.0;
-  double tlo = __builtin_inf(), thi = -__builtin_inf();
-  double xlo = 0.0, ylo = 0.0, xhi = 0.0, yhi = 0.0;
-  int count = 0;
-#pragma unroll 1
-  for (int v = 0; v < nbox + N; ++v) {
-    if (v == nbox && count > 0) break;
-    const bool box = v < nbox;
-    const int k = box ? v >> (N - 1) : v - nbox;
-    const int bits = v & ((1 << (N - 1)) - 1);
-    double rest = 0.0, s11 = 0.0, s12 = 0.0, s22 = 0.0, k11 = 0.0, k12 = 0.0, k22 = 0.0;
-    double lok = 0.0, hik = 0.0;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double a = fp0[i], b = fx0[i];
-      if (i == k) {
-        k11 = a * a;
-        k12 = a * b;
-        k22 = b * b;
-        lok = lo[i];
-        hik = hi[i];
-      } else {
-        const int bit = i < k ? i : i - 1;
-        const double w = box && ((bits >> bit) & 1) ? hi[i] : lo[i];
-        rest += w;
-        s11 += w * (a * a);
-        s12 += w * (a * b);
-        s22 += w * (b * b);
-      }
-    }
-    const double wk = 1.0 - rest;
-    if (box && !(wk >= lok - 1e-12 && wk <= hik + 1e-12)) continue;
-    const double g11 = s11 + wk * k11, g12 = s12 + wk * k12, g22 = s22 + wk * k22;
-    const double h = 0.5 * (g11 + g22), det = g11 * g22 - g12 * g12;
-    const double disc = h * h - det;
-    const double lm = h - __builtin_sqrt(disc > 0.0 ? disc : 0.0);
-    C = lm < C ? lm : C;
-    const double q = u1 * u1 * g11 + 2.0 * u1 * u2 * g12 + u2 * u2 * g22;
-    Bq = q > Bq ? q : Bq;
-    const double x = g11 - g22, y = 2.0 * g12, tr = g11 + g22;
-    trmin = tr < trmin ? tr : trmin;
-    trmax = tr > trmax ? tr : trmax;
-    const double pr = dx * x + dy * y;
-    rho = pr < rho ? pr : rho;
-    if (pr > 0.0) {
-      const double t = (dx * y - dy * x) / pr;
-      if (t < tlo) {
-        tlo = t;
-        xlo = x;
-        ylo = y;
-      }
-      if (t > thi) {
-        thi = t;
-        xhi = x;
-        yhi = y;
-      }
-    }
-    ++count;
-  }
-  // upper bound on ni from the cone of dominant directions
-  double ni_hi = 1.0;
-  const bool cone = rho > 1e-3 * trmax && thi >= tlo;
-  const double delta = cone ? 1e-5 * (trmax / rho + 1.0) : 1.0;
-  if (cone && delta < 0.1) {
-    const double nlo = __builtin_sqrt(xlo * xlo + ylo * ylo);
-    const double nhi = __builtin_sqrt(xhi * xhi + yhi * yhi);
-    const double elx = xlo / nlo, ely = ylo / nlo, ehx = xhi / nhi, ehy = yhi / nhi;
-    const double lam = 0.5 * (trmin + rho);
-    double cap[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double a = fp0[i], b = fx0[i];
-      const double v2 = a * a + b * b;
-      double mc = 1.0;
-      if (v2 > 0.0) {
-        const double qx = (a * a - b * b) / v2, qy = 2.0 * (a * b) / v2;
-        const bool inside = elx * qy - ely * qx >= 0.0 && qx * ehy - qy * ehx >= 0.0;
-        if (!inside) {
-          const double c1 = elx * qx + ely * qy, c2 = ehx * qx + ehy * qy;
-          mc = (c1 > c2 ? c1 : c2) + delta;
-          mc = mc < 1.0 ? mc : 1.0;
-        }
-      }
-      cap[i] = hi[i] * v2 * 0.5 * (1.0 + mc) / lam * (1.0 + 1e-5);
-    }
-    // fill the largest caps first (ties in index order) up to a total of 1:
-    // cap i takes min(cap_i, 1 - the caps ahead of it), at least 0
-    double tot = 0.0, all = 0.0;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double ahead = 0.0;
-#pragma unroll
-      for (int k = 0; k < N; ++k)
-        if (k != i && (cap[k] > cap[i] || (cap[k] == cap[i] && k < i))) ahead += cap[k];
-      double take = 1.0 - ahead;
-      take = take < cap[i] ? take : cap[i];
-      take = take > 0.0 ? take : 0.0;
-      tot += take * take;
-      all += cap[i];
-    }
-    if (tot == tot && all >= 1.0) ni_hi = tot < 1.0 ? tot : 1.0;
-  }
-  RegBounds r;
-  double den = ni_hi * (1.0 + 1e-3) + 1e-9;
-  den = den < 1.0 + 2e-5 ? den : 1.0 + 2e-5;
-  r.t_lb = (R2min * C - slack) / den;
-  if (!(r.t_lb > 0.0)) r.t_lb = 0.0;
-  r.t_ub = -1.0;
-  const double fp_lb = 0.5 * R2min * fpw * (1.0 - 1e-5);
-  if (fp_lb * fp_lb >= 1e-3) {
-    const double b1 = B1 < 0.0 ? sampled_projection_bound<N>(fp0, fx0) : B1;
-    const double B = b1 < Bq ? b1 : Bq;
-    r.t_ub = (double)N * (1.0 + 2e-5) * (R2max * B + slack);
-  }
-  if (!(r.t_ub == r.t_ub)) r.t_ub = -1.0;  // NaN statistics: no upper bound
-  return r;
-}
-
 // ---------------------------------------------------------------------------
 // Pixel classes of a cluster for the regulator sweep.  The pixels are split
 // by their normalised noise weights w_ji = r_ji^2 / R2_j into 2^L classes (L =
@@ -637,14 +444,16 @@ __device__ __forceinline__ float half_bits_value(unsigned short b) {
   return (float)__builtin_bit_cast(_Float16, b);
 }
 
-// ---- class bounds in float32 -------------------------------------------
-// The class bounds follow regulator_bounds step for step in float32 with the
-// hardware's reciprocal and square root (about 1 ulp): every margin of those
-// bounds (1e-4 R2max max|v|^2 on lambda_min, the angle widening 1e-5 (tr /
-// rho + 1), 1e-5 on the caps, 1e-3 on ni, 2e-5 on the upper bound) is at
-// least a hundred times the float32 error of the quantity it covers, so the
-// bounds stay valid; the extreme rays of the cone are found by cross-product
-// comparisons instead of angles.
+// ---- term bounds in float32 --------------------------------------------
+// The bounds described above (RegBounds), for a cluster's statistics or a
+// pixel class's, computed in float32 with the hardware's reciprocal and
+// square root (about 1 ulp): every margin of the bounds (1e-4 R2max max|v|^2
+// on lambda_min, the angle widening 1e-5 (tr / rho + 1), 1e-5 on the caps,
+// 1e-3 on ni, 2e-5 on the upper bound) is at least a hundred times the
+// float32 error of the quantity it covers, so the bounds stay valid.  The
+// extreme rays of the cone are found by cross-product comparisons instead of
+// angles, and the result is kept as a pair of half-precision values rounded
+// outwards (lower bound down, upper bound up).
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
@@ -891,7 +700,14 @@ __device__ __forceinline__ unsigned regulator_lane(const float* __restrict__ FP,
   RegBounds bd;
   bd.t_lb = 0.0;
   bd.t_ub = -1.0;
-  if (ws) bd = regulator_bounds<N>(fp0, fx0, ws);
+  if (ws) {  // the float32 bounds of the batched regulator's first pass
+    DirPre<N> dp;
+    dir_pre<N>(fp0, fx0, dp);
+    const unsigned pr = class_bound_pair<N>(dp, ws);
+    bd.t_lb = half_bits_value((unsigned short)(pr & 0xffffu));
+    const float uh = half_bits_value((unsigned short)(pr >> 16));
+    bd.t_ub = __builtin_isinf(uh) ? -1.0 : (double)uh;
+  }
   const double den = (double)P + 0.01;
   const double thr_hi = gamma * gamma * den * (1.0 + 1e-9);
   const double thr_lo = gamma * gamma * den * (1.0 - 1e-9);
@@ -1262,9 +1078,12 @@ __global__ __launch_bounds__(256) void k_term_bounds(const float* __restrict__ F
     fp0[i] = FP[s * N + i];
     fx0[i] = FX[s * N + i];
   }
-  const RegBounds bd = regulator_bounds<N>(fp0, fx0, ws);
-  t_lb[s] = bd.t_lb;
-  t_ub[s] = bd.t_ub;
+  DirPre<N> dp;
+  dir_pre<N>(fp0, fx0, dp);
+  const unsigned pr = class_bound_pair<N>(dp, ws);
+  t_lb[s] = half_bits_value((unsigned short)(pr & 0xffffu));
+  const float uh = half_bits_value((unsigned short)(pr >> 16));
+  t_ub[s] = __builtin_isinf(uh) ? -1.0 : (double)uh;
 }
 
 __global__ __launch_bounds__(256) void k_classes_plain(const float* __restrict__ rms, int P,
