@@ -236,12 +236,107 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly(
   }
 }
 
+// The polyphase kernel with the fold's loads issued up front: thread (k, r)
+// reads the R + 4 KK samples x[(nb0 - 2 KK + u) M + k] (u < R + 4 KK) and
+// the 2 KK + 1 filter taps h[|k + 2 M q|] into registers first, then forms
+// acc[b] = sum over q = -KK..KK (ascending, the order of k_t2w_poly) of
+// h_q x[(nb0 + b + 2 q) M + k]: the same terms in the same order, so the
+// maps are identical, with one memory latency per thread instead of one per
+// step of the sliding window.  KK = (m_H - 1) / 2M is a template parameter
+// (6 for the WDM filters of every level here).
+template <int R, int KK>
+__global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
+    const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int M, int64_t N,
+    const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int log2_2M, int B,
+    int quadrature, double2* __restrict__ out) {
+  extern __shared__ double2 buf[];  // B * M complex (B / 2 bin pairs of length 2M)
+  const int M2 = 2 * M;
+  const int64_t tiles = (N + B - 1) / B;
+  const int64_t blk = blockIdx.x;
+  const int s = (int)(blk / tiles);
+  const int64_t n0 = (blk % tiles) * B;
+  if (s >= n_series) return;
+  const double* x = X + (int64_t)s * ldx;
+  double* lds = reinterpret_cast<double*>(buf);
+  const int items = (B / R) * M2;
+  for (int it = threadIdx.x; it < items; it += kThreads) {
+    const int k = it & (M2 - 1), r = it >> log2_2M;
+    const int64_t nb0 = n0 + (int64_t)r * R;
+    double xs[R + 4 * KK], h[2 * KK + 1];
+#pragma unroll
+    for (int u = 0; u < R + 4 * KK; ++u) xs[u] = ext_sample(x, L, (nb0 - 2 * KK + u) * M + k);
+#pragma unroll
+    for (int q = -KK; q <= KK; ++q) {
+      const int t = k + M2 * q;
+      const int at = t < 0 ? -t : t;
+      h[q + KK] = at <= m_H - 1 ? filt[at] : 0.0;
+    }
+    double acc[R];
+#pragma unroll
+    for (int b = 0; b < R; ++b) acc[b] = 0.0;
+#pragma unroll
+    for (int q = -KK; q <= KK; ++q) {
+#pragma unroll
+      for (int b = 0; b < R; ++b) acc[b] += h[q + KK] * xs[b + 2 * (q + KK)];
+    }
+    const int kr = (int)bitrev((unsigned)k, log2_2M);
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int bb = r * R + b;
+      lds[2 * ((bb >> 1) * M2 + kr) + (bb & 1)] = acc[b];
+    }
+  }
+  __syncthreads();
+  lds_fft(buf, B / 2, log2_2M, tw);
+  // split the pairs; edges and 00/90 assignment (WDM.cc:1494-1510); e / (M + 1)
+  // by a multiply-high (exact for e < 2^16, M + 1 <= 2^16)
+  const double sqrt2 = 1.4142135623730951;
+  const unsigned M1 = (unsigned)M + 1u;
+  const unsigned magic = 0xffffffffu / M1 + 1u;
+  for (int e = threadIdx.x; e < B * (M + 1); e += kThreads) {
+    const int b = (int)__umulhi((unsigned)e, magic);
+    const int f = e - b * (int)M1;
+    const int64_t n = n0 + b;
+    if (n >= N) continue;
+    const int p = b >> 1;
+    const double2 z = buf[p * M2 + f];
+    const double2 zc = buf[p * M2 + ((M2 - f) & (M2 - 1))];
+    double R_, I_;
+    if ((b & 1) == 0) {
+      R_ = 0.5 * (z.x + zc.x);
+      I_ = 0.5 * (z.y - zc.y);
+    } else {
+      R_ = 0.5 * (z.y + zc.y);
+      I_ = -0.5 * (z.x - zc.x);
+    }
+    if (f == 0 || f == M) {
+      R_ = R_ / sqrt2;
+      I_ = R_;
+    }
+    double2 o;
+    if (!quadrature) {
+      o = make_double2(R_ * R_ + I_ * I_, 0.0);
+    } else if ((f + (int)(n & 1)) & 1) {
+      o = make_double2(sqrt2 * I_, sqrt2 * R_);
+    } else {
+      o = make_double2(sqrt2 * R_, -sqrt2 * I_);
+    }
+    out[((int64_t)s * N + n) * (M + 1) + f] = o;
+  }
+}
+
 template <int R>
 void launch_poly(dim3 grid, hipStream_t st, const double* x, int64_t L, int64_t ldx,
                  int n_series, int M, int64_t N, const double* filt, int m_H, const double2* tw,
                  int log2_2M, int B, int quadrature, double2* out) {
-  hipLaunchKernelGGL((k_t2w_poly<R>), grid, dim3(kThreads), (size_t)B * M * sizeof(double2), st,
-                     x, L, ldx, n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
+  const size_t lds = (size_t)B * M * sizeof(double2);
+  if ((m_H - 1) / (2 * M) == 6) {
+    hipLaunchKernelGGL((k_t2w_poly_h<R, 6>), grid, dim3(kThreads), lds, st, x, L, ldx,
+                       n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
+  } else {
+    hipLaunchKernelGGL((k_t2w_poly<R>), grid, dim3(kThreads), lds, st, x, L, ldx, n_series, M,
+                       N, filt, m_H, tw, log2_2M, B, quadrature, out);
+  }
 }
 
 // Inverse transform w2t / w2tQ (cwb-core/WDM.cc:1536-1710), two kernels.
