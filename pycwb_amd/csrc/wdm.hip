@@ -325,12 +325,109 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
   }
 }
 
+// The polyphase kernel with its inputs staged in LDS (M <= 32): the block's
+// input window x[(n0 - 2 KK) M + i], i < (B + 4 KK + 1) M, is read once with
+// coalesced loads (edges mirrored as ext_sample does), the M twiddles once,
+// and the fold and the FFT read LDS only.  The fold forms the same terms in
+// the same order as k_t2w_poly_h, so the maps are identical.
+template <int R, int KK>
+__global__ __launch_bounds__(kThreads) void k_t2w_poly_s(
+    const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int M, int64_t N,
+    const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int log2_2M, int B,
+    int quadrature, double2* __restrict__ out) {
+  extern __shared__ double2 buf[];  // B * M complex, then the window, then M twiddles
+  const int M2 = 2 * M;
+  const int64_t tiles = (N + B - 1) / B;
+  const int64_t blk = blockIdx.x;
+  const int s = (int)(blk / tiles);
+  const int64_t n0 = (blk % tiles) * B;
+  if (s >= n_series) return;
+  const double* x = X + (int64_t)s * ldx;
+  double* lds = reinterpret_cast<double*>(buf);
+  const int W = (B + 4 * KK + 1) * M;
+  double* win = lds + 2 * B * M;
+  double2* tws = reinterpret_cast<double2*>(win + W);
+  const int64_t base = (n0 - 2 * KK) * M;
+  for (int i = threadIdx.x; i < W; i += kThreads) win[i] = ext_sample(x, L, base + i);
+  for (int i = threadIdx.x; i < M; i += kThreads) tws[i] = tw[i];
+  __syncthreads();
+  const int items = (B / R) * M2;
+  for (int it = threadIdx.x; it < items; it += kThreads) {
+    const int k = it & (M2 - 1), r = it >> log2_2M;
+    double xs[R + 4 * KK], h[2 * KK + 1];
+#pragma unroll
+    for (int u = 0; u < R + 4 * KK; ++u) xs[u] = win[(r * R + u) * M + k];
+#pragma unroll
+    for (int q = -KK; q <= KK; ++q) {
+      const int t = k + M2 * q;
+      const int at = t < 0 ? -t : t;
+      h[q + KK] = at <= m_H - 1 ? filt[at] : 0.0;
+    }
+    double acc[R];
+#pragma unroll
+    for (int b = 0; b < R; ++b) acc[b] = 0.0;
+#pragma unroll
+    for (int q = -KK; q <= KK; ++q) {
+#pragma unroll
+      for (int b = 0; b < R; ++b) acc[b] += h[q + KK] * xs[b + 2 * (q + KK)];
+    }
+    const int kr = (int)bitrev((unsigned)k, log2_2M);
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int bb = r * R + b;
+      lds[2 * ((bb >> 1) * M2 + kr) + (bb & 1)] = acc[b];
+    }
+  }
+  __syncthreads();
+  lds_fft(buf, B / 2, log2_2M, tws);
+  const double sqrt2 = 1.4142135623730951;
+  const unsigned M1 = (unsigned)M + 1u;
+  const unsigned magic = 0xffffffffu / M1 + 1u;
+  for (int e = threadIdx.x; e < B * (M + 1); e += kThreads) {
+    const int b = (int)__umulhi((unsigned)e, magic);
+    const int f = e - b * (int)M1;
+    const int64_t n = n0 + b;
+    if (n >= N) continue;
+    const int p = b >> 1;
+    const double2 z = buf[p * M2 + f];
+    const double2 zc = buf[p * M2 + ((M2 - f) & (M2 - 1))];
+    double R_, I_;
+    if ((b & 1) == 0) {
+      R_ = 0.5 * (z.x + zc.x);
+      I_ = 0.5 * (z.y - zc.y);
+    } else {
+      R_ = 0.5 * (z.y + zc.y);
+      I_ = -0.5 * (z.x - zc.x);
+    }
+    if (f == 0 || f == M) {
+      R_ = R_ / sqrt2;
+      I_ = R_;
+    }
+    double2 o;
+    if (!quadrature) {
+      o = make_double2(R_ * R_ + I_ * I_, 0.0);
+    } else if ((f + (int)(n & 1)) & 1) {
+      o = make_double2(sqrt2 * I_, sqrt2 * R_);
+    } else {
+      o = make_double2(sqrt2 * R_, -sqrt2 * I_);
+    }
+    out[((int64_t)s * N + n) * (M + 1) + f] = o;
+  }
+}
+
 template <int R>
 void launch_poly(dim3 grid, hipStream_t st, const double* x, int64_t L, int64_t ldx,
                  int n_series, int M, int64_t N, const double* filt, int m_H, const double2* tw,
                  int log2_2M, int B, int quadrature, double2* out) {
   const size_t lds = (size_t)B * M * sizeof(double2);
-  if ((m_H - 1) / (2 * M) == 6) {
+  // staged form: FFT buffer + input window + twiddles
+  const size_t lds_s = lds + (size_t)(B + 4 * 6 + 1) * M * sizeof(double) + M * sizeof(double2);
+  // staged below M = 64: there its 14-21 KiB per block leave the occupancy;
+  // at M = 64 and 128 the 37 and 52 KiB measured slower than k_t2w_poly_h
+  if ((m_H - 1) / (2 * M) == 6 && M <= 32) {
+    hipLaunchKernelGGL((k_t2w_poly_s<R, 6>), grid, dim3(kThreads), lds_s, st, x, L, ldx,
+                       n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
+  } else if ((m_H - 1) / (2 * M) == 6) {
     hipLaunchKernelGGL((k_t2w_poly_h<R, 6>), grid, dim3(kThreads), lds, st, x, L, ldx,
                        n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
   } else {
