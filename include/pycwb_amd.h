@@ -229,9 +229,11 @@ int pycwb_batch_destroy(pycwb_batch* batch);
  * counterpart in the reference; for callers that cannot allocate (code under
  * stream capture, serving loops, harnesses that forbid allocation).  The
  * kernels and results are those of pycwb_batch_launch on the same description;
- * the description's tickets, counts, best and weight_stats are ignored (may
- * be NULL): the work queues, regulator counts, argmax keys and noise-weight
- * statistics live in a device array of the library (about 2.3 MB per GPU,
+ * the description's tickets, counts, best, weight_stats and pixel-class
+ * buffers are ignored (may be NULL; the regulator then bounds its sweep by
+ * the cluster-wide bounds only, with the same counts): the work queues,
+ * regulator counts, argmax keys and noise-weight statistics live in a device
+ * array of the library (about 3.3 MB per GPU,
  * held until the process ends), cleared by a fill kernel at the start of
  * every call.  At most pycwb_batch_noalloc_max_clusters() (16,384) clusters;
  * more return PYCWB_ERR_ARG with the reason in pycwb_last_error().  The
