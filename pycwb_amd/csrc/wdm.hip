@@ -149,6 +149,11 @@ constexpr int kPolyLdsMax = 2048;  // complex doubles of LDS per block (32 KiB):
 
 // Bins per block: B * M complex doubles of LDS; measured on MI355X (5 x 2.46 M
 // samples), 16 KiB blocks are fastest up to M = 128 and 32 KiB blocks above.
+// Levels with M up to kStagedMaxM take the LDS-staged kernel (k_t2w_poly_s);
+// measured on MI355X, staging at M = 64 and 128 (with 1,024 or 512 complex
+// doubles of FFT buffer per block) lost more occupancy than it saved.
+constexpr int kStagedMaxM = 32;
+
 __host__ int poly_bins(int M) {
   const int budget = M <= 128 ? 1024 : kPolyLdsMax;
   const int B = budget / M;
@@ -242,14 +247,15 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly(
 // acc[b] = sum over q = -KK..KK (ascending, the order of k_t2w_poly) of
 // h_q x[(nb0 + b + 2 q) M + k]: the same terms in the same order, so the
 // maps are identical, with one memory latency per thread instead of one per
-// step of the sliding window.  KK = (m_H - 1) / 2M is a template parameter
-// (6 for the WDM filters of every level here).
+// step of the sliding window; the FFT reads its twiddles from LDS.  KK =
+// (m_H - 1) / 2M is a template parameter (6 for the WDM filters of every level
+// here).
 template <int R, int KK>
 __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
     const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int M, int64_t N,
     const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int log2_2M, int B,
     int quadrature, double2* __restrict__ out) {
-  extern __shared__ double2 buf[];  // B * M complex (B / 2 bin pairs of length 2M)
+  extern __shared__ double2 buf[];  // B * M complex (B / 2 bin pairs of length 2M), M twiddles
   const int M2 = 2 * M;
   const int64_t tiles = (N + B - 1) / B;
   const int64_t blk = blockIdx.x;
@@ -286,8 +292,12 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
       lds[2 * ((bb >> 1) * M2 + kr) + (bb & 1)] = acc[b];
     }
   }
+  // the M twiddles into LDS after the fold's buffer: every FFT stage reads
+  // them (0.157 against 0.167 ms at M = 64 with them in global memory)
+  double2* tws = buf + (size_t)B * M;
+  for (int i = threadIdx.x; i < M; i += kThreads) tws[i] = tw[i];
   __syncthreads();
-  lds_fft(buf, B / 2, log2_2M, tw);
+  lds_fft(buf, B / 2, log2_2M, tws);
   // split the pairs; edges and 00/90 assignment (WDM.cc:1494-1510); e / (M + 1)
   // by a multiply-high (exact for e < 2^16, M + 1 <= 2^16)
   const double sqrt2 = 1.4142135623730951;
@@ -422,13 +432,12 @@ void launch_poly(dim3 grid, hipStream_t st, const double* x, int64_t L, int64_t 
   const size_t lds = (size_t)B * M * sizeof(double2);
   // staged form: FFT buffer + input window + twiddles
   const size_t lds_s = lds + (size_t)(B + 4 * 6 + 1) * M * sizeof(double) + M * sizeof(double2);
-  // staged below M = 64: there its 14-21 KiB per block leave the occupancy;
-  // at M = 64 and 128 the 37 and 52 KiB measured slower than k_t2w_poly_h
-  if ((m_H - 1) / (2 * M) == 6 && M <= 32) {
+  if ((m_H - 1) / (2 * M) == 6 && M <= kStagedMaxM) {
     hipLaunchKernelGGL((k_t2w_poly_s<R, 6>), grid, dim3(kThreads), lds_s, st, x, L, ldx,
                        n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
   } else if ((m_H - 1) / (2 * M) == 6) {
-    hipLaunchKernelGGL((k_t2w_poly_h<R, 6>), grid, dim3(kThreads), lds, st, x, L, ldx,
+    hipLaunchKernelGGL((k_t2w_poly_h<R, 6>), grid, dim3(kThreads), lds + M * sizeof(double2),
+                       st, x, L, ldx,
                        n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
   } else {
     hipLaunchKernelGGL((k_t2w_poly<R>), grid, dim3(kThreads), lds, st, x, L, ldx, n_series, M,
