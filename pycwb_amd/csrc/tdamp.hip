@@ -65,6 +65,49 @@ __device__ __forceinline__ void band_sums(const float* __restrict__ plane, int64
   so = o;
 }
 
+// band_sums for a fixed number of taps W: every tap's plane value and filter
+// coefficient is loaded before any is summed (rows outside the planes at a
+// safe index, then taken as zero), and the even and odd sums add them in the
+// same order as band_sums, so the results are identical.  Out of line, so
+// that the six sums of a thread run one after another, each with its loads in
+// flight together, instead of all at once beyond the register file.
+template <int W>
+__device__ __attribute__((noinline)) void band_sums_w(const float* __restrict__ plane,
+                                                      int64_t n_rows, int M1, int n, int m,
+                                                      const double* __restrict__ T, int64_t D,
+                                                      double& se, double& so) {
+  const float* col = plane + m;
+  float pv[W];
+  double tv[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const int64_t r = (int64_t)n + k;
+    const bool in = r >= 0 && r < n_rows;
+    pv[k] = col[(in ? r : 0) * M1];
+    pv[k] = in ? pv[k] : 0.f;
+    tv[k] = T[k * D];
+  }
+  double e = 0.0, o = 0.0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    if (k & 1) o += (double)pv[k] * tv[k];
+    else e += (double)pv[k] * tv[k];
+  }
+  se = e;
+  so = o;
+}
+
+template <int WT>
+__device__ __forceinline__ void layer_sums(const float* __restrict__ plane, int64_t n_rows,
+                                           int M1, int n, int m, const double* __restrict__ T,
+                                           int64_t D, int W, double& se, double& so) {
+  if constexpr (WT > 0) {
+    band_sums_w<WT>(plane, n_rows, M1, n, m, T, D, se, so);
+  } else {
+    band_sums(plane, n_rows, M1, n, m, T, D, W, se, so);
+  }
+}
+
 // sin and cos of q pi / (2J) for q in [0, 4J): every phase of the filter sums
 // is such a multiple (m pi dt / M = 2 m r pi / 2J, (2m -+ 1) dt pi / 2M =
 // (2m -+ 1) r pi / 2J with dt = r M / J), as WDM<T>::setTDFilter tabulates
@@ -75,6 +118,7 @@ __device__ __forceinline__ double2 phase(const double2* __restrict__ trig, int J
   return trig[r];  // (sin, cos)
 }
 
+template <int WT>
 __device__ __forceinline__ double pixel_amp(const TdArgs& a, int n, int m, int dT,
                                             const float* __restrict__ plane, bool quad) {
   const int M = a.M, M1 = M + 1, W = 2 * a.nc + 1;
@@ -85,7 +129,7 @@ __device__ __forceinline__ double pixel_amp(const TdArgs& a, int n, int m, int d
   const double* t0 = a.T0 + (dT + a.J);
   const double* tx = a.Tx + (dT + a.J);
   double se, so;
-  band_sums(plane, a.n_rows, M1, n, m, t0, D, W, se, so);
+  layer_sums<WT>(plane, a.n_rows, M1, n, m, t0, D, W, se, so);
   if (edge && cancel_even) se = 0.0;
   if (edge && !cancel_even) so = 0.0;
   const int J4 = 4 * a.J;
@@ -93,7 +137,7 @@ __device__ __forceinline__ double pixel_amp(const TdArgs& a, int n, int m, int d
   double r = odd_n ? se * ph.y - so * ph.x : se * ph.y + so * ph.x;
   if (m > 0) {
     double le, lo;
-    band_sums(plane, a.n_rows, M1, n, m - 1, tx, D, W, le, lo);
+    layer_sums<WT>(plane, a.n_rows, M1, n, m - 1, tx, D, W, le, lo);
     if (m == 1 && cancel_even) le = 0.0;
     if (m == 1 && !cancel_even) lo = 0.0;
     const double slp = phase(a.trig, J4, (2 * m - 1) * dT).x;
@@ -103,7 +147,7 @@ __device__ __forceinline__ double pixel_amp(const TdArgs& a, int n, int m, int d
   }
   if (m < M) {
     double he, ho;
-    band_sums(plane, a.n_rows, M1, n, m + 1, tx, D, W, he, ho);
+    layer_sums<WT>(plane, a.n_rows, M1, n, m + 1, tx, D, W, he, ho);
     if (m == M - 1 && cancel_even) he = 0.0;
     if (m == M - 1 && !cancel_even) ho = 0.0;
     const double shp = phase(a.trig, J4, (2 * m + 1) * dT).x;
@@ -115,6 +159,9 @@ __device__ __forceinline__ double pixel_amp(const TdArgs& a, int n, int m, int d
   return r;
 }
 
+// WT: the filters' tap count 2 n_c + 1 when it is a compile-time case (25),
+// else 0 (runtime loop).
+template <int WT>
 __global__ __launch_bounds__(256) void k_td_vectors(TdArgs a) {
   const int half = 2 * a.K + 1;
   const int64_t total = a.n_pixels * half;
@@ -130,8 +177,8 @@ __global__ __launch_bounds__(256) void k_td_vectors(TdArgs a) {
     // odd whole-bin shifts swap the quadratures (getTDamp): a00 from the 90
     // plane, a90 from the 00 plane, one of them negated by the (n + m) parity
     const bool swap = (sh & 1) != 0;
-    double a00 = pixel_amp(a, ne, m, sub, swap ? a.p90 : a.p00, swap);
-    double a90 = pixel_amp(a, ne, m, sub, swap ? a.p00 : a.p90, !swap);
+    double a00 = pixel_amp<WT>(a, ne, m, sub, swap ? a.p90 : a.p00, swap);
+    double a90 = pixel_amp<WT>(a, ne, m, sub, swap ? a.p00 : a.p90, !swap);
     if (swap) {
       if ((n + m) & 1) a00 = -a00;
       else a90 = -a90;
@@ -168,7 +215,13 @@ int pycwb_td_vectors(const int32_t* pixel_index, int64_t n_pixels, const float* 
   const int64_t total = n_pixels * (2 * K + 1);
   int64_t grid = (total + 255) / 256;
   if (grid > 65536) grid = 65536;
-  hipLaunchKernelGGL(k_td_vectors, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, a);
+  if (2 * n_coeffs + 1 == 25) {
+    hipLaunchKernelGGL((k_td_vectors<25>), dim3((unsigned)grid), dim3(256), 0,
+                       (hipStream_t)stream, a);
+  } else {
+    hipLaunchKernelGGL((k_td_vectors<0>), dim3((unsigned)grid), dim3(256), 0,
+                       (hipStream_t)stream, a);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     pycwb::record_error("k_td_vectors", hipGetErrorString(e));
