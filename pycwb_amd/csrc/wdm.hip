@@ -6,9 +6,11 @@
 //   per time bin n (stride MM) the symmetric window fold of 2*m_H - 1 samples
 //   into 2M bins, a length-2M FFT (radix-2, in LDS), the 1/sqrt(2) edge layers
 //   and the parity-alternating 00/90 assignment times sqrt(2).
-// One workgroup transforms B consecutive time bins of one series; the folds
-// read the series straight from HBM (neighbouring bins overlap, so the
-// re-reads hit L2) and the FFTs run on LDS.
+// The stride == M levels run polyphase kernels: k_t2w_stream (M <= 128) keeps
+// each block's input window in an LDS ring and streams consecutive tiles of
+// one series through it; k_t2w_poly_h (larger M) reads each thread's window
+// straight from HBM (neighbouring bins overlap, so the re-reads hit L2).  The
+// FFTs run on LDS.
 // Output layout: (series, N, M+1) complex (map00, map90) pairs, bin-major so
 // every store is coalesced; with quadrature off, the power map in map00.
 #include <hip/hip_runtime.h>
@@ -149,10 +151,8 @@ constexpr int kPolyLdsMax = 2048;  // complex doubles of LDS per block (32 KiB):
 
 // Bins per block: B * M complex doubles of LDS; measured on MI355X (5 x 2.46 M
 // samples), 16 KiB blocks are fastest up to M = 128 and 32 KiB blocks above.
-// Levels with M up to kStagedMaxM take the LDS-staged kernel (k_t2w_poly_s);
-// measured on MI355X, staging at M = 64 and 128 (with 1,024 or 512 complex
-// doubles of FFT buffer per block) lost more occupancy than it saved.
-constexpr int kStagedMaxM = 32;
+// Levels with m_H = 12 M + 1 and M <= 128 take the streaming kernel
+// (k_t2w_stream) instead.
 
 __host__ int poly_bins(int M) {
   const int budget = M <= 128 ? 1024 : kPolyLdsMax;
@@ -335,94 +335,175 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
   }
 }
 
-// The polyphase kernel with its inputs staged in LDS (M <= 32): the block's
-// input window x[(n0 - 2 KK) M + i], i < (B + 4 KK + 1) M, is read once with
-// coalesced loads (edges mirrored as ext_sample does), the M twiddles once,
-// and the fold and the FFT read LDS only.  The fold forms the same terms in
-// the same order as k_t2w_poly_h, so the maps are identical.
-template <int R, int KK>
-__global__ __launch_bounds__(kThreads) void k_t2w_poly_s(
-    const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int M, int64_t N,
-    const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int log2_2M, int B,
+// Streaming form of the polyphase kernel (M = 2^LG, 2 <= M <= 128, KK = 6): a
+// block owns C consecutive tiles of B = 1024 / M bins of one series.  Thread
+// (k, r) folds residue k of bins r R .. r R + R - 1 of every tile (R = 8, so the
+// 256 threads cover the tile's (B / R) x 2M items once).  The input window of
+// the current tile, rows n0 - 2 KK .. n0 + B + 2 KK of M samples, sits in an
+// LDS ring of B + 4 KK + 1 rows: consecutive tiles share all but B rows, and
+// the B rows the next tile adds (1,024 samples, 4 per thread) are loaded into
+// registers while the current tile is folded, transformed and stored, then
+// written over the ring rows this tile no longer needs.  A thread's 2 KK + 1
+// filter taps stay in registers for the whole chunk.  The fold forms the same
+// terms in the same order as k_t2w_poly_h (each window sample is added to every
+// bin that uses it as soon as it is read, so each bin sums in ascending q), and
+// the split computes the same expressions, so the maps are identical.  The
+// split handles both bins of a pair from one read of Z[f] and Z[2M - f], layers
+// 1 .. M - 1 first and the two edge layers (the only ones with a division) in a
+// loop of their own.
+constexpr int kStreamR = 8;
+constexpr int kStreamMaxM = 128;
+#ifndef PYCWB_T2W_STREAM_BLOCKS
+#define PYCWB_T2W_STREAM_BLOCKS 1024   // blocks to aim for: 4 per CU of an MI355X
+#endif
+
+__host__ size_t stream_lds_bytes(int M, int KK) {
+  const int B = 1024 / M;
+  return (size_t)B * M * sizeof(double2) + (size_t)(B + 4 * KK + 1) * M * sizeof(double) +
+         (size_t)M * sizeof(double2);
+}
+
+template <int KK, int LG>
+__global__ __launch_bounds__(kThreads) void k_t2w_stream(
+    const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int64_t N,
+    const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int C,
     int quadrature, double2* __restrict__ out) {
-  extern __shared__ double2 buf[];  // B * M complex, then the window, then M twiddles
-  const int M2 = 2 * M;
+  constexpr int M = 1 << LG, M2 = 2 * M, LG2 = LG + 1;
+  constexpr int R = kStreamR;
+  constexpr int P = 4;             // new samples per thread and tile: B M / kThreads
+  constexpr int B = 1024 >> LG;    // bins per tile
+  constexpr int WR = B + 4 * KK + 1;
+  constexpr int NP = B / 2;        // bin pairs per tile
+  extern __shared__ double2 buf[];  // B M complex (FFT), the window ring, M twiddles
   const int64_t tiles = (N + B - 1) / B;
-  const int64_t blk = blockIdx.x;
-  const int s = (int)(blk / tiles);
-  const int64_t n0 = (blk % tiles) * B;
+  const int64_t chunks = (tiles + C - 1) / C;
+  const int s = (int)(blockIdx.x / chunks);
   if (s >= n_series) return;
+  const int64_t t0 = (blockIdx.x % chunks) * C;
+  const int nt = (int)((tiles - t0) < C ? (tiles - t0) : C);
   const double* x = X + (int64_t)s * ldx;
   double* lds = reinterpret_cast<double*>(buf);
-  const int W = (B + 4 * KK + 1) * M;
   double* win = lds + 2 * B * M;
-  double2* tws = reinterpret_cast<double2*>(win + W);
-  const int64_t base = (n0 - 2 * KK) * M;
-  for (int i = threadIdx.x; i < W; i += kThreads) win[i] = ext_sample(x, L, base + i);
+  double2* tws = reinterpret_cast<double2*>(win + WR * M);
+  const int64_t N0 = t0 * B;
+  const int64_t base = (N0 - 2 * KK) * M;   // sample of ring row 0, column 0 at the start
+  for (int i = threadIdx.x; i < WR * M; i += kThreads) win[i] = ext_sample(x, L, base + i);
   for (int i = threadIdx.x; i < M; i += kThreads) tws[i] = tw[i];
+  const int k = threadIdx.x & (M2 - 1), r = threadIdx.x >> LG2;
+  const int krow = k >> LG, kcol = k & (M - 1);
+  const int kr = (int)bitrev((unsigned)k, LG2);
+  double h[2 * KK + 1];
+#pragma unroll
+  for (int q = -KK; q <= KK; ++q) {
+    const int t = k + M2 * q;
+    const int at = t < 0 ? -t : t;
+    h[q + KK] = at <= m_H - 1 ? filt[at] : 0.0;
+  }
+  const double sqrt2 = 1.4142135623730951;
   __syncthreads();
-  const int items = (B / R) * M2;
-  for (int it = threadIdx.x; it < items; it += kThreads) {
-    const int k = it & (M2 - 1), r = it >> log2_2M;
-    double xs[R + 4 * KK], h[2 * KK + 1];
+  for (int t = 0; t < nt; ++t) {
+    const int64_t n0 = N0 + (int64_t)t * B;
+    const bool more = t + 1 < nt;
+    double pre[P];
+    if (more) {
+      // ring rows (t + 1) B + 4 KK + 1 .. + B - 1, relative to the chunk's first row
+      const int64_t nb = base + ((int64_t)(t + 1) * B + 4 * KK + 1) * M;
 #pragma unroll
-    for (int u = 0; u < R + 4 * KK; ++u) xs[u] = win[(r * R + u) * M + k];
-#pragma unroll
-    for (int q = -KK; q <= KK; ++q) {
-      const int t = k + M2 * q;
-      const int at = t < 0 ? -t : t;
-      h[q + KK] = at <= m_H - 1 ? filt[at] : 0.0;
+      for (int p = 0; p < P; ++p) pre[p] = ext_sample(x, L, nb + threadIdx.x + p * kThreads);
     }
+    // fold: sample u of the thread's column (ring row t B + r R + u + krow,
+    // wrapping at most once) adds h[(u - b) / 2] x to every bin b of the same
+    // parity within reach
     double acc[R];
 #pragma unroll
     for (int b = 0; b < R; ++b) acc[b] = 0.0;
+    const int sl = (t * B + r * R + krow) % WR;
+    const int lim = WR - sl;
+    const double* wa = win + sl * M + kcol;
+    const double* wb = wa - WR * M;
 #pragma unroll
-    for (int q = -KK; q <= KK; ++q) {
+    for (int u = 0; u < R + 4 * KK; ++u) {
+      const double v = (u < lim ? wa : wb)[u * M];
 #pragma unroll
-      for (int b = 0; b < R; ++b) acc[b] += h[q + KK] * xs[b + 2 * (q + KK)];
+      for (int b = 0; b < R; ++b) {
+        const int d = u - b;
+        if (d >= 0 && !(d & 1) && d <= 4 * KK) acc[b] += h[d >> 1] * v;
+      }
     }
-    const int kr = (int)bitrev((unsigned)k, log2_2M);
 #pragma unroll
     for (int b = 0; b < R; ++b) {
       const int bb = r * R + b;
       lds[2 * ((bb >> 1) * M2 + kr) + (bb & 1)] = acc[b];
     }
+    __syncthreads();
+    lds_fft(buf, NP, LG2, tws);
+    // split the pairs (bin 2p in the real part, 2p + 1 in the imaginary part);
+    // edges and 00/90 assignment (WDM.cc:1494-1510).  n0 is even, so bin 2p
+    // has even n and bin 2p + 1 odd n.
+    double2* ot = out + ((int64_t)s * N + n0) * (M + 1);
+    const int nb_valid = (int)((N - n0) < B ? (N - n0) : B);
+    for (int e = threadIdx.x; e < NP * (M - 1); e += kThreads) {
+      const int p = e / (M - 1);
+      const int f = e - p * (M - 1) + 1;
+      const double2 z = buf[p * M2 + f];
+      const double2 zc = buf[p * M2 + M2 - f];
+      const double R0 = 0.5 * (z.x + zc.x), I0 = 0.5 * (z.y - zc.y);
+      const double R1 = 0.5 * (z.y + zc.y), I1 = -0.5 * (z.x - zc.x);
+      double2 oa, ob;
+      if (!quadrature) {
+        oa = make_double2(R0 * R0 + I0 * I0, 0.0);
+        ob = make_double2(R1 * R1 + I1 * I1, 0.0);
+      } else if (f & 1) {
+        oa = make_double2(sqrt2 * I0, sqrt2 * R0);
+        ob = make_double2(sqrt2 * R1, -sqrt2 * I1);
+      } else {
+        oa = make_double2(sqrt2 * R0, -sqrt2 * I0);
+        ob = make_double2(sqrt2 * I1, sqrt2 * R1);
+      }
+      if (2 * p < nb_valid) ot[(2 * p) * (M + 1) + f] = oa;
+      if (2 * p + 1 < nb_valid) ot[(2 * p + 1) * (M + 1) + f] = ob;
+    }
+    for (int e = threadIdx.x; e < NP * 2; e += kThreads) {
+      const int p = e >> 1, f = (e & 1) ? M : 0;
+      const double2 z = buf[p * M2 + f];   // Z[2M - f] is Z[f] for f = 0, M
+      double R0 = 0.5 * (z.x + z.x), R1 = 0.5 * (z.y + z.y);
+      R0 = R0 / sqrt2;
+      R1 = R1 / sqrt2;
+      const double I0 = R0, I1 = R1;
+      double2 oa, ob;
+      if (!quadrature) {
+        oa = make_double2(R0 * R0 + I0 * I0, 0.0);
+        ob = make_double2(R1 * R1 + I1 * I1, 0.0);
+      } else {
+        oa = make_double2(sqrt2 * R0, -sqrt2 * I0);
+        ob = make_double2(sqrt2 * I1, sqrt2 * R1);
+      }
+      if (2 * p < nb_valid) ot[(2 * p) * (M + 1) + f] = oa;
+      if (2 * p + 1 < nb_valid) ot[(2 * p + 1) * (M + 1) + f] = ob;
+    }
+    if (more) {
+      // the new rows replace ring rows t B .. t B + B - 1, which only this
+      // tile's fold read (finished before the FFT's first barrier)
+      const int rel0 = ((t + 1) * B + 4 * KK + 1) % WR;
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int i = threadIdx.x + p * kThreads;
+        int row = rel0 + (i >> LG);
+        row = row >= WR ? row - WR : row;
+        win[row * M + (i & (M - 1))] = pre[p];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  lds_fft(buf, B / 2, log2_2M, tws);
-  const double sqrt2 = 1.4142135623730951;
-  const unsigned M1 = (unsigned)M + 1u;
-  const unsigned magic = 0xffffffffu / M1 + 1u;
-  for (int e = threadIdx.x; e < B * (M + 1); e += kThreads) {
-    const int b = (int)__umulhi((unsigned)e, magic);
-    const int f = e - b * (int)M1;
-    const int64_t n = n0 + b;
-    if (n >= N) continue;
-    const int p = b >> 1;
-    const double2 z = buf[p * M2 + f];
-    const double2 zc = buf[p * M2 + ((M2 - f) & (M2 - 1))];
-    double R_, I_;
-    if ((b & 1) == 0) {
-      R_ = 0.5 * (z.x + zc.x);
-      I_ = 0.5 * (z.y - zc.y);
-    } else {
-      R_ = 0.5 * (z.y + zc.y);
-      I_ = -0.5 * (z.x - zc.x);
-    }
-    if (f == 0 || f == M) {
-      R_ = R_ / sqrt2;
-      I_ = R_;
-    }
-    double2 o;
-    if (!quadrature) {
-      o = make_double2(R_ * R_ + I_ * I_, 0.0);
-    } else if ((f + (int)(n & 1)) & 1) {
-      o = make_double2(sqrt2 * I_, sqrt2 * R_);
-    } else {
-      o = make_double2(sqrt2 * R_, -sqrt2 * I_);
-    }
-    out[((int64_t)s * N + n) * (M + 1) + f] = o;
-  }
+}
+
+template <int LG>
+void launch_stream(unsigned grid, hipStream_t st, const double* x, int64_t L, int64_t ldx,
+                   int n_series, int64_t N, const double* filt, int m_H, const double2* tw,
+                   int C, int quadrature, double2* out) {
+  hipLaunchKernelGGL((k_t2w_stream<6, LG>), dim3(grid), dim3(kThreads),
+                     stream_lds_bytes(1 << LG, 6), st, x, L, ldx, n_series, N, filt, m_H, tw, C,
+                     quadrature, out);
 }
 
 template <int R>
@@ -430,12 +511,7 @@ void launch_poly(dim3 grid, hipStream_t st, const double* x, int64_t L, int64_t 
                  int n_series, int M, int64_t N, const double* filt, int m_H, const double2* tw,
                  int log2_2M, int B, int quadrature, double2* out) {
   const size_t lds = (size_t)B * M * sizeof(double2);
-  // staged form: FFT buffer + input window + twiddles
-  const size_t lds_s = lds + (size_t)(B + 4 * 6 + 1) * M * sizeof(double) + M * sizeof(double2);
-  if ((m_H - 1) / (2 * M) == 6 && M <= kStagedMaxM) {
-    hipLaunchKernelGGL((k_t2w_poly_s<R, 6>), grid, dim3(kThreads), lds_s, st, x, L, ldx,
-                       n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
-  } else if ((m_H - 1) / (2 * M) == 6) {
+  if ((m_H - 1) / (2 * M) == 6) {
     hipLaunchKernelGGL((k_t2w_poly_h<R, 6>), grid, dim3(kThreads), lds + M * sizeof(double2),
                        st, x, L, ldx,
                        n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
@@ -548,6 +624,36 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
   const int64_t N = padded / stride;
   int log2_2M = 0;
   while ((1 << log2_2M) < 2 * M) ++log2_2M;
+  if (stride == M && (m_H - 1) == 12 * M && M >= 2 && M <= kStreamMaxM) {
+    // streaming polyphase path: tiles of 1024 / M bins, C consecutive tiles per block
+    const int B = 1024 / M;
+    const int64_t tiles = (N + B - 1) / B;
+    int64_t C = (tiles * n_series + PYCWB_T2W_STREAM_BLOCKS - 1) / PYCWB_T2W_STREAM_BLOCKS;
+    if (C < 1) C = 1;
+    if (C > 1 << 20) C = 1 << 20;
+    const int64_t chunks = (tiles + C - 1) / C;
+    const int64_t grid = chunks * n_series;
+    if (grid > 0x7fffffff) return PYCWB_ERR_ARG;
+    const hipStream_t st = (hipStream_t)stream;
+    const double2* tw = (const double2*)twiddle;
+    double2* o = (double2*)out;
+    const unsigned g = (unsigned)grid;
+    switch (log2_2M - 1) {
+      case 1: launch_stream<1>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 2: launch_stream<2>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 3: launch_stream<3>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 4: launch_stream<4>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 5: launch_stream<5>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 6: launch_stream<6>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      default: launch_stream<7>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      pycwb::record_error("k_t2w_stream", hipGetErrorString(e));
+      return PYCWB_ERR_HIP;
+    }
+    return PYCWB_OK;
+  }
   if (stride == M && (m_H - 1) % (2 * M) == 0 && M >= 2 && M <= kPolyLdsMax / 2) {
     // polyphase path: B bins per block (even, >= 2), R bins per thread
     const int B = poly_bins(M);
