@@ -6,11 +6,11 @@
 //   per time bin n (stride MM) the symmetric window fold of 2*m_H - 1 samples
 //   into 2M bins, a length-2M FFT (radix-2, in LDS), the 1/sqrt(2) edge layers
 //   and the parity-alternating 00/90 assignment times sqrt(2).
-// The stride == M levels run polyphase kernels: k_t2w_stream (M <= 128) keeps
-// each block's input window in an LDS ring and streams consecutive tiles of
-// one series through it; k_t2w_poly_h (larger M) reads each thread's window
-// straight from HBM (neighbouring bins overlap, so the re-reads hit L2).  The
-// FFTs run on LDS.
+// The stride == M levels run polyphase kernels: k_t2w_stream (M <= 128 and
+// M = 512) keeps each block's input window in an LDS ring and streams
+// consecutive tiles of one series through it; k_t2w_poly_h (the other M) reads
+// each thread's window straight from HBM (neighbouring bins overlap, so the
+// re-reads hit L2).  The FFTs run on LDS.
 // Output layout: (series, N, M+1) complex (map00, map90) pairs, bin-major so
 // every store is coalesced; with quadrature off, the power map in map00.
 #include <hip/hip_runtime.h>
@@ -151,8 +151,8 @@ constexpr int kPolyLdsMax = 2048;  // complex doubles of LDS per block (32 KiB):
 
 // Bins per block: B * M complex doubles of LDS; measured on MI355X (5 x 2.46 M
 // samples), 16 KiB blocks are fastest up to M = 128 and 32 KiB blocks above.
-// Levels with m_H = 12 M + 1 and M <= 128 take the streaming kernel
-// (k_t2w_stream) instead.
+// Levels with m_H = 12 M + 1 and M <= 128 or M = 512 take the streaming
+// kernel (k_t2w_stream) instead.
 
 __host__ int poly_bins(int M) {
   const int budget = M <= 128 ? 1024 : kPolyLdsMax;
@@ -335,45 +335,126 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
   }
 }
 
-// Streaming form of the polyphase kernel (M = 2^LG, 2 <= M <= 128, KK = 6): a
-// block owns C consecutive tiles of B = 1024 / M bins of one series.  Thread
-// (k, r) folds residue k of bins r R .. r R + R - 1 of every tile (R = 8, so the
-// 256 threads cover the tile's (B / R) x 2M items once).  The input window of
-// the current tile, rows n0 - 2 KK .. n0 + B + 2 KK of M samples, sits in an
-// LDS ring of B + 4 KK + 1 rows: consecutive tiles share all but B rows, and
-// the B rows the next tile adds (1,024 samples, 4 per thread) are loaded into
+// Streaming form of the polyphase kernel (M = 2^LG, KK = 6): a block of THR
+// threads owns C consecutive tiles of B = THR R / 2M bins of one series.
+// Thread (k, r) folds residue k of bins r R .. r R + R - 1 of every tile, so
+// the block covers the tile's (B / R) x 2M items once.  The input window of the
+// current tile, rows n0 - 2 KK .. n0 + B + 2 KK of M samples, sits in an LDS
+// ring of B + 4 KK + 1 rows: consecutive tiles share all but B rows, and the B
+// rows the next tile adds (B M / THR samples per thread) are loaded into
 // registers while the current tile is folded, transformed and stored, then
 // written over the ring rows this tile no longer needs.  A thread's 2 KK + 1
-// filter taps stay in registers for the whole chunk.  The fold forms the same
-// terms in the same order as k_t2w_poly_h (each window sample is added to every
-// bin that uses it as soon as it is read, so each bin sums in ascending q), and
-// the split computes the same expressions, so the maps are identical.  The
-// split handles both bins of a pair from one read of Z[f] and Z[2M - f], layers
-// 1 .. M - 1 first and the two edge layers (the only ones with a division) in a
-// loop of their own.
-constexpr int kStreamR = 8;
-constexpr int kStreamMaxM = 128;
+// filter taps stay in registers for the whole chunk.  Two shapes:
+//   M <= 128:  THR = 256, R = 8 (tiles of 1,024 samples); the FFT twiddles of a
+//              thread's butterflies are kept in registers too (stream_fft);
+//   M = 512:   THR = 1024, R = 4 (tiles of 4 bins, 156 KiB of LDS, one block
+//              per CU); the FFT is lds_fft with its twiddles in LDS.
+// The fold forms the same terms in the same order as k_t2w_poly_h (each window
+// sample is added to every bin that uses it as soon as it is read, so each bin
+// sums in ascending q), the FFT runs the same stages as lds_fft and the split
+// computes the same expressions, so the maps are identical.  The split handles
+// both bins of a pair from one read of Z[f] and Z[2M - f], layers 1 .. M - 1
+// first and the two edge layers (the only ones with a division) in a loop of
+// their own.
+constexpr int kStreamMaxM = 128;      // the 256-thread shape
+constexpr int kStreamWideM = 512;     // the 1024-thread shape
 #ifndef PYCWB_T2W_STREAM_BLOCKS
-#define PYCWB_T2W_STREAM_BLOCKS 1024   // blocks to aim for: 4 per CU of an MI355X
+#define PYCWB_T2W_STREAM_BLOCKS 1024   // 256-thread blocks to aim for: 4 per CU of an MI355X
+                                       // (a quarter as many 1024-thread blocks)
 #endif
 
-__host__ size_t stream_lds_bytes(int M, int KK) {
-  const int B = 1024 / M;
+__host__ size_t stream_lds_bytes(int M, int KK, int thr, int R) {
+  const int B = thr * R / (2 * M);
   return (size_t)B * M * sizeof(double2) + (size_t)(B + 4 * KK + 1) * M * sizeof(double) +
-         (size_t)M * sizeof(double2);
+         (thr == kThreads ? 0 : (size_t)M * sizeof(double2));
 }
 
-template <int KK, int LG>
-__global__ __launch_bounds__(kThreads) void k_t2w_stream(
+// The FFT of the streaming kernel: NP = 2^(9 - LG) sequences of 2M = 2^LG2
+// points (LG2 = LG + 1), the same radix-2 DIT stages in the same pairing and
+// order as lds_fft, with the index arithmetic fixed at compile time.  With 256
+// threads every pass is exactly one radix-4 butterfly per thread (the last
+// radix-2 stage, for odd LG2, two), and a thread's butterfly sits at the same
+// place in every tile, so its twiddles are read once per block into registers
+// (LDS then carries only the data: a quarter less traffic per pass).
+template <int LG2>
+struct StreamTw {
+  static constexpr int NR4 = LG2 / 2;   // radix-4 passes
+  double2 w[NR4 > 0 ? NR4 : 1][3];
+  double2 wl;                            // the radix-2 stage's twiddle (odd LG2)
+};
+
+template <int LG2>
+__device__ __forceinline__ void stream_tw_load(StreamTw<LG2>& T, const double2* __restrict__ tw) {
+  const int e = threadIdx.x;
+#pragma unroll
+  for (int ps = 0; ps < StreamTw<LG2>::NR4; ++ps) {
+    const int lh = 2 * ps;
+    const int h = 1 << lh;
+    const int j = (e & ((1 << (LG2 - 2)) - 1)) & (h - 1);
+    T.w[ps][0] = tw[j << (LG2 - 1 - lh)];
+    T.w[ps][1] = tw[j << (LG2 - 2 - lh)];
+    T.w[ps][2] = tw[(j + h) << (LG2 - 2 - lh)];
+  }
+  if (LG2 & 1) T.wl = tw[e & ((1 << (LG2 - 1)) - 1)];   // 256 is a multiple of M
+}
+
+template <int LG2>
+__device__ __forceinline__ void stream_fft(double2* __restrict__ buf, const StreamTw<LG2>& T) {
+  const int e = threadIdx.x;
+#pragma unroll
+  for (int ps = 0; ps < StreamTw<LG2>::NR4; ++ps) {
+    constexpr int qpl = LG2 - 2;
+    const int lh = 2 * ps;
+    const int h = 1 << lh;
+    const int p = e >> qpl, r = e & ((1 << qpl) - 1);
+    const int g = r >> lh, j = r & (h - 1);
+    const int i0 = (p << LG2) + (g << (lh + 2)) + j;
+    const double2 w1 = T.w[ps][0], w2a = T.w[ps][1], w2b = T.w[ps][2];
+    const double2 x0 = buf[i0], x1 = buf[i0 + h], x2 = buf[i0 + 2 * h], x3 = buf[i0 + 3 * h];
+    double vr = x1.x * w1.x - x1.y * w1.y, vi = x1.x * w1.y + x1.y * w1.x;
+    const double2 a0 = make_double2(x0.x + vr, x0.y + vi), a1 = make_double2(x0.x - vr, x0.y - vi);
+    vr = x3.x * w1.x - x3.y * w1.y;
+    vi = x3.x * w1.y + x3.y * w1.x;
+    const double2 a2 = make_double2(x2.x + vr, x2.y + vi), a3 = make_double2(x2.x - vr, x2.y - vi);
+    vr = a2.x * w2a.x - a2.y * w2a.y;
+    vi = a2.x * w2a.y + a2.y * w2a.x;
+    buf[i0] = make_double2(a0.x + vr, a0.y + vi);
+    buf[i0 + 2 * h] = make_double2(a0.x - vr, a0.y - vi);
+    vr = a3.x * w2b.x - a3.y * w2b.y;
+    vi = a3.x * w2b.y + a3.y * w2b.x;
+    buf[i0 + h] = make_double2(a1.x + vr, a1.y + vi);
+    buf[i0 + 3 * h] = make_double2(a1.x - vr, a1.y - vi);
+    __syncthreads();
+  }
+  if constexpr (LG2 & 1) {
+    constexpr int h = 1 << (LG2 - 1);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ee = e + q * kThreads;
+      const int p = ee >> (LG2 - 1), j = ee & (h - 1);
+      const int i0 = (p << LG2) + j, i1 = i0 + h;
+      const double2 w = T.wl;
+      const double2 u = buf[i0], v = buf[i1];
+      const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
+      buf[i0] = make_double2(u.x + vr, u.y + vi);
+      buf[i1] = make_double2(u.x - vr, u.y - vi);
+    }
+    __syncthreads();
+  }
+}
+
+template <int KK, int LG, int R, int THR>
+__global__ __launch_bounds__(THR) void k_t2w_stream(
     const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int64_t N,
     const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int C,
     int quadrature, double2* __restrict__ out) {
   constexpr int M = 1 << LG, M2 = 2 * M, LG2 = LG + 1;
-  constexpr int R = kStreamR;
-  constexpr int P = 4;             // new samples per thread and tile: B M / kThreads
-  constexpr int B = 1024 >> LG;    // bins per tile
+  constexpr int B = THR * R / M2;  // bins per tile
+  constexpr int P = B * M / THR;   // new samples per thread and tile
   constexpr int WR = B + 4 * KK + 1;
   constexpr int NP = B / 2;        // bin pairs per tile
+  constexpr bool kRegTw = THR == kThreads;
+  static_assert(B >= R && B % 2 == 0 && P * THR == B * M, "tile shape");
   extern __shared__ double2 buf[];  // B M complex (FFT), the window ring, M twiddles
   const int64_t tiles = (N + B - 1) / B;
   const int64_t chunks = (tiles + C - 1) / C;
@@ -384,11 +465,13 @@ __global__ __launch_bounds__(kThreads) void k_t2w_stream(
   const double* x = X + (int64_t)s * ldx;
   double* lds = reinterpret_cast<double*>(buf);
   double* win = lds + 2 * B * M;
-  double2* tws = reinterpret_cast<double2*>(win + WR * M);
+  double2* tws = reinterpret_cast<double2*>(win + WR * M);   // kRegTw false only
   const int64_t N0 = t0 * B;
   const int64_t base = (N0 - 2 * KK) * M;   // sample of ring row 0, column 0 at the start
-  for (int i = threadIdx.x; i < WR * M; i += kThreads) win[i] = ext_sample(x, L, base + i);
-  for (int i = threadIdx.x; i < M; i += kThreads) tws[i] = tw[i];
+  for (int i = threadIdx.x; i < WR * M; i += THR) win[i] = ext_sample(x, L, base + i);
+  if constexpr (!kRegTw) {
+    for (int i = threadIdx.x; i < M; i += THR) tws[i] = tw[i];
+  }
   const int k = threadIdx.x & (M2 - 1), r = threadIdx.x >> LG2;
   const int krow = k >> LG, kcol = k & (M - 1);
   const int kr = (int)bitrev((unsigned)k, LG2);
@@ -400,6 +483,8 @@ __global__ __launch_bounds__(kThreads) void k_t2w_stream(
     h[q + KK] = at <= m_H - 1 ? filt[at] : 0.0;
   }
   const double sqrt2 = 1.4142135623730951;
+  StreamTw<LG2> T;
+  if constexpr (kRegTw) stream_tw_load<LG2>(T, tw);
   __syncthreads();
   for (int t = 0; t < nt; ++t) {
     const int64_t n0 = N0 + (int64_t)t * B;
@@ -409,7 +494,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_stream(
       // ring rows (t + 1) B + 4 KK + 1 .. + B - 1, relative to the chunk's first row
       const int64_t nb = base + ((int64_t)(t + 1) * B + 4 * KK + 1) * M;
 #pragma unroll
-      for (int p = 0; p < P; ++p) pre[p] = ext_sample(x, L, nb + threadIdx.x + p * kThreads);
+      for (int p = 0; p < P; ++p) pre[p] = ext_sample(x, L, nb + threadIdx.x + p * THR);
     }
     // fold: sample u of the thread's column (ring row t B + r R + u + krow,
     // wrapping at most once) adds h[(u - b) / 2] x to every bin b of the same
@@ -436,13 +521,14 @@ __global__ __launch_bounds__(kThreads) void k_t2w_stream(
       lds[2 * ((bb >> 1) * M2 + kr) + (bb & 1)] = acc[b];
     }
     __syncthreads();
-    lds_fft(buf, NP, LG2, tws);
+    if constexpr (kRegTw) stream_fft<LG2>(buf, T);
+    else lds_fft(buf, NP, LG2, tws);
     // split the pairs (bin 2p in the real part, 2p + 1 in the imaginary part);
     // edges and 00/90 assignment (WDM.cc:1494-1510).  n0 is even, so bin 2p
     // has even n and bin 2p + 1 odd n.
     double2* ot = out + ((int64_t)s * N + n0) * (M + 1);
     const int nb_valid = (int)((N - n0) < B ? (N - n0) : B);
-    for (int e = threadIdx.x; e < NP * (M - 1); e += kThreads) {
+    for (int e = threadIdx.x; e < NP * (M - 1); e += THR) {
       const int p = e / (M - 1);
       const int f = e - p * (M - 1) + 1;
       const double2 z = buf[p * M2 + f];
@@ -463,7 +549,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_stream(
       if (2 * p < nb_valid) ot[(2 * p) * (M + 1) + f] = oa;
       if (2 * p + 1 < nb_valid) ot[(2 * p + 1) * (M + 1) + f] = ob;
     }
-    for (int e = threadIdx.x; e < NP * 2; e += kThreads) {
+    for (int e = threadIdx.x; e < NP * 2; e += THR) {
       const int p = e >> 1, f = (e & 1) ? M : 0;
       const double2 z = buf[p * M2 + f];   // Z[2M - f] is Z[f] for f = 0, M
       double R0 = 0.5 * (z.x + z.x), R1 = 0.5 * (z.y + z.y);
@@ -487,7 +573,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_stream(
       const int rel0 = ((t + 1) * B + 4 * KK + 1) % WR;
 #pragma unroll
       for (int p = 0; p < P; ++p) {
-        const int i = threadIdx.x + p * kThreads;
+        const int i = threadIdx.x + p * THR;
         int row = rel0 + (i >> LG);
         row = row >= WR ? row - WR : row;
         win[row * M + (i & (M - 1))] = pre[p];
@@ -497,13 +583,27 @@ __global__ __launch_bounds__(kThreads) void k_t2w_stream(
   }
 }
 
-template <int LG>
+template <int LG, int R, int THR>
 void launch_stream(unsigned grid, hipStream_t st, const double* x, int64_t L, int64_t ldx,
                    int n_series, int64_t N, const double* filt, int m_H, const double2* tw,
                    int C, int quadrature, double2* out) {
-  hipLaunchKernelGGL((k_t2w_stream<6, LG>), dim3(grid), dim3(kThreads),
-                     stream_lds_bytes(1 << LG, 6), st, x, L, ldx, n_series, N, filt, m_H, tw, C,
-                     quadrature, out);
+  hipLaunchKernelGGL((k_t2w_stream<6, LG, R, THR>), dim3(grid), dim3(THR),
+                     stream_lds_bytes(1 << LG, 6, THR, R), st, x, L, ldx, n_series, N, filt, m_H,
+                     tw, C, quadrature, out);
+}
+
+// The 1024-thread shape needs more than 64 KiB of dynamic LDS: allowed once per
+// process; false if the device refuses (the level then takes k_t2w_poly_h).
+bool stream_wide_ready() {
+  static const bool ok = [] {
+    const size_t need = stream_lds_bytes(kStreamWideM, 6, 1024, 4);
+    const bool set = hipFuncSetAttribute((const void*)k_t2w_stream<6, 9, 4, 1024>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)need) == hipSuccess;
+    (void)hipGetLastError();
+    return set;
+  }();
+  return ok;
 }
 
 template <int R>
@@ -624,11 +724,14 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
   const int64_t N = padded / stride;
   int log2_2M = 0;
   while ((1 << log2_2M) < 2 * M) ++log2_2M;
-  if (stride == M && (m_H - 1) == 12 * M && M >= 2 && M <= kStreamMaxM) {
-    // streaming polyphase path: tiles of 1024 / M bins, C consecutive tiles per block
-    const int B = 1024 / M;
+  const bool wide = stride == M && (m_H - 1) == 12 * M && M == kStreamWideM && stream_wide_ready();
+  if ((stride == M && (m_H - 1) == 12 * M && M >= 2 && M <= kStreamMaxM) || wide) {
+    // streaming polyphase path: tiles of B bins (1024 / M, or 4 at M = 512), C
+    // consecutive tiles per block
+    const int B = wide ? 4 : 1024 / M;
     const int64_t tiles = (N + B - 1) / B;
-    int64_t C = (tiles * n_series + PYCWB_T2W_STREAM_BLOCKS - 1) / PYCWB_T2W_STREAM_BLOCKS;
+    const int64_t target = wide ? PYCWB_T2W_STREAM_BLOCKS / 4 : PYCWB_T2W_STREAM_BLOCKS;
+    int64_t C = (tiles * n_series + target - 1) / target;
     if (C < 1) C = 1;
     if (C > 1 << 20) C = 1 << 20;
     const int64_t chunks = (tiles + C - 1) / C;
@@ -639,13 +742,14 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
     double2* o = (double2*)out;
     const unsigned g = (unsigned)grid;
     switch (log2_2M - 1) {
-      case 1: launch_stream<1>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 2: launch_stream<2>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 3: launch_stream<3>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 4: launch_stream<4>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 5: launch_stream<5>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 6: launch_stream<6>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      default: launch_stream<7>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 1: launch_stream<1, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 2: launch_stream<2, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 3: launch_stream<3, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 4: launch_stream<4, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 5: launch_stream<5, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 6: launch_stream<6, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 7: launch_stream<7, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      default: launch_stream<9, 4, 1024>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -101,8 +101,12 @@ def test_oracle_power_map_matches_quadrature(oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("M,L,S", [(2, 3001, 2), (8, 8192, 3), (16, 5000, 1), (64, 20000, 2),
-                                   (256, 65536, 3), (1024, 200000, 1)])
+@pytest.mark.parametrize("M,L,S", [(2, 3001, 2), (4, 4003, 2), (8, 8192, 3), (16, 5000, 1),
+                                   (32, 30000, 2), (64, 20000, 2), (128, 70001, 2),
+                                   (256, 65536, 3), (512, 150000, 2), (1024, 200000, 1),
+                                   # long enough for blocks that stream several tiles
+                                   # through one window ring
+                                   (16, 2000003, 1), (512, 1100000, 1)])
 def test_gpu_t2w_matches_oracle(oracle, M, L, S):
     import torch
 
