@@ -708,6 +708,68 @@ __global__ __launch_bounds__(kThreads) void k_w2t_gather(
   out[(int64_t)s * n_out + t] = acc;
 }
 
+// Polyphase form of k_w2t_gather for m_H = 12 M + 1.  Output t = j M + k
+// (k < M) sums, over the bins n = j - d with |d M + k| <= 12 M and 0 <= n < N,
+// in ascending n, h[|d M + k|] r_n[(M (n & 1) + d M + k) mod 2M], and that
+// index is k + M (j & 1) for every term.  So a thread owns residue k and the R
+// outputs j = jb + 2i of one parity (i < R): all of them read column k + M (j &
+// 1) of the bins jb - 12 .. jb + 2R + 10, which the thread loads once (2R + 23
+// values, consecutive threads reading consecutive k), with its 25 taps
+// h[|d M + k|] loaded once too.  Each value is added to every output that uses
+// it as it arrives, in ascending n per output and skipping the same terms, so
+// the samples are identical to k_w2t_gather's.
+template <int R>
+__global__ __launch_bounds__(kThreads) void k_w2t_poly(
+    const double* __restrict__ work, int64_t N, int M, int log2_M,
+    const double* __restrict__ filt, int m_H, int64_t n_out, double* __restrict__ out) {
+  constexpr int D = 12;   // m_H - 1 = D M
+  const int s = blockIdx.y;
+  const int64_t gi = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int k = (int)(gi & (M - 1));
+  const int p = (int)((gi >> log2_M) & 1);
+  const int64_t g = gi >> (log2_M + 1);
+  const int64_t jb = 2 * R * g + p;
+  const int64_t J = (n_out + M - 1) >> log2_M;
+  if (jb >= J) return;
+  const int M2 = 2 * M;
+  // tap e = u - 2i of output i is d = D - e: h[|(D - e) M + k|] when < m_H
+  double tap[2 * D + 1];
+  unsigned tmask = 0;
+#pragma unroll
+  for (int e = 0; e <= 2 * D; ++e) {
+    const int t = (D - e) * M + k;
+    const int at = t < 0 ? -t : t;
+    const bool v = at < m_H;
+    tap[e] = v ? filt[at] : 0.0;
+    tmask |= (unsigned)v << e;
+  }
+  const double* w = work + (int64_t)s * N * M2 + k + M * p;
+  double acc[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) acc[i] = 0.0;
+#pragma unroll
+  for (int u = 0; u < 2 * (R - 1) + 2 * D + 1; ++u) {
+    const int64_t n = jb - D + u;
+    const bool nv = n >= 0 && n < N;
+    const double v = w[(nv ? n : 0) * M2];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int e = u - 2 * i;
+      if (e >= 0 && e <= 2 * D) {
+        const bool take = nv && ((tmask >> e) & 1u);
+        const double a = acc[i] + tap[e] * v;
+        acc[i] = take ? a : acc[i];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int64_t t = ((jb + 2 * i) << log2_M) + k;
+    if (jb + 2 * i < J && t < n_out) out[(int64_t)s * n_out + t] = acc[i];
+  }
+}
+constexpr int kW2tPolyR = 8;
+
 }  // namespace
 
 extern "C" {
@@ -829,9 +891,19 @@ int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const doubl
     pycwb::record_error("k_w2t_bins", hipGetErrorString(e));
     return PYCWB_ERR_HIP;
   }
-  hipLaunchKernelGGL(k_w2t_gather,
-                     dim3((unsigned)((n_out + kThreads - 1) / kThreads), (unsigned)n_series),
-                     dim3(kThreads), 0, st, work, N, M, log2_2M - 1, filter, m_H, n_out, out);
+  if (m_H - 1 == 12 * M) {
+    // polyphase gather: (ceil(J / 2R) groups) x 2 parities x M residues per series
+    const int64_t J = (n_out + M - 1) / M;
+    const int64_t threads = ((J + 2 * kW2tPolyR - 1) / (2 * kW2tPolyR)) * 2 * M;
+    const int64_t blocks = (threads + kThreads - 1) / kThreads;
+    if (blocks > 0x7fffffff) return PYCWB_ERR_ARG;
+    hipLaunchKernelGGL((k_w2t_poly<kW2tPolyR>), dim3((unsigned)blocks, (unsigned)n_series),
+                       dim3(kThreads), 0, st, work, N, M, log2_2M - 1, filter, m_H, n_out, out);
+  } else {
+    hipLaunchKernelGGL(k_w2t_gather,
+                       dim3((unsigned)((n_out + kThreads - 1) / kThreads), (unsigned)n_series),
+                       dim3(kThreads), 0, st, work, N, M, log2_2M - 1, filter, m_H, n_out, out);
+  }
   e = hipGetLastError();
   if (e != hipSuccess) {
     pycwb::record_error("k_w2t_gather", hipGetErrorString(e));
