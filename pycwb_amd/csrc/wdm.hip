@@ -708,6 +708,114 @@ __global__ __launch_bounds__(kThreads) void k_w2t_gather(
   out[(int64_t)s * n_out + t] = acc;
 }
 
+// Streaming form of k_w2t_bins for M = 2^LG <= 128: a block of 256 threads owns
+// C consecutive tiles of B = 1024 / M bins (B / 2 pairs, 1,024 complex points)
+// of one series.  Thread e handles the four (pair, slot) items e + 256 i of a
+// tile; the coefficients the next tile needs (one phase of two bins per item)
+// are loaded into registers while the current tile is transformed and stored,
+// and the FFT twiddles stay in registers (stream_fft).  The half spectrum, the
+// FFT stages and the outputs are those of k_w2t_bins, so the work rows are
+// identical.
+template <int LG>
+__global__ __launch_bounds__(kThreads) void k_w2t_stream(
+    const double2* __restrict__ map, int64_t N, int n_series, const double2* __restrict__ tw,
+    int C, int quad, double* __restrict__ work) {
+  constexpr int M = 1 << LG, M2 = 2 * M, LG2 = LG + 1;
+  constexpr int B = 1024 >> LG;
+  constexpr int NP = B / 2;
+  constexpr int I = NP * M2 / kThreads;   // items per thread: 4
+  static_assert(I * kThreads == NP * M2, "tile shape");
+  extern __shared__ double2 buf[];
+  const int64_t tiles = (N + B - 1) / B;
+  const int64_t chunks = (tiles + C - 1) / C;
+  const int s = (int)(blockIdx.x / chunks);
+  if (s >= n_series) return;
+  const int64_t t0 = (blockIdx.x % chunks) * C;
+  const int nt = (int)((tiles - t0) < C ? (tiles - t0) : C);
+  const double* mp = reinterpret_cast<const double*>(map + (int64_t)s * N * (M + 1)) + (quad ? 1 : 0);
+  const double r2 = 1.4142135623730951;
+  StreamTw<LG2> T;
+  stream_tw_load<LG2>(T, tw);
+  // the spectrum slot of item i: k = e & (2M - 1), j = min(k, 2M - k)
+  const int k = threadIdx.x & (M2 - 1);
+  const int j = k <= M ? k : M2 - k;
+  const int kr = (int)bitrev((unsigned)k, LG2);
+  double cv[I][2];
+  auto load = [&](int64_t n0, double (&c)[I][2]) {
+#pragma unroll
+    for (int i = 0; i < I; ++i) {
+      const int p = (threadIdx.x + i * kThreads) >> LG2;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int64_t n = n0 + 2 * p + h;
+        c[i][h] = n < N ? mp[2 * (n * (M + 1) + j)] : 0.0;
+      }
+    }
+  };
+  load(t0 * B, cv);
+  for (int t = 0; t < nt; ++t) {
+    const int64_t n0 = (t0 + t) * B;
+    double nx[I][2];
+    const bool more = t + 1 < nt;
+    if (more) load(n0 + B, nx);
+#pragma unroll
+    for (int i = 0; i < I; ++i) {
+      const int p = (threadIdx.x + i * kThreads) >> LG2;
+      double wr[2], wi[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int64_t n = n0 + 2 * p + h;
+        wr[h] = wi[h] = 0.0;
+        if (n >= N) continue;
+        const double c = cv[i][h];
+        double re = 0.0, im = 0.0;
+        if (j == 0) {
+          if (quad ? (n & 1) : !(n & 1)) re = c;
+        } else if (j == M) {
+          if (quad ? ((n + M) & 1) : !((n + M) & 1)) re = (!quad && (M & 1)) ? -c : c;
+        } else if (!quad) {
+          if ((n + j) & 1) im = c / r2;
+          else if (j & 1) re = -c / r2;
+          else re = c / r2;
+        } else {
+          if ((n + j) & 1) re = c / r2;
+          else if (j & 1) im = c / r2;
+          else im = -c / r2;
+        }
+        wr[h] = re;
+        wi[h] = (k <= M) ? -im : im;
+        if (k == 0 || k == M) wi[h] = 0.0;
+      }
+      buf[p * M2 + kr] = make_double2(wr[0] - wi[1], wi[0] + wr[1]);
+    }
+    __syncthreads();
+    stream_fft<LG2>(buf, T);
+    double* wk = work + ((int64_t)s * N + n0) * M2;
+#pragma unroll
+    for (int i = 0; i < I; ++i) {
+      const int p = (threadIdx.x + i * kThreads) >> LG2;
+      const double2 z = buf[p * M2 + k];
+      if (n0 + 2 * p < N) wk[(2 * p) * M2 + k] = z.x;
+      if (n0 + 2 * p + 1 < N) wk[(2 * p + 1) * M2 + k] = z.y;
+    }
+    if (more) {
+#pragma unroll
+      for (int i = 0; i < I; ++i) {
+        cv[i][0] = nx[i][0];
+        cv[i][1] = nx[i][1];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int LG>
+void launch_w2t_stream(unsigned grid, hipStream_t st, const double2* map, int64_t N, int n_series,
+                       const double2* tw, int C, int quad, double* work) {
+  hipLaunchKernelGGL((k_w2t_stream<LG>), dim3(grid), dim3(kThreads),
+                     (size_t)1024 * sizeof(double2), st, map, N, n_series, tw, C, quad, work);
+}
+
 // Polyphase form of k_w2t_gather for m_H = 12 M + 1.  Output t = j M + k
 // (k < M) sums, over the bins n = j - d with |d M + k| <= 12 M and 0 <= n < N,
 // in ascending n, h[|d M + k|] r_n[(M (n & 1) + d M + k) mod 2M], and that
@@ -883,9 +991,32 @@ int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const doubl
   if (grid > 0x7fffffff || n_out / kThreads > 0x7fffffff || n_series > 65535)
     return PYCWB_ERR_ARG;
   const hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_w2t_bins, dim3((unsigned)grid), dim3(kThreads),
-                     (size_t)(B / 2) * 2 * M * sizeof(double2), st, (const double2*)map, N,
-                     n_series, M, (const double2*)twiddle, log2_2M, B, quad, work);
+  if (M <= kStreamMaxM) {
+    // streaming form: tiles of 1024 / M bins, C consecutive tiles per block
+    const int Bs = 1024 / M;
+    const int64_t tiles_s = (N + Bs - 1) / Bs;
+    int64_t C = (tiles_s * n_series + PYCWB_T2W_STREAM_BLOCKS - 1) / PYCWB_T2W_STREAM_BLOCKS;
+    if (C < 1) C = 1;
+    if (C > 1 << 20) C = 1 << 20;
+    const int64_t grid_s = ((tiles_s + C - 1) / C) * n_series;
+    if (grid_s > 0x7fffffff) return PYCWB_ERR_ARG;
+    const unsigned g = (unsigned)grid_s;
+    const double2* mp = (const double2*)map;
+    const double2* tw = (const double2*)twiddle;
+    switch (log2_2M - 1) {
+      case 1: launch_w2t_stream<1>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      case 2: launch_w2t_stream<2>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      case 3: launch_w2t_stream<3>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      case 4: launch_w2t_stream<4>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      case 5: launch_w2t_stream<5>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      case 6: launch_w2t_stream<6>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      default: launch_w2t_stream<7>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+    }
+  } else {
+    hipLaunchKernelGGL(k_w2t_bins, dim3((unsigned)grid), dim3(kThreads),
+                       (size_t)(B / 2) * 2 * M * sizeof(double2), st, (const double2*)map, N,
+                       n_series, M, (const double2*)twiddle, log2_2M, B, quad, work);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     pycwb::record_error("k_w2t_bins", hipGetErrorString(e));
