@@ -36,13 +36,24 @@ __device__ __forceinline__ unsigned bitrev(unsigned v, int bits) {
   return __brev(v) >> (32 - bits);
 }
 
+// LDS position of complex element i of an FFT buffer in the swizzled layout:
+// bits 0-3 of i are XORed with bits 4-5 (once into bits 0-1, once into bits
+// 2-3).  Within any aligned block of 64 elements this is a bijection, so a
+// buffer keeps its size.  It spreads the radix-4 passes' strided accesses
+// (elements 4l + m, and groups of four 16 apart) over distinct 16-byte bank
+// groups, and leaves runs of consecutive elements consecutive within each
+// aligned group of 16.
+__device__ __forceinline__ int swz(int i) { return i ^ (((i >> 4) & 3) * 5); }
+
 // In-place radix-2 DIT FFT, exp(-2 pi i f k / 2M), of `count` sequences of
-// length 2M = 2^lg stored back to back in LDS (input in bit-reversed order).
-// Stages run in pairs as radix-4 butterflies (one barrier per pair), the last
-// stage alone when lg is odd; power-of-two index arithmetic throughout.
-// tw[k] = exp(-2 pi i k / 2M), k < M.
+// length 2M = 2^lg stored back to back in LDS (input in bit-reversed order;
+// element i at swz(i) when SW).  Stages run in pairs as radix-4 butterflies
+// (one barrier per pair), the last stage alone when lg is odd; power-of-two
+// index arithmetic throughout.  tw[k] = exp(-2 pi i k / 2M), k < M.
+template <bool SW = false>
 __device__ __forceinline__ void lds_fft(double2* __restrict__ buf, int count, int lg,
                                         const double2* __restrict__ tw) {
+  auto at = [](int i) { return SW ? swz(i) : i; };
   const int M2 = 1 << lg;
   const int qpl = lg - 2;  // log2 of the radix-4 butterflies per sequence
   int lh = 0;
@@ -55,7 +66,8 @@ __device__ __forceinline__ void lds_fft(double2* __restrict__ buf, int count, in
       const int i0 = (p << lg) + (g << (lh + 2)) + j;
       const double2 w1 = tw[j << (lg - 1 - lh)];
       const double2 w2a = tw[j << (lg - 2 - lh)], w2b = tw[(j + h) << (lg - 2 - lh)];
-      const double2 x0 = buf[i0], x1 = buf[i0 + h], x2 = buf[i0 + 2 * h], x3 = buf[i0 + 3 * h];
+      const int p0 = at(i0), p1 = at(i0 + h), p2 = at(i0 + 2 * h), p3 = at(i0 + 3 * h);
+      const double2 x0 = buf[p0], x1 = buf[p1], x2 = buf[p2], x3 = buf[p3];
       // stage h: (x0, x1) and (x2, x3) with w1
       double vr = x1.x * w1.x - x1.y * w1.y, vi = x1.x * w1.y + x1.y * w1.x;
       const double2 a0 = make_double2(x0.x + vr, x0.y + vi), a1 = make_double2(x0.x - vr, x0.y - vi);
@@ -65,12 +77,12 @@ __device__ __forceinline__ void lds_fft(double2* __restrict__ buf, int count, in
       // stage 2h: (a0, a2) with w2a, (a1, a3) with w2b
       vr = a2.x * w2a.x - a2.y * w2a.y;
       vi = a2.x * w2a.y + a2.y * w2a.x;
-      buf[i0] = make_double2(a0.x + vr, a0.y + vi);
-      buf[i0 + 2 * h] = make_double2(a0.x - vr, a0.y - vi);
+      buf[p0] = make_double2(a0.x + vr, a0.y + vi);
+      buf[p2] = make_double2(a0.x - vr, a0.y - vi);
       vr = a3.x * w2b.x - a3.y * w2b.y;
       vi = a3.x * w2b.y + a3.y * w2b.x;
-      buf[i0 + h] = make_double2(a1.x + vr, a1.y + vi);
-      buf[i0 + 3 * h] = make_double2(a1.x - vr, a1.y - vi);
+      buf[p1] = make_double2(a1.x + vr, a1.y + vi);
+      buf[p3] = make_double2(a1.x - vr, a1.y - vi);
     }
     __syncthreads();
   }
@@ -79,7 +91,7 @@ __device__ __forceinline__ void lds_fft(double2* __restrict__ buf, int count, in
     const int n2 = count << (lg - 1);
     for (int e = threadIdx.x; e < n2; e += blockDim.x) {
       const int p = e >> (lg - 1), j = e & (h - 1);
-      const int i0 = (p << lg) + j, i1 = i0 + h;
+      const int i0 = at((p << lg) + j), i1 = at((p << lg) + j + h);
       const double2 w = tw[j];
       const double2 u = buf[i0], v = buf[i1];
       const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
@@ -375,7 +387,8 @@ __host__ size_t stream_lds_bytes(int M, int KK, int thr, int R) {
 // threads every pass is exactly one radix-4 butterfly per thread (the last
 // radix-2 stage, for odd LG2, two), and a thread's butterfly sits at the same
 // place in every tile, so its twiddles are read once per block into registers
-// (LDS then carries only the data: a quarter less traffic per pass).
+// (LDS then carries only the data: a quarter less traffic per pass).  The
+// buffer is in the swizzled layout (swz).
 template <int LG2>
 struct StreamTw {
   static constexpr int NR4 = LG2 / 2;   // radix-4 passes
@@ -410,7 +423,8 @@ __device__ __forceinline__ void stream_fft(double2* __restrict__ buf, const Stre
     const int g = r >> lh, j = r & (h - 1);
     const int i0 = (p << LG2) + (g << (lh + 2)) + j;
     const double2 w1 = T.w[ps][0], w2a = T.w[ps][1], w2b = T.w[ps][2];
-    const double2 x0 = buf[i0], x1 = buf[i0 + h], x2 = buf[i0 + 2 * h], x3 = buf[i0 + 3 * h];
+    const int p0 = swz(i0), p1 = swz(i0 + h), p2 = swz(i0 + 2 * h), p3 = swz(i0 + 3 * h);
+    const double2 x0 = buf[p0], x1 = buf[p1], x2 = buf[p2], x3 = buf[p3];
     double vr = x1.x * w1.x - x1.y * w1.y, vi = x1.x * w1.y + x1.y * w1.x;
     const double2 a0 = make_double2(x0.x + vr, x0.y + vi), a1 = make_double2(x0.x - vr, x0.y - vi);
     vr = x3.x * w1.x - x3.y * w1.y;
@@ -418,12 +432,12 @@ __device__ __forceinline__ void stream_fft(double2* __restrict__ buf, const Stre
     const double2 a2 = make_double2(x2.x + vr, x2.y + vi), a3 = make_double2(x2.x - vr, x2.y - vi);
     vr = a2.x * w2a.x - a2.y * w2a.y;
     vi = a2.x * w2a.y + a2.y * w2a.x;
-    buf[i0] = make_double2(a0.x + vr, a0.y + vi);
-    buf[i0 + 2 * h] = make_double2(a0.x - vr, a0.y - vi);
+    buf[p0] = make_double2(a0.x + vr, a0.y + vi);
+    buf[p2] = make_double2(a0.x - vr, a0.y - vi);
     vr = a3.x * w2b.x - a3.y * w2b.y;
     vi = a3.x * w2b.y + a3.y * w2b.x;
-    buf[i0 + h] = make_double2(a1.x + vr, a1.y + vi);
-    buf[i0 + 3 * h] = make_double2(a1.x - vr, a1.y - vi);
+    buf[p1] = make_double2(a1.x + vr, a1.y + vi);
+    buf[p3] = make_double2(a1.x - vr, a1.y - vi);
     __syncthreads();
   }
   if constexpr (LG2 & 1) {
@@ -432,7 +446,7 @@ __device__ __forceinline__ void stream_fft(double2* __restrict__ buf, const Stre
     for (int q = 0; q < 2; ++q) {
       const int ee = e + q * kThreads;
       const int p = ee >> (LG2 - 1), j = ee & (h - 1);
-      const int i0 = (p << LG2) + j, i1 = i0 + h;
+      const int i0 = swz((p << LG2) + j), i1 = swz((p << LG2) + j + h);
       const double2 w = T.wl;
       const double2 u = buf[i0], v = buf[i1];
       const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
@@ -455,6 +469,9 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
   constexpr int NP = B / 2;        // bin pairs per tile
   constexpr bool kRegTw = THR == kThreads;
   static_assert(B >= R && B % 2 == 0 && P * THR == B * M, "tile shape");
+  // FFT buffer position of element i: swizzled in the 256-thread shape (measured
+  // faster there), plain in the 1024-thread one (measured slower swizzled)
+  auto at = [](int i) { return kRegTw ? swz(i) : i; };
   extern __shared__ double2 buf[];  // B M complex (FFT), the window ring, M twiddles
   const int64_t tiles = (N + B - 1) / B;
   const int64_t chunks = (tiles + C - 1) / C;
@@ -518,7 +535,7 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
 #pragma unroll
     for (int b = 0; b < R; ++b) {
       const int bb = r * R + b;
-      lds[2 * ((bb >> 1) * M2 + kr) + (bb & 1)] = acc[b];
+      lds[2 * at((bb >> 1) * M2 + kr) + (bb & 1)] = acc[b];
     }
     __syncthreads();
     if constexpr (kRegTw) stream_fft<LG2>(buf, T);
@@ -531,8 +548,8 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
     for (int e = threadIdx.x; e < NP * (M - 1); e += THR) {
       const int p = e / (M - 1);
       const int f = e - p * (M - 1) + 1;
-      const double2 z = buf[p * M2 + f];
-      const double2 zc = buf[p * M2 + M2 - f];
+      const double2 z = buf[at(p * M2 + f)];
+      const double2 zc = buf[at(p * M2 + M2 - f)];
       const double R0 = 0.5 * (z.x + zc.x), I0 = 0.5 * (z.y - zc.y);
       const double R1 = 0.5 * (z.y + zc.y), I1 = -0.5 * (z.x - zc.x);
       double2 oa, ob;
@@ -551,7 +568,7 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
     }
     for (int e = threadIdx.x; e < NP * 2; e += THR) {
       const int p = e >> 1, f = (e & 1) ? M : 0;
-      const double2 z = buf[p * M2 + f];   // Z[2M - f] is Z[f] for f = 0, M
+      const double2 z = buf[at(p * M2 + f)];   // Z[2M - f] is Z[f] for f = 0, M
       double R0 = 0.5 * (z.x + z.x), R1 = 0.5 * (z.y + z.y);
       R0 = R0 / sqrt2;
       R1 = R1 / sqrt2;
@@ -711,11 +728,12 @@ __global__ __launch_bounds__(kThreads) void k_w2t_gather(
 // Streaming form of k_w2t_bins for M = 2^LG <= 128: a block of 256 threads owns
 // C consecutive tiles of B = 1024 / M bins (B / 2 pairs, 1,024 complex points)
 // of one series.  Thread e handles the four (pair, slot) items e + 256 i of a
-// tile; the coefficients the next tile needs (one phase of two bins per item)
-// are loaded into registers while the current tile is transformed and stored,
-// and the FFT twiddles stay in registers (stream_fft).  The half spectrum, the
-// FFT stages and the outputs are those of k_w2t_bins, so the work rows are
-// identical.
+// tile.  The coefficients the next tile needs (one phase of two bins per item)
+// are loaded into registers once the current tile's spectrum is in LDS, so they
+// arrive while it is transformed and stored; the FFT twiddles stay in registers
+// (stream_fft) and the FFT buffer is in the swizzled layout.  The half
+// spectrum, the FFT stages and the outputs are those of k_w2t_bins, so the work
+// rows are identical.
 template <int LG>
 __global__ __launch_bounds__(kThreads) void k_w2t_stream(
     const double2* __restrict__ map, int64_t N, int n_series, const double2* __restrict__ tw,
@@ -755,9 +773,6 @@ __global__ __launch_bounds__(kThreads) void k_w2t_stream(
   load(t0 * B, cv);
   for (int t = 0; t < nt; ++t) {
     const int64_t n0 = (t0 + t) * B;
-    double nx[I][2];
-    const bool more = t + 1 < nt;
-    if (more) load(n0 + B, nx);
 #pragma unroll
     for (int i = 0; i < I; ++i) {
       const int p = (threadIdx.x + i * kThreads) >> LG2;
@@ -786,24 +801,20 @@ __global__ __launch_bounds__(kThreads) void k_w2t_stream(
         wi[h] = (k <= M) ? -im : im;
         if (k == 0 || k == M) wi[h] = 0.0;
       }
-      buf[p * M2 + kr] = make_double2(wr[0] - wi[1], wi[0] + wr[1]);
+      buf[swz(p * M2 + kr)] = make_double2(wr[0] - wi[1], wi[0] + wr[1]);
     }
+    // the next tile's coefficients, in flight during this tile's FFT and stores
+    const bool more = t + 1 < nt;
+    if (more) load(n0 + B, cv);
     __syncthreads();
     stream_fft<LG2>(buf, T);
     double* wk = work + ((int64_t)s * N + n0) * M2;
 #pragma unroll
     for (int i = 0; i < I; ++i) {
       const int p = (threadIdx.x + i * kThreads) >> LG2;
-      const double2 z = buf[p * M2 + k];
+      const double2 z = buf[swz(p * M2 + k)];
       if (n0 + 2 * p < N) wk[(2 * p) * M2 + k] = z.x;
       if (n0 + 2 * p + 1 < N) wk[(2 * p + 1) * M2 + k] = z.y;
-    }
-    if (more) {
-#pragma unroll
-      for (int i = 0; i < I; ++i) {
-        cv[i][0] = nx[i][0];
-        cv[i][1] = nx[i][1];
-      }
     }
     __syncthreads();
   }
