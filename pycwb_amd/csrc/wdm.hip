@@ -725,7 +725,7 @@ __global__ __launch_bounds__(kThreads) void k_w2t_gather(
   out[(int64_t)s * n_out + t] = acc;
 }
 
-// Streaming form of k_w2t_bins for M = 2^LG <= 128: a block of 256 threads owns
+// Streaming form of k_w2t_bins for M = 2^LG <= 512: a block of 256 threads owns
 // C consecutive tiles of B = 1024 / M bins (B / 2 pairs, 1,024 complex points)
 // of one series.  Thread e handles the four (pair, slot) items e + 256 i of a
 // tile.  The coefficients the next tile needs (one phase of two bins per item)
@@ -754,15 +754,16 @@ __global__ __launch_bounds__(kThreads) void k_w2t_stream(
   const double r2 = 1.4142135623730951;
   StreamTw<LG2> T;
   stream_tw_load<LG2>(T, tw);
-  // the spectrum slot of item i: k = e & (2M - 1), j = min(k, 2M - k)
-  const int k = threadIdx.x & (M2 - 1);
-  const int j = k <= M ? k : M2 - k;
-  const int kr = (int)bitrev((unsigned)k, LG2);
+  // the spectrum slot of item i: k = e & (2M - 1), j = min(k, 2M - k) (the
+  // same for every item when 2M <= 256)
+  auto slot = [](int i) { return (int)((threadIdx.x + i * kThreads) & (M2 - 1)); };
+  auto half = [](int k) { return k <= M ? k : M2 - k; };
   double cv[I][2];
   auto load = [&](int64_t n0, double (&c)[I][2]) {
 #pragma unroll
     for (int i = 0; i < I; ++i) {
       const int p = (threadIdx.x + i * kThreads) >> LG2;
+      const int j = half(slot(i));
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int64_t n = n0 + 2 * p + h;
@@ -776,6 +777,7 @@ __global__ __launch_bounds__(kThreads) void k_w2t_stream(
 #pragma unroll
     for (int i = 0; i < I; ++i) {
       const int p = (threadIdx.x + i * kThreads) >> LG2;
+      const int k = slot(i), j = half(k);
       double wr[2], wi[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -801,7 +803,8 @@ __global__ __launch_bounds__(kThreads) void k_w2t_stream(
         wi[h] = (k <= M) ? -im : im;
         if (k == 0 || k == M) wi[h] = 0.0;
       }
-      buf[swz(p * M2 + kr)] = make_double2(wr[0] - wi[1], wi[0] + wr[1]);
+      buf[swz(p * M2 + (int)bitrev((unsigned)k, LG2))] =
+          make_double2(wr[0] - wi[1], wi[0] + wr[1]);
     }
     // the next tile's coefficients, in flight during this tile's FFT and stores
     const bool more = t + 1 < nt;
@@ -812,6 +815,7 @@ __global__ __launch_bounds__(kThreads) void k_w2t_stream(
 #pragma unroll
     for (int i = 0; i < I; ++i) {
       const int p = (threadIdx.x + i * kThreads) >> LG2;
+      const int k = slot(i);
       const double2 z = buf[swz(p * M2 + k)];
       if (n0 + 2 * p < N) wk[(2 * p) * M2 + k] = z.x;
       if (n0 + 2 * p + 1 < N) wk[(2 * p + 1) * M2 + k] = z.y;
@@ -1002,7 +1006,7 @@ int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const doubl
   if (grid > 0x7fffffff || n_out / kThreads > 0x7fffffff || n_series > 65535)
     return PYCWB_ERR_ARG;
   const hipStream_t st = (hipStream_t)stream;
-  if (M <= kStreamMaxM) {
+  if (M <= kStreamWideM) {
     // streaming form: tiles of 1024 / M bins, C consecutive tiles per block
     const int Bs = 1024 / M;
     const int64_t tiles_s = (N + Bs - 1) / Bs;
@@ -1021,7 +1025,9 @@ int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const doubl
       case 4: launch_w2t_stream<4>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
       case 5: launch_w2t_stream<5>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
       case 6: launch_w2t_stream<6>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
-      default: launch_w2t_stream<7>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      case 7: launch_w2t_stream<7>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      case 8: launch_w2t_stream<8>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
+      default: launch_w2t_stream<9>(g, st, mp, N, n_series, tw, (int)C, quad, work); break;
     }
   } else {
     hipLaunchKernelGGL(k_w2t_bins, dim3((unsigned)grid), dim3(kThreads),

@@ -173,15 +173,19 @@ def test_oracle_inverse_reconstructs(oracle, M):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("M,S", [(2, 1), (8, 3), (16, 2), (64, 2), (256, 1), (1024, 1)])
-def test_gpu_inverse_matches_oracle(oracle, M, S):
+@pytest.mark.parametrize("M,S,L", [(2, 1, None), (8, 3, None), (16, 2, None), (32, 2, None),
+                                   (64, 2, None), (128, 2, None), (256, 1, None),
+                                   (512, 2, None), (1024, 1, None),
+                                   # long enough for blocks that stream several tiles
+                                   (8, 1, 2100003), (512, 1, 1100000)])
+def test_gpu_inverse_matches_oracle(oracle, M, S, L):
     import torch
 
     from pycwb_amd.wdm import WDM
 
-    print(f"case w2t M={M} S={S}", flush=True)
+    print(f"case w2t M={M} S={S} L={L}", flush=True)
     w = WDM(M, M, 6, 10)
-    L = 24 * M + 2 * w.m_H + 7
+    L = L or 24 * M + 2 * w.m_H + 7
     xs = np.stack([_series(L, seed=k + M) for k in range(S)])
     dev = torch.device("cuda", 0)
     tf = w.t2w_device(torch.from_numpy(xs).to(dev))
