@@ -194,6 +194,163 @@ __global__ __launch_bounds__(256) void k_td_vectors(TdArgs a) {
   }
 }
 
+// Pixel-major form for 25-tap filters and 2K + 1 <= 192 delays: a block of 192
+// threads walks over pixels, thread ki owning delay ki of every pixel.  Its
+// taps (T0 and Tx at its sub-step, 25 each) stay in registers; the block
+// stages each pixel's plane values once, as doubles in LDS: layers m - 1, m,
+// m + 1 of both planes over the rows n - s_max .. n - s_min + 24 that the
+// delays' whole-bin shifts reach (zero outside the planes and the layers), and
+// loads the next pixel's values into registers while the current one is
+// computed.  Every amplitude is the same sequence of operations as pixel_amp's
+// (even and odd sums in ascending tap order, the same phases, edges and
+// quadrature swap), so the tables are identical.
+constexpr int kPixThreads = 192;
+constexpr int kPixTaps = 25;
+constexpr int kPixStage = 8;   // staged values per thread and pixel (6 NR <= 1536)
+
+struct PixTaps {
+  double t0[kPixTaps], tx[kPixTaps];
+};
+
+__device__ __forceinline__ void pix_sums(const double* __restrict__ v, const double* tap,
+                                         double& se, double& so) {
+  // one sum's LDS reads at a time: the scheduler may not hoist the next sum's
+  // loads into this one (all six at once need more than the register file)
+  __builtin_amdgcn_sched_barrier(0);
+  double e = 0.0, o = 0.0;
+#pragma unroll
+  for (int k = 0; k < kPixTaps; ++k) {
+    if (k & 1) o += v[k] * tap[k];
+    else e += v[k] * tap[k];
+  }
+  se = e;
+  so = o;
+}
+
+// pixel_amp with the layer values at v(L)[k] = plane row ne + k of layer m - 1 + L
+__device__ __forceinline__ double pix_amp(const TdArgs& a, int n, int m, int dT,
+                                          const double* __restrict__ vm1,
+                                          const double* __restrict__ v0,
+                                          const double* __restrict__ vp1, const PixTaps& T,
+                                          bool quad) {
+  const int M = a.M;
+  const bool odd_n = n & 1, odd_mn = (m + n) & 1;
+  const bool cancel_even = odd_n ^ quad;
+  const bool edge = m == 0 || m == M;
+  double se, so;
+  pix_sums(v0, T.t0, se, so);
+  if (edge && cancel_even) se = 0.0;
+  if (edge && !cancel_even) so = 0.0;
+  const int J4 = 4 * a.J;
+  const double2 ph = phase(a.trig, J4, 2 * m * dT);
+  double r = odd_n ? se * ph.y - so * ph.x : se * ph.y + so * ph.x;
+  if (m > 0) {
+    double le, lo;
+    pix_sums(vm1, T.tx, le, lo);
+    if (m == 1 && cancel_even) le = 0.0;
+    if (m == 1 && !cancel_even) lo = 0.0;
+    const double slp = phase(a.trig, J4, (2 * m - 1) * dT).x;
+    double t = odd_n ? (le - lo) * slp : (le + lo) * slp;
+    if (m == 1 || m == M) t *= 1.4142135623730951;
+    r = odd_mn ? r - t : r + t;
+  }
+  if (m < M) {
+    double he, ho;
+    pix_sums(vp1, T.tx, he, ho);
+    if (m == M - 1 && cancel_even) he = 0.0;
+    if (m == M - 1 && !cancel_even) ho = 0.0;
+    const double shp = phase(a.trig, J4, (2 * m + 1) * dT).x;
+    double t = odd_n ? (he + ho) * shp : (he - ho) * shp;
+    if (m == 0 || m == M - 1) t *= 1.4142135623730951;
+    r = odd_mn ? r - t : r + t;
+  }
+  if (edge && cancel_even) return 0.0;
+  return r;
+}
+
+__device__ __forceinline__ int trunc_div(int x, int J) { return x >= 0 ? x / J : -((-x) / J); }
+
+__global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int s_min, int s_max) {
+  extern __shared__ double stage[];   // two buffers of 6 NR doubles: (plane, layer, row)
+  const int NR = kPixTaps + s_max - s_min;
+  const int NS = 6 * NR;
+  const int half = 2 * a.K + 1;
+  const int M1 = a.M + 1;
+  const int ki = threadIdx.x;
+  const bool active = ki < half;
+  const int dT = (active ? ki : 0) - a.K;
+  const int sh = trunc_div(dT, a.J);
+  const int sub = dT - sh * a.J;
+  const int off = s_max - sh;   // row ne + k of the lane is staged row off + k
+  PixTaps T;
+  {
+    const int64_t D = 2 * (int64_t)a.J + 1;
+#pragma unroll
+    for (int k = 0; k < kPixTaps; ++k) {
+      T.t0[k] = a.T0[k * D + sub + a.J];
+      T.tx[k] = a.Tx[k * D + sub + a.J];
+    }
+  }
+  // staged value i: plane i / (3 NR), layer m - 1 + (i / NR) % 3, row n - s_max + i % NR
+  auto fetch = [&](int64_t p, float (&v)[kPixStage]) {
+    const int idx = a.pix[p];
+    const int n = idx / M1, m = idx - n * M1;
+#pragma unroll
+    for (int j = 0; j < kPixStage; ++j) {
+      const int i = threadIdx.x + j * kPixThreads;
+      float x = 0.f;
+      if (i < NS) {
+        const int pl = i / (3 * NR), rem = i - pl * 3 * NR;
+        const int layer = m - 1 + rem / NR;
+        const int64_t row = (int64_t)n - s_max + rem % NR;
+        if (layer >= 0 && layer <= a.M && row >= 0 && row < a.n_rows)
+          x = (pl ? a.p90 : a.p00)[row * M1 + layer];
+      }
+      v[j] = x;
+    }
+  };
+  float nx[kPixStage];
+  int64_t p = blockIdx.x;
+  if (p < a.n_pixels) fetch(p, nx);
+  int b = 0;
+  for (; p < a.n_pixels; p += gridDim.x, b ^= 1) {
+    double* sb = stage + b * NS;
+#pragma unroll
+    for (int j = 0; j < kPixStage; ++j) {
+      const int i = threadIdx.x + j * kPixThreads;
+      if (i < NS) sb[i] = (double)nx[j];
+    }
+    __syncthreads();   // this pixel staged; the other buffer is free again
+    const int idx = a.pix[p];
+    const int n = idx / M1, m = idx - n * M1;
+    if (p + gridDim.x < a.n_pixels) fetch(p + gridDim.x, nx);
+    if (active) {
+      const int ne = n - sh;
+      // odd whole-bin shifts swap the quadratures (getTDamp): a00 from the 90
+      // plane, a90 from the 00 plane, one of them negated by the (n + m) parity
+      const bool swap = (sh & 1) != 0;
+      const double* q00 = sb + off;            // plane 00, layer m - 1
+      const double* q90 = sb + 3 * NR + off;   // plane 90, layer m - 1
+      const double* A = swap ? q90 : q00;
+      const double* Bq = swap ? q00 : q90;
+      double a00 = pix_amp(a, ne, m, sub, A, A + NR, A + 2 * NR, T, swap);
+      double a90 = pix_amp(a, ne, m, sub, Bq, Bq + NR, Bq + 2 * NR, T, !swap);
+      if (swap) {
+        if ((n + m) & 1) a00 = -a00;
+        else a90 = -a90;
+      }
+      float* o = a.out + p * a.row_stride;
+      if (a.interleave) {
+        o[2 * ki] = (float)a00;
+        o[2 * ki + 1] = (float)a90;
+      } else {
+        o[ki] = (float)a00;
+        o[half + ki] = (float)a90;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -212,6 +369,21 @@ int pycwb_td_vectors(const int32_t* pixel_index, int64_t n_pixels, const float* 
   a.n_rows = n_rows; a.M = M; a.T0 = T0; a.Tx = Tx; a.trig = (const double2*)trig;
   a.nc = n_coeffs; a.J = J; a.K = K;
   a.out = out; a.row_stride = row_stride; a.interleave = interleave ? 1 : 0;
+  const int s_min = K >= J ? -(K / J) : 0, s_max = K / J;   // whole-bin shifts of -K .. K
+  const int NR = kPixTaps + s_max - s_min;
+  if (2 * n_coeffs + 1 == kPixTaps && 2 * K + 1 <= kPixThreads &&
+      6 * NR <= kPixStage * kPixThreads) {
+    int64_t grid = n_pixels < 8192 ? n_pixels : 8192;
+    hipLaunchKernelGGL(k_td_pixels, dim3((unsigned)grid), dim3(kPixThreads),
+                       (size_t)2 * 6 * NR * sizeof(double), (hipStream_t)stream, a, s_min,
+                       s_max);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      pycwb::record_error("k_td_pixels", hipGetErrorString(e));
+      return PYCWB_ERR_HIP;
+    }
+    return PYCWB_OK;
+  }
   const int64_t total = n_pixels * (2 * K + 1);
   int64_t grid = (total + 255) / 256;
   if (grid > 65536) grid = 65536;
