@@ -194,14 +194,14 @@ __global__ __launch_bounds__(256) void k_td_vectors(TdArgs a) {
   }
 }
 
-// Pixel-major form for 25-tap filters and 2K + 1 <= 192 delays: a block of 192
-// threads walks over pixels, thread ki owning delay ki of every pixel.  Its
-// taps (T0 and Tx at its sub-step, 25 each) stay in registers; the block
-// stages each pixel's plane values once, as doubles in LDS: layers m - 1, m,
-// m + 1 of both planes over the rows n - s_max .. n - s_min + 24 that the
-// delays' whole-bin shifts reach (zero outside the planes and the layers), and
-// loads the next pixel's values into registers while the current one is
-// computed.  Every amplitude is the same sequence of operations as pixel_amp's
+// Pixel-major form for 25-tap filters: the 2K + 1 delays split into chunks of
+// 192, and block b serves chunk b mod n_chunks over its own pixels, thread t
+// owning delay 192 c + t of every pixel.  Its taps (T0 and Tx at its sub-step,
+// 25 each) stay in registers; the block stages each pixel's plane values once,
+// as doubles in LDS: layers m - 1, m, m + 1 of both planes over the rows
+// n - s_hi .. n - s_lo + 24 that the chunk's whole-bin shifts reach (zero
+// outside the planes and the layers), and loads the next pixel's values into
+// registers while the current one is computed.  Every amplitude is the same sequence of operations as pixel_amp's
 // (even and odd sums in ascending tap order, the same phases, edges and
 // quadrature swap), so the tables are identical.
 constexpr int kPixThreads = 192;
@@ -270,13 +270,27 @@ __device__ __forceinline__ double pix_amp(const TdArgs& a, int n, int m, int dT,
 
 __device__ __forceinline__ int trunc_div(int x, int J) { return x >= 0 ? x / J : -((-x) / J); }
 
-__global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int s_min, int s_max) {
-  extern __shared__ double stage[];   // two buffers of 6 NR doubles: (plane, layer, row)
-  const int NR = kPixTaps + s_max - s_min;
+// whole-bin shifts of the delays of chunk c: [s_lo, s_hi]
+__host__ __device__ inline int td_chunk_rows(int c, int K, int J, int& s_lo, int& s_hi) {
+  const int half = 2 * K + 1;
+  const int lo = c * kPixThreads, hi = (lo + kPixThreads < half ? lo + kPixThreads : half) - 1;
+  auto tdiv = [](int x, int j) { return x >= 0 ? x / j : -((-x) / j); };
+  s_lo = tdiv(lo - K, J);
+  s_hi = tdiv(hi - K, J);
+  return kPixTaps + s_hi - s_lo;
+}
+
+__global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int n_chunks, int NS2) {
+  extern __shared__ double stage[];   // two buffers of NS2 doubles: (plane, layer, row)
+  const int c = (int)(blockIdx.x % (unsigned)n_chunks);
+  const int64_t g = blockIdx.x / (unsigned)n_chunks;
+  const int64_t G = gridDim.x / (unsigned)n_chunks;
+  int s_min, s_max;
+  const int NR = td_chunk_rows(c, a.K, a.J, s_min, s_max);
   const int NS = 6 * NR;
   const int half = 2 * a.K + 1;
   const int M1 = a.M + 1;
-  const int ki = threadIdx.x;
+  const int ki = c * kPixThreads + threadIdx.x;
   const bool active = ki < half;
   const int dT = (active ? ki : 0) - a.K;
   const int sh = trunc_div(dT, a.J);
@@ -310,11 +324,11 @@ __global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int s_min, 
     }
   };
   float nx[kPixStage];
-  int64_t p = blockIdx.x;
+  int64_t p = g;
   if (p < a.n_pixels) fetch(p, nx);
   int b = 0;
-  for (; p < a.n_pixels; p += gridDim.x, b ^= 1) {
-    double* sb = stage + b * NS;
+  for (; p < a.n_pixels; p += G, b ^= 1) {
+    double* sb = stage + b * NS2;
 #pragma unroll
     for (int j = 0; j < kPixStage; ++j) {
       const int i = threadIdx.x + j * kPixThreads;
@@ -323,7 +337,7 @@ __global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int s_min, 
     __syncthreads();   // this pixel staged; the other buffer is free again
     const int idx = a.pix[p];
     const int n = idx / M1, m = idx - n * M1;
-    if (p + gridDim.x < a.n_pixels) fetch(p + gridDim.x, nx);
+    if (p + G < a.n_pixels) fetch(p + G, nx);
     if (active) {
       const int ne = n - sh;
       // odd whole-bin shifts swap the quadratures (getTDamp): a00 from the 90
@@ -369,14 +383,21 @@ int pycwb_td_vectors(const int32_t* pixel_index, int64_t n_pixels, const float* 
   a.n_rows = n_rows; a.M = M; a.T0 = T0; a.Tx = Tx; a.trig = (const double2*)trig;
   a.nc = n_coeffs; a.J = J; a.K = K;
   a.out = out; a.row_stride = row_stride; a.interleave = interleave ? 1 : 0;
-  const int s_min = K >= J ? -(K / J) : 0, s_max = K / J;   // whole-bin shifts of -K .. K
-  const int NR = kPixTaps + s_max - s_min;
-  if (2 * n_coeffs + 1 == kPixTaps && 2 * K + 1 <= kPixThreads &&
-      6 * NR <= kPixStage * kPixThreads) {
-    int64_t grid = n_pixels < 8192 ? n_pixels : 8192;
+  const int n_chunks = (2 * K + 1 + kPixThreads - 1) / kPixThreads;
+  int NRmax = 0;
+  for (int c = 0; c < n_chunks; ++c) {
+    int lo, hi;
+    const int nr = td_chunk_rows(c, K, J, lo, hi);
+    NRmax = nr > NRmax ? nr : NRmax;
+  }
+  if (2 * n_coeffs + 1 == kPixTaps && 6 * NRmax <= kPixStage * kPixThreads) {
+    int64_t G = 8192 / n_chunks;
+    G = G < 1 ? 1 : (G > n_pixels ? n_pixels : G);
+    const int64_t grid = G * n_chunks;
+    if (grid > 0x7fffffff) return PYCWB_ERR_ARG;
     hipLaunchKernelGGL(k_td_pixels, dim3((unsigned)grid), dim3(kPixThreads),
-                       (size_t)2 * 6 * NR * sizeof(double), (hipStream_t)stream, a, s_min,
-                       s_max);
+                       (size_t)2 * 6 * NRmax * sizeof(double), (hipStream_t)stream, a, n_chunks,
+                       6 * NRmax);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       pycwb::record_error("k_td_pixels", hipGetErrorString(e));

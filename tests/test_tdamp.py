@@ -112,7 +112,9 @@ def test_oracle_td_amplitude_is_delayed_coefficient(oracle, M):
 @pytest.mark.parametrize("M,nc,L,K", [(8, 6, 1, 20), (16, 12, 1, 16), (32, 8, 2, 40),
                                       (64, 12, 1, 64),
                                       # whole-bin shifts -4 .. 4 (J = 16, K = 64)
-                                      (16, 12, 1, 64)])
+                                      (16, 12, 1, 64),
+                                      # 401 delays: three chunks of the pixel-major kernel
+                                      (16, 12, 1, 200)])
 def test_gpu_td_vectors_match_oracle(oracle, M, nc, L, K):
     import torch
 
