@@ -490,6 +490,20 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
                   int stride, const double* filter, int m_H, const double* twiddle,
                   int quadrature, double* out, void* stream);
 
+/* pycwb_wdm_t2w (stride M) of n_shifts shifted copies of one series ts,
+ * without the copies being written: copy s is ts shifted by shifts[s] as
+ * pycwb_max_energy_shift defines it (k > 0: x[i] = ts[i + k] for i < L - k,
+ * k < 0: x[i] = ts[i + k] for i >= -k, the other end keeping the original
+ * samples).  The maps equal pycwb_wdm_t2w's of the copies.  Replaces the
+ * shifted copies' transforms in the delay loop of time_delay_max_energy
+ * (pycwb/modules/coherence_native/time_delay_numba.py:183-288, which shifts
+ * the series and transforms each copy with wdm_wavelet's t2w).  Requires
+ * m_H = 12 M + 1 (the WDM filters of precision 10 and their relatives);
+ * PYCWB_ERR_ARG otherwise.  shifts: device int64, n_shifts entries. */
+int pycwb_wdm_t2w_shifted(const double* ts, int64_t n_samples, const int64_t* shifts,
+                          int n_shifts, int M, const double* filter, int m_H,
+                          const double* twiddle, int quadrature, double* out, void* stream);
+
 /* WDM inverse transforms w2t (quad = 0: from the 00 coefficients) and w2tQ
  * (quad = 1: from the 90 coefficients) of n_series maps.  Replaces
  * WDM<T>::w2t / w2tQ (cwb-core/WDM.cc:1536-1710) as called through

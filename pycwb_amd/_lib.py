@@ -184,6 +184,7 @@ SIGNATURES = {
     "pycwb_sky_stats_scratch64": (sz, [i32, i32]),
     "pycwb_sky_statistics": (i32, [C.POINTER(StatsDesc), vp]),
     "pycwb_wdm_t2w": (i32, [vp, i64, i64, i32, i32, i32, vp, i32, vp, i32, vp, vp]),
+    "pycwb_wdm_t2w_shifted": (i32, [vp, i64, vp, i32, i32, vp, i32, vp, i32, vp, vp]),
     "pycwb_sky_grad": (i32, [C.POINTER(GradDesc), vp]),
     "pycwb_td_vectors": (i32, [vp, i64, vp, vp, i64, i32, vp, vp, vp, i32, i32, i32, vp, i64,
                                i32, vp]),

@@ -5,9 +5,11 @@ coherence_native projection / time-delay modules.
   time_delay_max_energy_numba    (time_delay_numba.py:578-679), pattern path:
       the series and its copies shifted by k = d, 2d, ... <= max_delay samples
       to both sides go through the quadrature t2w in batches
-      (pycwb_max_energy_shift, pycwb_wdm_t2w), each map's packet energy is
-      folded into the pixel maximum (pycwb_max_energy_packet), and the lowest
-      layers are zeroed (pycwb_max_energy_finish)
+      (pycwb_wdm_t2w_shifted, which reads each copy straight from the series;
+      pycwb_max_energy_shift + pycwb_wdm_t2w for other filter lengths), each
+      map's packet energy is folded into the pixel maximum
+      (pycwb_max_energy_packet), and the lowest layers are zeroed
+      (pycwb_max_energy_finish)
   gamma_to_gauss_device          (types/time_frequency_map.py:143-246,
       TimeFrequencyMap.Gamma2Gauss): counts, order statistics, sums and the
       transform on the GPU (pycwb_g2g_phase), the Gamma fit on the host;
@@ -100,16 +102,22 @@ def max_energy_device(ts: torch.Tensor, wdm, max_delay: int, downsample: int, pa
     for b0 in range(0, shifts.size, batch):
         sh = torch.from_numpy(shifts[b0:b0 + batch]).to(dev)
         n = int(sh.numel())
-        xs = torch.empty((n, L), dtype=torch.float64, device=dev)
-        _lib.check(lib().pycwb_max_energy_shift(ptr(ts), L, ptr(sh), n, ptr(xs), st),
-                   "pycwb_max_energy_shift")
-        maps = wdm.t2w_device(xs, MM=-1, stream=stream)             # (n, M+1, T) view
+        # the transforms read the shifted copies straight from ts where the
+        # filter allows it; otherwise the copies are written out first
+        maps = wdm.t2w_shifted_device(ts, sh, stream=stream) \
+            if hasattr(wdm, "t2w_shifted_device") else None
+        if maps is None:
+            xs = torch.empty((n, L), dtype=torch.float64, device=dev)
+            _lib.check(lib().pycwb_max_energy_shift(ptr(ts), L, ptr(sh), n, ptr(xs), st),
+                       "pycwb_max_energy_shift")
+            maps = wdm.t2w_device(xs, MM=-1, stream=stream)         # (n, M+1, T) view
+            del xs
         mr = torch.view_as_real(maps.transpose(1, 2))              # (n, T, M+1, 2)
         if not mr.is_contiguous():
             mr = mr.contiguous()
         d.maps, d.n_maps, d.init = ptr(mr), n, 1 if b0 == 0 else 0
         _lib.check(lib().pycwb_max_energy_packet(C.byref(d), st), "pycwb_max_energy_packet")
-        del xs, maps, mr, sh
+        del maps, mr, sh
     _lib.check(lib().pycwb_max_energy_finish(ptr(cur), T, M1,
                                              1 if abs(int(pattern)) in (5, 6, 9) else 0, st),
                "pycwb_max_energy_finish")

@@ -32,6 +32,21 @@ __device__ __forceinline__ double ext_sample(const double* __restrict__ x, int64
   return (k >= 0 && k < L) ? x[k] : 0.0;
 }
 
+// Sample k of the mirror-padded input of a shifted copy of x: the padding
+// index is resolved first, then the copy's shift sh (sh > 0: xs[i] = x[i + sh]
+// for i < L - sh, sh < 0: xs[i] = x[i + sh] for i >= -sh, the other end
+// keeping the original samples), so the transform reads the shifted series
+// without it being written out.
+__device__ __forceinline__ double ext_sample_s(const double* __restrict__ x, int64_t L, int64_t k,
+                                               int64_t sh) {
+  if (k < 0) k = -k;
+  else if (k >= L) k = 2 * L - 1 - k;
+  if (!(k >= 0 && k < L)) return 0.0;
+  if (sh > 0 && k < L - sh) k += sh;
+  else if (sh < 0 && k >= -sh) k += sh;
+  return x[k];
+}
+
 __device__ __forceinline__ unsigned bitrev(unsigned v, int bits) {
   return __brev(v) >> (32 - bits);
 }
@@ -266,7 +281,7 @@ template <int R, int KK>
 __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
     const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int M, int64_t N,
     const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int log2_2M, int B,
-    int quadrature, double2* __restrict__ out) {
+    int quadrature, double2* __restrict__ out, const int64_t* __restrict__ shifts) {
   extern __shared__ double2 buf[];  // B * M complex (B / 2 bin pairs of length 2M), M twiddles
   const int M2 = 2 * M;
   const int64_t tiles = (N + B - 1) / B;
@@ -275,6 +290,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
   const int64_t n0 = (blk % tiles) * B;
   if (s >= n_series) return;
   const double* x = X + (int64_t)s * ldx;
+  const int64_t sh = shifts ? shifts[s] : 0;
   double* lds = reinterpret_cast<double*>(buf);
   const int items = (B / R) * M2;
   for (int it = threadIdx.x; it < items; it += kThreads) {
@@ -282,7 +298,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
     const int64_t nb0 = n0 + (int64_t)r * R;
     double xs[R + 4 * KK], h[2 * KK + 1];
 #pragma unroll
-    for (int u = 0; u < R + 4 * KK; ++u) xs[u] = ext_sample(x, L, (nb0 - 2 * KK + u) * M + k);
+    for (int u = 0; u < R + 4 * KK; ++u) xs[u] = ext_sample_s(x, L, (nb0 - 2 * KK + u) * M + k, sh);
 #pragma unroll
     for (int q = -KK; q <= KK; ++q) {
       const int t = k + M2 * q;
@@ -461,7 +477,7 @@ template <int KK, int LG, int R, int THR>
 __global__ __launch_bounds__(THR) void k_t2w_stream(
     const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int64_t N,
     const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int C,
-    int quadrature, double2* __restrict__ out) {
+    int quadrature, double2* __restrict__ out, const int64_t* __restrict__ shifts) {
   constexpr int M = 1 << LG, M2 = 2 * M, LG2 = LG + 1;
   constexpr int B = THR * R / M2;  // bins per tile
   constexpr int P = B * M / THR;   // new samples per thread and tile
@@ -480,12 +496,13 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
   const int64_t t0 = (blockIdx.x % chunks) * C;
   const int nt = (int)((tiles - t0) < C ? (tiles - t0) : C);
   const double* x = X + (int64_t)s * ldx;
+  const int64_t sh = shifts ? shifts[s] : 0;   // shifted copies of one series (ldx 0)
   double* lds = reinterpret_cast<double*>(buf);
   double* win = lds + 2 * B * M;
   double2* tws = reinterpret_cast<double2*>(win + WR * M);   // kRegTw false only
   const int64_t N0 = t0 * B;
   const int64_t base = (N0 - 2 * KK) * M;   // sample of ring row 0, column 0 at the start
-  for (int i = threadIdx.x; i < WR * M; i += THR) win[i] = ext_sample(x, L, base + i);
+  for (int i = threadIdx.x; i < WR * M; i += THR) win[i] = ext_sample_s(x, L, base + i, sh);
   if constexpr (!kRegTw) {
     for (int i = threadIdx.x; i < M; i += THR) tws[i] = tw[i];
   }
@@ -511,7 +528,7 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
       // ring rows (t + 1) B + 4 KK + 1 .. + B - 1, relative to the chunk's first row
       const int64_t nb = base + ((int64_t)(t + 1) * B + 4 * KK + 1) * M;
 #pragma unroll
-      for (int p = 0; p < P; ++p) pre[p] = ext_sample(x, L, nb + threadIdx.x + p * THR);
+      for (int p = 0; p < P; ++p) pre[p] = ext_sample_s(x, L, nb + threadIdx.x + p * THR, sh);
     }
     // fold: sample u of the thread's column (ring row t B + r R + u + krow,
     // wrapping at most once) adds h[(u - b) / 2] x to every bin b of the same
@@ -603,10 +620,10 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
 template <int LG, int R, int THR>
 void launch_stream(unsigned grid, hipStream_t st, const double* x, int64_t L, int64_t ldx,
                    int n_series, int64_t N, const double* filt, int m_H, const double2* tw,
-                   int C, int quadrature, double2* out) {
+                   int C, int quadrature, double2* out, const int64_t* shifts) {
   hipLaunchKernelGGL((k_t2w_stream<6, LG, R, THR>), dim3(grid), dim3(THR),
                      stream_lds_bytes(1 << LG, 6, THR, R), st, x, L, ldx, n_series, N, filt, m_H,
-                     tw, C, quadrature, out);
+                     tw, C, quadrature, out, shifts);
 }
 
 // The 1024-thread shape needs more than 64 KiB of dynamic LDS: allowed once per
@@ -626,12 +643,12 @@ bool stream_wide_ready() {
 template <int R>
 void launch_poly(dim3 grid, hipStream_t st, const double* x, int64_t L, int64_t ldx,
                  int n_series, int M, int64_t N, const double* filt, int m_H, const double2* tw,
-                 int log2_2M, int B, int quadrature, double2* out) {
+                 int log2_2M, int B, int quadrature, double2* out, const int64_t* shifts) {
   const size_t lds = (size_t)B * M * sizeof(double2);
   if ((m_H - 1) / (2 * M) == 6) {
     hipLaunchKernelGGL((k_t2w_poly_h<R, 6>), grid, dim3(kThreads), lds + M * sizeof(double2),
                        st, x, L, ldx,
-                       n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out);
+                       n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out, shifts);
   } else {
     hipLaunchKernelGGL((k_t2w_poly<R>), grid, dim3(kThreads), lds, st, x, L, ldx, n_series, M,
                        N, filt, m_H, tw, log2_2M, B, quadrature, out);
@@ -893,17 +910,17 @@ __global__ __launch_bounds__(kThreads) void k_w2t_poly(
 }
 constexpr int kW2tPolyR = 8;
 
-}  // namespace
-
-extern "C" {
-
-int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series, int M,
-                  int stride, const double* filter, int m_H, const double* twiddle,
-                  int quadrature, double* out, void* stream) {
+// pycwb_wdm_t2w, and with shifts (one per series, ldx 0) the transforms of
+// shifted copies of one series; shifts are honoured by the polyphase kernels
+// for m_H = 12 M + 1 (k_t2w_stream, k_t2w_poly_h) only.
+int t2w_impl(const double* x, int64_t n_samples, int64_t ldx, int n_series, int M, int stride,
+             const double* filter, int m_H, const double* twiddle, int quadrature, double* out,
+             void* stream, const int64_t* shifts) {
   if (!x || !filter || !twiddle || !out) return PYCWB_ERR_ARG;
   if (n_series < 1 || M < 1 || (M & (M - 1)) || 2 * M > kLdsDoubles / 2 || stride < 1 ||
-      m_H < 1 || n_samples < 1 || ldx < n_samples || n_samples <= m_H)
+      m_H < 1 || n_samples < 1 || (shifts ? ldx != 0 : ldx < n_samples) || n_samples <= m_H)
     return PYCWB_ERR_ARG;
+  if (shifts && !(stride == M && (m_H - 1) == 12 * M && M >= 2)) return PYCWB_ERR_ARG;
   (void)hipGetLastError();
   const int64_t padded = n_samples + (n_samples % stride ? stride - n_samples % stride : 0);
   const int64_t N = padded / stride;
@@ -927,14 +944,14 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
     double2* o = (double2*)out;
     const unsigned g = (unsigned)grid;
     switch (log2_2M - 1) {
-      case 1: launch_stream<1, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 2: launch_stream<2, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 3: launch_stream<3, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 4: launch_stream<4, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 5: launch_stream<5, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 6: launch_stream<6, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      case 7: launch_stream<7, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
-      default: launch_stream<9, 4, 1024>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o); break;
+      case 1: launch_stream<1, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
+      case 2: launch_stream<2, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
+      case 3: launch_stream<3, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
+      case 4: launch_stream<4, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
+      case 5: launch_stream<5, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
+      case 6: launch_stream<6, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
+      case 7: launch_stream<7, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
+      default: launch_stream<9, 4, 1024>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -959,10 +976,10 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
     const double2* tw = (const double2*)twiddle;
     double2* o = (double2*)out;
     switch (R) {
-      case 2: launch_poly<2>(g, st, x, n_samples, ldx, n_series, M, N, filter, m_H, tw, log2_2M, B, quadrature, o); break;
-      case 4: launch_poly<4>(g, st, x, n_samples, ldx, n_series, M, N, filter, m_H, tw, log2_2M, B, quadrature, o); break;
-      case 8: launch_poly<8>(g, st, x, n_samples, ldx, n_series, M, N, filter, m_H, tw, log2_2M, B, quadrature, o); break;
-      default: launch_poly<16>(g, st, x, n_samples, ldx, n_series, M, N, filter, m_H, tw, log2_2M, B, quadrature, o); break;
+      case 2: launch_poly<2>(g, st, x, n_samples, ldx, n_series, M, N, filter, m_H, tw, log2_2M, B, quadrature, o, shifts); break;
+      case 4: launch_poly<4>(g, st, x, n_samples, ldx, n_series, M, N, filter, m_H, tw, log2_2M, B, quadrature, o, shifts); break;
+      case 8: launch_poly<8>(g, st, x, n_samples, ldx, n_series, M, N, filter, m_H, tw, log2_2M, B, quadrature, o, shifts); break;
+      default: launch_poly<16>(g, st, x, n_samples, ldx, n_series, M, N, filter, m_H, tw, log2_2M, B, quadrature, o, shifts); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -971,6 +988,7 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
     }
     return PYCWB_OK;
   }
+  if (shifts) return PYCWB_ERR_ARG;
   int B = (kLdsDoubles / 2) / (2 * M);
   if (B > 64) B = 64;
   if (B < 1) B = 1;
@@ -986,6 +1004,25 @@ int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series,
     return PYCWB_ERR_HIP;
   }
   return PYCWB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pycwb_wdm_t2w(const double* x, int64_t n_samples, int64_t ldx, int n_series, int M,
+                  int stride, const double* filter, int m_H, const double* twiddle,
+                  int quadrature, double* out, void* stream) {
+  return t2w_impl(x, n_samples, ldx, n_series, M, stride, filter, m_H, twiddle, quadrature, out,
+                  stream, nullptr);
+}
+
+int pycwb_wdm_t2w_shifted(const double* ts, int64_t n_samples, const int64_t* shifts,
+                          int n_shifts, int M, const double* filter, int m_H,
+                          const double* twiddle, int quadrature, double* out, void* stream) {
+  if (!shifts) return PYCWB_ERR_ARG;
+  return t2w_impl(ts, n_samples, 0, n_shifts, M, M, filter, m_H, twiddle, quadrature, out, stream,
+                  shifts);
 }
 
 int pycwb_wdm_w2t(const double* map, int64_t N, int n_series, int M, const double* filter,

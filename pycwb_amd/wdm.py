@@ -301,6 +301,29 @@ class WDM:
         _lib.check(rc, "pycwb_wdm_t2w")
         return torch.view_as_complex(out).transpose(1, 2)
 
+    def t2w_shifted_device(self, ts: torch.Tensor, shifts: torch.Tensor, stream=None):
+        """Quadrature transforms (like t2w_device(x, MM=-1)) of shifted copies of
+        the device series ts, copy s shifted by shifts[s] (int64 device tensor;
+        k > 0: x[i] = ts[i + k] for i < L - k, k < 0: x[i] = ts[i + k] for
+        i >= -k).  The copies are never written out.  None when the filter is
+        not of the form m_H = 12 M + 1 that the shifted path supports."""
+        if self.m_H - 1 != 12 * self.M or self.M < 2:
+            return None
+        dev = ts.device
+        X = ts.reshape(-1).to(torch.float64).contiguous()
+        L = int(X.numel())
+        if L <= self.m_H:
+            raise ValueError(f"series of {L} samples is not longer than the filter ({self.m_H})")
+        sh = shifts.to(device=dev, dtype=torch.int64).contiguous()
+        n = int(sh.numel())
+        N = (L + (self.M - L % self.M if L % self.M else 0)) // self.M
+        filt, tw = self._device(dev)
+        out = torch.empty((n, N, self.M + 1, 2), dtype=torch.float64, device=dev)
+        rc = lib().pycwb_wdm_t2w_shifted(ptr(X), L, ptr(sh), n, self.M, ptr(filt), self.m_H,
+                                         ptr(tw), 1, ptr(out), stream_ptr(stream))
+        _lib.check(rc, "pycwb_wdm_t2w_shifted")
+        return torch.view_as_complex(out).transpose(1, 2)
+
     def w2t_device(self, tf: torch.Tensor, n_out: int, quad: bool = False,
                    stream=None) -> torch.Tensor:
         """Inverse transform on the GPU (WDM.cc:1536-1710): tf complex128

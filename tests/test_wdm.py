@@ -130,6 +130,35 @@ def test_gpu_t2w_matches_oracle(oracle, M, L, S):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("M,L", [(8, 20003), (64, 50000), (256, 70001), (512, 150000)])
+def test_gpu_t2w_shifted_equals_transforms_of_shifted_copies(M, L):
+    """pycwb_wdm_t2w_shifted (the copies read straight from the series) gives
+    bit for bit the maps of pycwb_wdm_t2w on the copies pycwb_max_energy_shift
+    writes out."""
+    import torch
+
+    from pycwb_amd import _lib
+    from pycwb_amd.device import lib, ptr
+    from pycwb_amd.wdm import WDM
+
+    print(f"case t2w shifted M={M} L={L}", flush=True)
+    dev = torch.device("cuda", 0)
+    w = WDM(M, M, 6, 10)
+    ts = torch.from_numpy(np.random.default_rng(M).normal(size=L)).to(dev)
+    shifts = torch.tensor([0, 3, -3, 17, -17, 4 * M + 1, -(4 * M + 1), L - 5, -(L - 5)],
+                          dtype=torch.int64, device=dev)
+    got = w.t2w_shifted_device(ts, shifts)
+    xs = torch.empty((shifts.numel(), L), dtype=torch.float64, device=dev)
+    _lib.check(lib().pycwb_max_energy_shift(ptr(ts), L, ptr(shifts), shifts.numel(), ptr(xs),
+                                            None), "pycwb_max_energy_shift")
+    want = w.t2w_device(xs, MM=-1)
+    torch.cuda.synchronize()
+    assert torch.equal(torch.view_as_real(got), torch.view_as_real(want))
+    # a filter of another length is declined (the caller writes the copies)
+    assert WDM(M, M, 4, 8).t2w_shifted_device(ts, shifts) is None
+
+
+@pytest.mark.gpu
 def test_gpu_wdm_object_surface():
     from pycwb_amd.wdm import WDM
 
