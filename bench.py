@@ -1036,7 +1036,7 @@ def maxenergy_leg(stream, log, M=64, seconds=600, rate=4096, max_delay_s=0.042, 
     md = int(rate * max_delay_s)
     n_maps = int(delay_shifts(md, ds, L).size)
     args = (w, md, ds, pattern, 1.0, rate // M, 32.0, 1024.0, rate / (2.0 * M))
-    max_energy_device(x[:8 * w.m_H], *args)      # warm (filter upload)
+    max_energy_device(x, *args, stream=stream)   # warm: filter upload, device buffers cached
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)

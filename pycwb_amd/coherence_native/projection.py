@@ -99,8 +99,9 @@ def max_energy_device(ts: torch.Tensor, wdm, max_delay: int, downsample: int, pa
     for k in range(9):
         d.p[k] = int(p[k])
     d.cur = ptr(cur)
+    shifts_dev = torch.from_numpy(shifts).to(dev)   # one upload for every batch
     for b0 in range(0, shifts.size, batch):
-        sh = torch.from_numpy(shifts[b0:b0 + batch]).to(dev)
+        sh = shifts_dev[b0:b0 + batch]
         n = int(sh.numel())
         # the transforms read the shifted copies straight from ts where the
         # filter allows it; otherwise the copies are written out first
