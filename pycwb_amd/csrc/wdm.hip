@@ -61,14 +61,12 @@ __device__ __forceinline__ unsigned bitrev(unsigned v, int bits) {
 __device__ __forceinline__ int swz(int i) { return i ^ (((i >> 4) & 3) * 5); }
 
 // In-place radix-2 DIT FFT, exp(-2 pi i f k / 2M), of `count` sequences of
-// length 2M = 2^lg stored back to back in LDS (input in bit-reversed order;
-// element i at swz(i) when SW).  Stages run in pairs as radix-4 butterflies
-// (one barrier per pair), the last stage alone when lg is odd; power-of-two
-// index arithmetic throughout.  tw[k] = exp(-2 pi i k / 2M), k < M.
-template <bool SW = false>
+// length 2M = 2^lg stored back to back in LDS (input in bit-reversed order).
+// Stages run in pairs as radix-4 butterflies (one barrier per pair), the last
+// stage alone when lg is odd; power-of-two index arithmetic throughout.
+// tw[k] = exp(-2 pi i k / 2M), k < M.
 __device__ __forceinline__ void lds_fft(double2* __restrict__ buf, int count, int lg,
                                         const double2* __restrict__ tw) {
-  auto at = [](int i) { return SW ? swz(i) : i; };
   const int M2 = 1 << lg;
   const int qpl = lg - 2;  // log2 of the radix-4 butterflies per sequence
   int lh = 0;
@@ -81,7 +79,7 @@ __device__ __forceinline__ void lds_fft(double2* __restrict__ buf, int count, in
       const int i0 = (p << lg) + (g << (lh + 2)) + j;
       const double2 w1 = tw[j << (lg - 1 - lh)];
       const double2 w2a = tw[j << (lg - 2 - lh)], w2b = tw[(j + h) << (lg - 2 - lh)];
-      const int p0 = at(i0), p1 = at(i0 + h), p2 = at(i0 + 2 * h), p3 = at(i0 + 3 * h);
+      const int p0 = i0, p1 = i0 + h, p2 = i0 + 2 * h, p3 = i0 + 3 * h;
       const double2 x0 = buf[p0], x1 = buf[p1], x2 = buf[p2], x3 = buf[p3];
       // stage h: (x0, x1) and (x2, x3) with w1
       double vr = x1.x * w1.x - x1.y * w1.y, vi = x1.x * w1.y + x1.y * w1.x;
@@ -106,7 +104,7 @@ __device__ __forceinline__ void lds_fft(double2* __restrict__ buf, int count, in
     const int n2 = count << (lg - 1);
     for (int e = threadIdx.x; e < n2; e += blockDim.x) {
       const int p = e >> (lg - 1), j = e & (h - 1);
-      const int i0 = at((p << lg) + j), i1 = at((p << lg) + j + h);
+      const int i0 = (p << lg) + j, i1 = i0 + h;
       const double2 w = tw[j];
       const double2 u = buf[i0], v = buf[i1];
       const double vr = v.x * w.x - v.y * w.y, vi = v.x * w.y + v.y * w.x;
@@ -374,9 +372,13 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
 // written over the ring rows this tile no longer needs.  A thread's 2 KK + 1
 // filter taps stay in registers for the whole chunk.  Two shapes:
 //   M <= 128:  THR = 256, R = 8 (tiles of 1,024 samples); the FFT twiddles of a
-//              thread's butterflies are kept in registers too (stream_fft);
+//              thread's butterflies are kept in registers too (stream_fft) and
+//              the FFT buffer is in the swizzled layout (swz);
 //   M = 512:   THR = 1024, R = 4 (tiles of 4 bins, 156 KiB of LDS, one block
-//              per CU); the FFT is lds_fft with its twiddles in LDS.
+//              per CU); the FFT is lds_fft with its twiddles in LDS, the buffer
+//              plain.
+// Series s may be a shifted copy of one series (shifts, ldx 0), read through
+// ext_sample_s.
 // The fold forms the same terms in the same order as k_t2w_poly_h (each window
 // sample is added to every bin that uses it as soon as it is read, so each bin
 // sums in ascending q), the FFT runs the same stages as lds_fft and the split
