@@ -1280,13 +1280,20 @@ constexpr int kTicketsPerGroup = PYCWB_XCD_QUEUES ? 16 : 2;  // per phase: 8 que
 #ifndef PYCWB_OCC_SCAN_LARGE
 #define PYCWB_OCC_SCAN_LARGE 4
 #endif
+// five or more detectors: at 4 waves the scan spills 18 VGPRs, at 3 none;
+// measured on MI355X (five-detector order-8 batch of 2,607 clusters) 0.778
+// against 0.799 s for the regulator + scan + argmax
+#ifndef PYCWB_OCC_SCAN_WIDE
+#define PYCWB_OCC_SCAN_WIDE 3
+#endif
 template <int PHASE, int N>
 struct PhaseOccupancy {
   static constexpr int waves =
       PHASE == 0 ? (N <= 3 ? PYCWB_OCC_REG_SMALL : PYCWB_OCC_REG_LARGE)
-                 : ((PHASE == 1 || PHASE == 3) ? (N <= 3 ? PYCWB_OCC_SCAN_SMALL
-                                                         : PYCWB_OCC_SCAN_LARGE)
-                                               : 8);
+                 : ((PHASE == 1 || PHASE == 3)
+                        ? (N <= 3 ? PYCWB_OCC_SCAN_SMALL
+                                  : (N == 4 ? PYCWB_OCC_SCAN_LARGE : PYCWB_OCC_SCAN_WIDE))
+                        : 8);
 };
 
 // Regulator work item: reg_pool (at most kRegPool) tiles of BLOCK directions,
