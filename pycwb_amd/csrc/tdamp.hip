@@ -212,17 +212,22 @@ struct PixTaps {
   double t0[kPixTaps], tx[kPixTaps];
 };
 
+// v is 16-byte aligned: taps 2j and 2j + 1 are read as one pair (the even
+// one to e, the odd one to o, in ascending order as before)
 __device__ __forceinline__ void pix_sums(const double* __restrict__ v, const double* tap,
                                          double& se, double& so) {
   // one sum's LDS reads at a time: the scheduler may not hoist the next sum's
   // loads into this one (all six at once need more than the register file)
   __builtin_amdgcn_sched_barrier(0);
+  const double2* v2 = reinterpret_cast<const double2*>(v);
   double e = 0.0, o = 0.0;
 #pragma unroll
-  for (int k = 0; k < kPixTaps; ++k) {
-    if (k & 1) o += v[k] * tap[k];
-    else e += v[k] * tap[k];
+  for (int j = 0; j < kPixTaps / 2; ++j) {
+    const double2 x = v2[j];
+    e += x.x * tap[2 * j];
+    o += x.y * tap[2 * j + 1];
   }
+  e += v[kPixTaps - 1] * tap[kPixTaps - 1];
   se = e;
   so = o;
 }
@@ -281,13 +286,18 @@ __host__ __device__ inline int td_chunk_rows(int c, int K, int J, int& s_lo, int
 }
 
 __global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int n_chunks, int NS2) {
-  extern __shared__ double stage[];   // two buffers of NS2 doubles: (plane, layer, row)
+  // two buffers of NS2 doubles: (copy, plane, layer, row), rows of each slice
+  // NRp = NR rounded up to even; copy 1 starts one row later than copy 0, so
+  // every lane's 25 rows begin at an even (16-byte aligned) slice position
+  extern __shared__ double2 stage2[];
+  double* stage = reinterpret_cast<double*>(stage2);
   const int c = (int)(blockIdx.x % (unsigned)n_chunks);
   const int64_t g = blockIdx.x / (unsigned)n_chunks;
   const int64_t G = gridDim.x / (unsigned)n_chunks;
   int s_min, s_max;
   const int NR = td_chunk_rows(c, a.K, a.J, s_min, s_max);
-  const int NS = 6 * NR;
+  const int NRp = (NR + 1) & ~1;
+  const int NS = 12 * NRp;
   const int half = 2 * a.K + 1;
   const int M1 = a.M + 1;
   const int ki = c * kPixThreads + threadIdx.x;
@@ -296,6 +306,8 @@ __global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int n_chunk
   const int sh = trunc_div(dT, a.J);
   const int sub = dT - sh * a.J;
   const int off = s_max - sh;   // row ne + k of the lane is staged row off + k
+  // copy and aligned slice position of the lane's first row
+  const int lane_base = (off & 1) ? 6 * NRp + off - 1 : off;
   PixTaps T;
   {
     const int64_t D = 2 * (int64_t)a.J + 1;
@@ -305,7 +317,8 @@ __global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int n_chunk
       T.tx[k] = a.Tx[k * D + sub + a.J];
     }
   }
-  // staged value i: plane i / (3 NR), layer m - 1 + (i / NR) % 3, row n - s_max + i % NR
+  // staged value i: copy i / (6 NRp), then plane, layer m - 1 + L, and row
+  // n - s_max + r + copy
   auto fetch = [&](int64_t p, float (&v)[kPixStage]) {
     const int idx = a.pix[p];
     const int n = idx / M1, m = idx - n * M1;
@@ -314,9 +327,10 @@ __global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int n_chunk
       const int i = threadIdx.x + j * kPixThreads;
       float x = 0.f;
       if (i < NS) {
-        const int pl = i / (3 * NR), rem = i - pl * 3 * NR;
-        const int layer = m - 1 + rem / NR;
-        const int64_t row = (int64_t)n - s_max + rem % NR;
+        const int cp = i / (6 * NRp), rem0 = i - cp * 6 * NRp;
+        const int pl = rem0 / (3 * NRp), rem = rem0 - pl * 3 * NRp;
+        const int layer = m - 1 + rem / NRp;
+        const int64_t row = (int64_t)n - s_max + rem % NRp + cp;
         if (layer >= 0 && layer <= a.M && row >= 0 && row < a.n_rows)
           x = (pl ? a.p90 : a.p00)[row * M1 + layer];
       }
@@ -343,12 +357,12 @@ __global__ __launch_bounds__(kPixThreads) void k_td_pixels(TdArgs a, int n_chunk
       // odd whole-bin shifts swap the quadratures (getTDamp): a00 from the 90
       // plane, a90 from the 00 plane, one of them negated by the (n + m) parity
       const bool swap = (sh & 1) != 0;
-      const double* q00 = sb + off;            // plane 00, layer m - 1
-      const double* q90 = sb + 3 * NR + off;   // plane 90, layer m - 1
+      const double* q00 = sb + lane_base;             // plane 00, layer m - 1
+      const double* q90 = sb + lane_base + 3 * NRp;   // plane 90, layer m - 1
       const double* A = swap ? q90 : q00;
       const double* Bq = swap ? q00 : q90;
-      double a00 = pix_amp(a, ne, m, sub, A, A + NR, A + 2 * NR, T, swap);
-      double a90 = pix_amp(a, ne, m, sub, Bq, Bq + NR, Bq + 2 * NR, T, !swap);
+      double a00 = pix_amp(a, ne, m, sub, A, A + NRp, A + 2 * NRp, T, swap);
+      double a90 = pix_amp(a, ne, m, sub, Bq, Bq + NRp, Bq + 2 * NRp, T, !swap);
       if (swap) {
         if ((n + m) & 1) a00 = -a00;
         else a90 = -a90;
@@ -390,14 +404,15 @@ int pycwb_td_vectors(const int32_t* pixel_index, int64_t n_pixels, const float* 
     const int nr = td_chunk_rows(c, K, J, lo, hi);
     NRmax = nr > NRmax ? nr : NRmax;
   }
-  if (2 * n_coeffs + 1 == kPixTaps && 6 * NRmax <= kPixStage * kPixThreads) {
+  const int NRpmax = (NRmax + 1) & ~1;
+  if (2 * n_coeffs + 1 == kPixTaps && 12 * NRpmax <= kPixStage * kPixThreads) {
     int64_t G = 8192 / n_chunks;
     G = G < 1 ? 1 : (G > n_pixels ? n_pixels : G);
     const int64_t grid = G * n_chunks;
     if (grid > 0x7fffffff) return PYCWB_ERR_ARG;
     hipLaunchKernelGGL(k_td_pixels, dim3((unsigned)grid), dim3(kPixThreads),
-                       (size_t)2 * 6 * NRmax * sizeof(double), (hipStream_t)stream, a, n_chunks,
-                       6 * NRmax);
+                       (size_t)2 * 12 * NRpmax * sizeof(double), (hipStream_t)stream, a,
+                       n_chunks, 12 * NRpmax);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       pycwb::record_error("k_td_pixels", hipGetErrorString(e));
