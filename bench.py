@@ -347,7 +347,7 @@ def wdm_leg(stream, log, seconds=600, rate=4096, n_det=5):
         w.t2w_device(x)  # warm (filter upload)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        reps = 5
+        reps = 20   # back-to-back calls: the mean is the kernel's time plus small launch gaps
         e0.record(stream)
         for _ in range(reps):
             y = w.t2w_device(x, stream=stream)
