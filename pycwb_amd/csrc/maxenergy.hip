@@ -102,6 +102,81 @@ __global__ void k_me_packet_max(PacketArgs a) {
   a.cur[j] = best;
 }
 
+// The packet pass with each map's pixels staged in LDS: a block owns kPkTile
+// consecutive pixels and, map by map, loads the span they and their 8
+// neighbours reach (offsets lo .. hi of the flat index) with coalesced reads,
+// each position holding map[clamp(index)] as packet() reads it; packet_tile
+// then forms the same sums in the same order, so the maximum is identical.
+// (packet() alone reads each map value 9 times through the caches.)
+constexpr int kPkTile = 1024;
+constexpr int kPkPer = kPkTile / kBlock;
+
+__device__ __forceinline__ double packet_tile(const double2* __restrict__ tile, int64_t pos,
+                                              int64_t j, int mm, const PacketArgs& a) {
+  const double2 c = tile[pos];
+  const double re0 = c.x;
+  if (j < a.jb || j >= a.je) return re0 > 0.0 ? re0 : 0.0;
+  if (mm < a.mL || mm > a.mH) return 0.0;
+  double ss = 0.0, ee = 0.0, EE = 0.0;
+#pragma unroll
+  for (int n = 1; n < 9; ++n) {
+    const double2 q = tile[pos + a.p[n]];
+    ss += q.x * q.y;
+    ee += q.x * q.x;
+    EE += q.y * q.y;
+  }
+  const double im0 = c.y;
+  ss += re0 * im0 * (a.mean - 8.0);
+  ee += re0 * re0 * (a.mean - 8.0);
+  EE += im0 * im0 * (a.mean - 8.0);
+  const double cc = ee - EE, s2 = 2.0 * ss;
+  const double nn2 = cc * cc + s2 * s2;
+  double nn = nn2 > 0.0 ? sqrt(nn2) : 0.0;
+  const double se = ee + EE;
+  if (se < nn) nn = se;
+  const double h1 = (se + nn) / 2.0, h2 = (se - nn) / 2.0;
+  const double aa = (h1 > 0.0 ? sqrt(h1) : 0.0) + (h2 > 0.0 ? sqrt(h2) : 0.0);
+  return a.mean == 1.0 ? se / 2.0 : aa * aa / 4.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_me_packet_tiled(PacketArgs a, int64_t lo,
+                                                            int64_t span) {
+  extern __shared__ double2 tile[];   // span values: flat indices j0 + lo .. j0 + lo + span - 1
+  const int64_t J = a.T * a.M1;
+  const int64_t j0 = (int64_t)blockIdx.x * kPkTile;
+  double best[kPkPer];
+  int mmv[kPkPer];
+#pragma unroll
+  for (int q = 0; q < kPkPer; ++q) {
+    const int64_t j = j0 + threadIdx.x + q * kBlock;
+    mmv[q] = (int)(j % a.M1);
+    best[q] = (!a.init && j < J) ? a.cur[j] : 0.0;
+  }
+  for (int s = 0; s < a.n_maps; ++s) {
+    const double2* m = a.maps + (int64_t)s * J;
+    __syncthreads();   // the previous map's reads are done
+    for (int64_t i = threadIdx.x; i < span; i += kBlock) {
+      int64_t idx = j0 + lo + i;
+      idx = idx < 0 ? 0 : (idx >= J ? J - 1 : idx);
+      tile[i] = m[idx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPkPer; ++q) {
+      const int64_t j = j0 + threadIdx.x + q * kBlock;
+      if (j >= J) continue;
+      const double e = packet_tile(tile, j - j0 - lo, j, mmv[q], a);
+      if (a.init && s == 0) best[q] = e;
+      else if (e > best[q]) best[q] = e;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kPkPer; ++q) {
+    const int64_t j = j0 + threadIdx.x + q * kBlock;
+    if (j < J) a.cur[j] = best[q];
+  }
+}
+
 __global__ void k_me_finish(double* cur, int64_t T, int M1, int two) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= T) return;
@@ -140,6 +215,18 @@ int pycwb_max_energy_packet(const pycwb_packet_desc* d, void* stream) {
   for (int k = 0; k < 9; ++k) a.p[k] = d->p[k];
   a.init = d->init; a.cur = d->cur;
   const int64_t J = d->T * d->M1;
+  int64_t lo = 0, hi = 0;
+  for (int k = 1; k < 9; ++k) {
+    lo = a.p[k] < lo ? a.p[k] : lo;
+    hi = a.p[k] > hi ? a.p[k] : hi;
+  }
+  const int64_t span = kPkTile + hi - lo;
+  if (span * (int64_t)sizeof(double2) <= 64 * 1024) {
+    hipLaunchKernelGGL(k_me_packet_tiled, dim3((unsigned)((J + kPkTile - 1) / kPkTile)),
+                       dim3(kBlock), (size_t)span * sizeof(double2), (hipStream_t)stream, a, lo,
+                       span);
+    return check_launch("k_me_packet_tiled");
+  }
   hipLaunchKernelGGL(k_me_packet_max, dim3((unsigned)((J + kBlock - 1) / kBlock)), dim3(kBlock),
                      0, (hipStream_t)stream, a);
   return check_launch("k_me_packet_max");
