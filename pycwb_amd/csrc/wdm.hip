@@ -294,9 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
   for (int it = threadIdx.x; it < items; it += kThreads) {
     const int k = it & (M2 - 1), r = it >> log2_2M;
     const int64_t nb0 = n0 + (int64_t)r * R;
-    double xs[R + 4 * KK], h[2 * KK + 1];
-#pragma unroll
-    for (int u = 0; u < R + 4 * KK; ++u) xs[u] = ext_sample_s(x, L, (nb0 - 2 * KK + u) * M + k, sh);
+    double h[2 * KK + 1];
 #pragma unroll
     for (int q = -KK; q <= KK; ++q) {
       const int t = k + M2 * q;
@@ -306,10 +304,17 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
     double acc[R];
 #pragma unroll
     for (int b = 0; b < R; ++b) acc[b] = 0.0;
+    // sample u = b + 2(q + KK) of the window goes to every bin b it reaches as
+    // it arrives (each bin sums in ascending q); at most 8 loads ahead
 #pragma unroll
-    for (int q = -KK; q <= KK; ++q) {
+    for (int u = 0; u < R + 4 * KK; ++u) {
+      if (u % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+      const double v = ext_sample_s(x, L, (nb0 - 2 * KK + u) * M + k, sh);
 #pragma unroll
-      for (int b = 0; b < R; ++b) acc[b] += h[q + KK] * xs[b + 2 * (q + KK)];
+      for (int b = 0; b < R; ++b) {
+        const int d = u - b;
+        if (d >= 0 && !(d & 1) && d <= 4 * KK) acc[b] += h[d >> 1] * v;
+      }
     }
     const int kr = (int)bitrev((unsigned)k, log2_2M);
 #pragma unroll
