@@ -935,43 +935,62 @@ __device__ __forceinline__ unsigned long long scan_lane(
   sums_init(acc);
   unsigned n_tested = 0;  // candidate pixels (COUNT_TESTED)
   const int64_t pstride = (int64_t)N * nd;
-  for (int j0 = 0; j0 < P; j0 += 64) {
-    const int left = P - j0;
-    unsigned long long cand = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
-    if (use_bound) {
-      const int j = j0 + lane;
-      const bool c = j < P && pixel_candidate<N>(td + (int64_t)j * pstride, nd, lo, hi, En);
-      cand = __ballot(c);
-    }
-    if constexpr (COUNT_TESTED) n_tested += (unsigned)__popcll(cand);
-    if (!cand) continue;
-    // software pipeline: the next candidate's amplitudes load while this one computes
-    int b = __ffsll((long long)cand) - 1;
-    cand &= cand - 1ull;
-    float2 cur[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) cur[i] = td[(int64_t)(j0 + b) * pstride + tap[i]];
-    for (;;) {
-      const int j = j0 + b;
-      const bool more = cand != 0ull;
-      const int bn = more ? __ffsll((long long)cand) - 1 : b;
-      cand &= cand - 1ull;
-      float2 nxt[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) nxt[i] = td[(int64_t)(j0 + bn) * pstride + tap[i]];
-      float v00[N], v90[N];
+  if (!skip_masked) {
+    // every lane evaluates every pixel itself, in increasing order
+    if constexpr (COUNT_TESTED) n_tested = (unsigned)P;
+    for (int j = 0; j < P; ++j) {
+      float v00[N], v90[N], r[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        v00[i] = cur[i].x;
-        v90[i] = cur[i].y;
+        const float2 v = td[(int64_t)j * pstride + tap[i]];
+        v00[i] = v.x;
+        v90[i] = v.y;
+        r[i] = rms[(int64_t)j * N + i];
       }
-      if (skip_masked) {
-        float e, et;
-        int m;
-        pixel_energy<N>(v00, v90, En, e, m, et);
-        const bool contrib = pixel_contributes(e, m);
-        const unsigned long long bm = __ballot(contrib);
-        if (bm) {
+      scan_pixel<N>(v00, v90, r, fp0, fx0, rr, RR, En, false, acc);
+    }
+  } else {
+    for (int j0 = 0; j0 < P; j0 += 64) {
+      const int left = P - j0;
+      unsigned long long cand = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
+      if (use_bound) {
+        const int j = j0 + lane;
+        const bool c = j < P && pixel_candidate<N>(td + (int64_t)j * pstride, nd, lo, hi, En);
+        cand = __ballot(c);
+      }
+      if constexpr (COUNT_TESTED) n_tested += (unsigned)__popcll(cand);
+      if (!cand) continue;
+      // software pipeline: the next candidate's amplitudes load while this one
+      // is tested.  The inner loop only tests and enqueues; the flush runs
+      // between its passes, at the same points of the pixel order (right after
+      // the enqueue that fills 64 slots), so the sums keep no copies live
+      // through the candidate loop.
+      int b = __ffsll((long long)cand) - 1;
+      cand &= cand - 1ull;
+      float2 cur[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) cur[i] = td[(int64_t)(j0 + b) * pstride + tap[i]];
+      bool more = true;
+      while (more) {
+        for (;;) {
+          const int j = j0 + b;
+          more = cand != 0ull;
+          const int bn = more ? __ffsll((long long)cand) - 1 : b;
+          cand &= cand - 1ull;
+          float2 nxt[N];
+#pragma unroll
+          for (int i = 0; i < N; ++i) nxt[i] = td[(int64_t)(j0 + bn) * pstride + tap[i]];
+          float v00[N], v90[N];
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            v00[i] = cur[i].x;
+            v90[i] = cur[i].y;
+          }
+          float e, et;
+          int m;
+          pixel_energy<N>(v00, v90, En, e, m, et);
+          const bool contrib = pixel_contributes(e, m);
+          const unsigned long long bm = __ballot(contrib);
           if (contrib) {
             const int slot = T + __popcll(bm & lanemask_lt(lane));
 #pragma unroll
@@ -982,26 +1001,22 @@ __device__ __forceinline__ unsigned long long scan_lane(
             q.qjd[slot] = (j << 6) | lane;
           }
           T += __popcll(bm);
-          if (T >= 64) {
-            wave_sync();
-            queue_flush<N>(q, 64, T, lane, rms, P, rr, RR, En, acc);
-          }
+          if (!more) break;
+          b = bn;
+#pragma unroll
+          for (int i = 0; i < N; ++i) cur[i] = nxt[i];
+          if (T >= 64) break;
         }
-      } else {
-        float r[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) r[i] = rms[(int64_t)j * N + i];
-        scan_pixel<N>(v00, v90, r, fp0, fx0, rr, RR, En, false, acc);
+        if (T >= 64) {
+          wave_sync();
+          queue_flush<N>(q, 64, T, lane, rms, P, rr, RR, En, acc);
+        }
       }
-      if (!more) break;
-      b = bn;
-#pragma unroll
-      for (int i = 0; i < N; ++i) cur[i] = nxt[i];
     }
-  }
-  if (T > 0) {
-    wave_sync();
-    queue_flush<N>(q, T, T, lane, rms, P, rr, RR, En, acc);
+    if (T > 0) {
+      wave_sync();
+      queue_flush<N>(q, T, T, lane, rms, P, rr, RR, En, acc);
+    }
   }
   if constexpr (COUNT_TESTED) {
     const unsigned lanes = (unsigned)__popcll(__ballot(active));
