@@ -803,7 +803,7 @@ constexpr int kQueue = 128;  // pair slots: at most 63 waiting + 64 of one pixel
 
 template <int N>
 struct WaveQueue {
-  float qv[kQueue][2 * N];        // v00 (0..N) and v90 (N..2N) of each pair
+  float2 qv[kQueue][N];           // (v00, v90) of each detector of each pair
   int qjd[kQueue];                // (pixel << 6) | owner lane
   unsigned long long own[64];     // per owner lane: its slots among those evaluated
   float pat[64][2 * N];           // F+ (0..N) and Fx (N..2N) of each lane's direction
@@ -826,8 +826,9 @@ __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, in
     float v00[N], v90[N], r[N], fp0[N], fx0[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      v00[i] = q.qv[lane][i];
-      v90[i] = q.qv[lane][N + i];
+      const float2 v = q.qv[lane][i];
+      v00[i] = v.x;
+      v90[i] = v.y;
       fp0[i] = q.pat[d][i];
       fx0[i] = q.pat[d][N + i];
       r[i] = rms[(int64_t)j * N + i];
@@ -874,17 +875,17 @@ __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, in
   }
   // move the waiting pairs to the front
   const bool mv = lane + cnt < T;
-  float v[2 * N];
+  float2 v[N];
   int jd = 0;
   if (mv) {
 #pragma unroll
-    for (int i = 0; i < 2 * N; ++i) v[i] = q.qv[lane + cnt][i];
+    for (int i = 0; i < N; ++i) v[i] = q.qv[lane + cnt][i];
     jd = q.qjd[lane + cnt];
   }
   wave_sync();
   if (mv) {
 #pragma unroll
-    for (int i = 0; i < 2 * N; ++i) q.qv[lane][i] = v[i];
+    for (int i = 0; i < N; ++i) q.qv[lane][i] = v[i];
     q.qjd[lane] = jd;
   }
   T -= cnt;
@@ -935,6 +936,22 @@ __device__ __forceinline__ unsigned long long scan_lane(
   sums_init(acc);
   unsigned n_tested = 0;  // candidate pixels (COUNT_TESTED)
   const int64_t pstride = (int64_t)N * nd;
+  // the cluster's amplitudes are the same for every lane of the wavefront:
+  // held in scalar registers, each candidate's row address is scalar and the
+  // lanes add only their 32-bit tap offsets
+  const float2* tdw;
+  {
+    const uint64_t u = (uint64_t)td;
+    const uint32_t lo32 = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi32 = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    tdw = (const float2*)(((uint64_t)hi32 << 32) | lo32);
+  }
+  uint32_t toff[N];  // byte offset of each detector's tap within a pixel row
+#pragma unroll
+  for (int i = 0; i < N; ++i) toff[i] = (uint32_t)tap[i] * (uint32_t)sizeof(float2);
+  auto tap_load = [&](const float2* row, int i) -> float2 {
+    return *(const float2*)((const char*)row + toff[i]);
+  };
   if (!skip_masked) {
     // every lane evaluates every pixel itself, in increasing order
     if constexpr (COUNT_TESTED) n_tested = (unsigned)P;
@@ -961,51 +978,63 @@ __device__ __forceinline__ unsigned long long scan_lane(
       if constexpr (COUNT_TESTED) n_tested += (unsigned)__popcll(cand);
       if (!cand) continue;
       // software pipeline: the next candidate's amplitudes load while this one
-      // is tested.  The inner loop only tests and enqueues; the flush runs
-      // between its passes, at the same points of the pixel order (right after
-      // the enqueue that fills 64 slots), so the sums keep no copies live
-      // through the candidate loop.
+      // is tested, into the other of two buffers (no register copies).  The
+      // inner loops only test and enqueue; the flush runs between their passes,
+      // at the same points of the pixel order (right after the enqueue that
+      // fills 64 slots), so the sums keep no copies live through the loop.
       int b = __ffsll((long long)cand) - 1;
       cand &= cand - 1ull;
-      float2 cur[N];
+      float2 bufA[N], bufB[N];
+      {
+        const float2* row = tdw + (int64_t)(j0 + b) * pstride;
 #pragma unroll
-      for (int i = 0; i < N; ++i) cur[i] = td[(int64_t)(j0 + b) * pstride + tap[i]];
+        for (int i = 0; i < N; ++i) bufA[i] = tap_load(row, i);
+      }
       bool more = true;
+      // tests pixel j0 + b from c, loads the next candidate into n; false when
+      // the chunk is done or the queue holds 64 pairs
+      auto step = [&](const float2 (&c)[N], float2 (&n)[N]) -> bool {
+        const int j = j0 + b;
+        more = cand != 0ull;
+        const int bn = more ? __ffsll((long long)cand) - 1 : b;
+        cand &= cand - 1ull;
+        const float2* row = tdw + (int64_t)(j0 + bn) * pstride;
+#pragma unroll
+        for (int i = 0; i < N; ++i) n[i] = tap_load(row, i);
+        float v00[N], v90[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          v00[i] = c[i].x;
+          v90[i] = c[i].y;
+        }
+        float e, et;
+        int m;
+        pixel_energy<N>(v00, v90, En, e, m, et);
+        const bool contrib = pixel_contributes(e, m);
+        const unsigned long long bm = __ballot(contrib);
+        if (contrib) {
+          const int slot = T + __popcll(bm & lanemask_lt(lane));
+#pragma unroll
+          for (int i = 0; i < N; ++i) q.qv[slot][i] = c[i];
+          q.qjd[slot] = (j << 6) | lane;
+        }
+        T += __popcll(bm);
+        if (!more) return false;
+        b = bn;
+        return T < 64;
+      };
+      bool curA = true;  // the next candidate's amplitudes are in bufA (wave-uniform)
       while (more) {
-        for (;;) {
-          const int j = j0 + b;
-          more = cand != 0ull;
-          const int bn = more ? __ffsll((long long)cand) - 1 : b;
-          cand &= cand - 1ull;
-          float2 nxt[N];
-#pragma unroll
-          for (int i = 0; i < N; ++i) nxt[i] = td[(int64_t)(j0 + bn) * pstride + tap[i]];
-          float v00[N], v90[N];
-#pragma unroll
-          for (int i = 0; i < N; ++i) {
-            v00[i] = cur[i].x;
-            v90[i] = cur[i].y;
+        if (curA) {
+          for (;;) {
+            if (!step(bufA, bufB)) { curA = false; break; }
+            if (!step(bufB, bufA)) { curA = true; break; }
           }
-          float e, et;
-          int m;
-          pixel_energy<N>(v00, v90, En, e, m, et);
-          const bool contrib = pixel_contributes(e, m);
-          const unsigned long long bm = __ballot(contrib);
-          if (contrib) {
-            const int slot = T + __popcll(bm & lanemask_lt(lane));
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-              q.qv[slot][i] = v00[i];
-              q.qv[slot][N + i] = v90[i];
-            }
-            q.qjd[slot] = (j << 6) | lane;
+        } else {
+          for (;;) {
+            if (!step(bufB, bufA)) { curA = true; break; }
+            if (!step(bufA, bufB)) { curA = false; break; }
           }
-          T += __popcll(bm);
-          if (!more) break;
-          b = bn;
-#pragma unroll
-          for (int i = 0; i < N; ++i) cur[i] = nxt[i];
-          if (T >= 64) break;
         }
         if (T >= 64) {
           wave_sync();
