@@ -813,8 +813,10 @@ struct WaveQueue {
 };
 
 // Evaluate slots [0, cnt), add them to their owners' sums, drop them from the
-// queue.  own[] is zero on entry and on exit.
-template <int N>
+// queue.  own[] is zero on entry and on exit.  Without MAPS only the sums the
+// sky statistic needs are carried (the map-only terms are neither formed nor
+// stored).
+template <int N, bool MAPS>
 __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, int lane,
                                             const float* __restrict__ rms, int P, float rr,
                                             float RR, double En, SkySums& acc) {
@@ -842,11 +844,13 @@ __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, in
     q.c_ec[lane] = o.ec;
     q.c_gn[lane] = o.gn;
     q.c_rn[lane] = o.rn;
-    q.c_fpet[lane] = o.d.fp * o.et;
-    q.c_fxet[lane] = o.d.fx * o.et;
+    if constexpr (MAPS) {
+      q.c_fpet[lane] = o.d.fp * o.et;
+      q.c_fxet[lane] = o.d.fx * o.et;
+    }
     q.c_llcc[lane] = o.ll * o.ccd;
     q.c_ll[lane] = o.ll;
-    q.c_flags[lane] = o.m | (o.ec > 1e-9f ? 2 : 0) | (o.mk > 0.f ? 4 : 0);
+    q.c_flags[lane] = MAPS ? (o.m | (o.ec > 1e-9f ? 2 : 0) | (o.mk > 0.f ? 4 : 0)) : o.m;
     __hip_atomic_fetch_or(&q.own[d], 1ull << lane, __ATOMIC_RELAXED,
                           __HIP_MEMORY_SCOPE_WORKGROUP);
   }
@@ -866,6 +870,7 @@ __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, in
     acc.EC += q.c_ec[slot];
     acc.GN += q.c_gn[slot];
     acc.RN += q.c_rn[slot];
+    if constexpr (!MAPS) continue;
     acc.Mp += (f & 2) ? 1.0 : 0.0;
     if (f & 4) {
       acc.sde += et;
@@ -901,7 +906,7 @@ __device__ __forceinline__ void queue_flush(WaveQueue<N>& q, int cnt, int& T, in
 // COUNT_TESTED (a measurement pass) adds to *tested the (direction, pixel)
 // pairs whose energy the wavefront evaluates: candidate pixels times its
 // directions.
-template <int N, int WAVES, bool COUNT_TESTED = false>
+template <int N, int WAVES, bool COUNT_TESTED = false, bool MAPS = true>
 __device__ __forceinline__ unsigned long long scan_lane(
     const float* __restrict__ FP, const float* __restrict__ FX, const int32_t* __restrict__ ml,
     int64_t S, const float* __restrict__ rms, const float2* __restrict__ td, int P, int K,
@@ -1038,13 +1043,13 @@ __device__ __forceinline__ unsigned long long scan_lane(
         }
         if (T >= 64) {
           wave_sync();
-          queue_flush<N>(q, 64, T, lane, rms, P, rr, RR, En, acc);
+          queue_flush<N, MAPS>(q, 64, T, lane, rms, P, rr, RR, En, acc);
         }
       }
     }
     if (T > 0) {
       wave_sync();
-      queue_flush<N>(q, T, T, lane, rms, P, rr, RR, En, acc);
+      queue_flush<N, MAPS>(q, T, T, lane, rms, P, rr, RR, En, acc);
     }
   }
   if constexpr (COUNT_TESTED) {
@@ -1053,7 +1058,7 @@ __device__ __forceinline__ unsigned long long scan_lane(
   }
   SkyStat st;
   sky_epilogue(acc, N, netCC, st);
-  if (maps && active) {
+  if (MAPS && maps && active) {
 #pragma unroll
     for (int m = 0; m < kNumMaps; ++m) maps[(int64_t)m * S + s] = st.gated ? 0.f : st.maps[m];
   }
@@ -1612,21 +1617,22 @@ __device__ __forceinline__ void batch_item(const BatchDev& b, int c, int tile) {
     const double En = b.net_En[net];
     const int32_t* ml = b.ml + b.net_off[net];
     const float2* td = b.td + b.td_off[c];
-    (void)scan_lane<N, BLOCK / 64, true>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, b.REG1[c],
-                                         En, b.netCC, s, k, active,
-                                         (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
-                                         (b.flags & PYCWB_TILE_ENERGY_BOUND) != 0, nullptr,
-                                         &b.counts[c]);
+    (void)scan_lane<N, BLOCK / 64, true, false>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr,
+                                                b.REG1[c], En, b.netCC, s, k, active,
+                                                (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
+                                                (b.flags & PYCWB_TILE_ENERGY_BOUND) != 0,
+                                                nullptr, &b.counts[c]);
   } else {
     const double En = b.net_En[net];
     const float RR = (float)regulator_value(b.n_valid, b.counts[c], En);
     const int32_t* ml = b.ml + b.net_off[net];
     const float2* td = b.td + b.td_off[c];
     const unsigned long long key =
-        scan_lane<N, BLOCK / 64>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR, En, b.netCC, s,
-                                 k, active,
-                     (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
-                     (b.flags & PYCWB_TILE_ENERGY_BOUND) != 0, nullptr);
+        scan_lane<N, BLOCK / 64, false, false>(FP, FX, ml, b.S, rms, td, P, b.K[c], b.rr, RR,
+                                               En, b.netCC, s, k, active,
+                                               (b.flags & PYCWB_SKIP_MASKED_PIXELS) != 0,
+                                               (b.flags & PYCWB_TILE_ENERGY_BOUND) != 0,
+                                               nullptr);
     block_argmax<BLOCK>(key, &b.best[c]);
   }
 }
