@@ -954,7 +954,11 @@ __device__ __forceinline__ unsigned long long scan_lane(
   uint32_t toff[N];  // byte offset of each detector's tap within a pixel row
 #pragma unroll
   for (int i = 0; i < N; ++i) toff[i] = (uint32_t)tap[i] * (uint32_t)sizeof(float2);
+  // the empty asm keeps each offset a 32-bit register read at the load, so the
+  // load takes the scalar row base plus the offset (no 64-bit copies of the
+  // offsets held through the loop, no address arithmetic per candidate)
   auto tap_load = [&](const float2* row, int i) -> float2 {
+    asm volatile("" : "+v"(toff[i]));
     return *(const float2*)((const char*)row + toff[i]);
   };
   if (!skip_masked) {
