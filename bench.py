@@ -1107,9 +1107,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        log("multi-GPU runs are not part of this build (one GPU per box); use --gpus 1")
+        print(json.dumps({"error": "the likelihood path runs on one GPU; --gpus must be 1 "
+                          "(sharding by cluster is not implemented)", "n_gpus": args.gpus}))
+        log("multi-GPU runs are not implemented (clusters would shard by rank); use --gpus 1")
         return 2
 
     import torch

@@ -796,7 +796,7 @@ __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
 // contributions, and each owner adds its own contributions to its sums in
 // queue order, which is its pixel order (a direction enqueues at most one pair
 // per pixel, and pixels are enqueued in increasing order).  Pairs beyond the
-// first 64 move to the front and wait for the next round.  The sums therefore
+// first 64 move to the front and wait for the next flush.  The sums therefore
 // see exactly the terms of the per-lane sweep in the same order (bit-identical
 // results) while the expensive chain runs with every lane busy.
 constexpr int kQueue = 128;  // pair slots: at most 63 waiting + 64 of one pixel

@@ -2,8 +2,8 @@
 
 Bar: float32 per-pixel arrays produced by the Numba-typed stages (pixel mask,
 coherent energy, packet energies, DPF vectors, delayed data) bit-exact; the
-float64 NumPy-stage scalars within 1e-9 relative (the north-star bound for
-Lm/Ec/rho is 1e-5).
+float64 NumPy-stage scalars within 1e-9 relative (the path's stated tolerance
+for Lm/Ec/rho is 1e-5 relative).
 """
 
 import numpy as np
