@@ -47,6 +47,15 @@ __device__ __forceinline__ double ext_sample_s(const double* __restrict__ x, int
   return x[k];
 }
 
+// Sample of a shifted copy (SH) or of the series itself: without shifts the
+// kernels compile the plain mirror-padding read (no per-sample shift tests).
+template <bool SH>
+__device__ __forceinline__ double ext_sample_t(const double* __restrict__ x, int64_t L, int64_t k,
+                                               int64_t sh) {
+  if constexpr (SH) return ext_sample_s(x, L, k, sh);
+  else return ext_sample(x, L, k);
+}
+
 __device__ __forceinline__ unsigned bitrev(unsigned v, int bits) {
   return __brev(v) >> (32 - bits);
 }
@@ -275,7 +284,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly(
 // step of the sliding window; the FFT reads its twiddles from LDS.  KK =
 // (m_H - 1) / 2M is a template parameter (6 for the WDM filters of every level
 // here).
-template <int R, int KK>
+template <int R, int KK, bool SH>
 __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
     const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int M, int64_t N,
     const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int log2_2M, int B,
@@ -288,7 +297,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
   const int64_t n0 = (blk % tiles) * B;
   if (s >= n_series) return;
   const double* x = X + (int64_t)s * ldx;
-  const int64_t sh = shifts ? shifts[s] : 0;
+  const int64_t sh = SH ? shifts[s] : 0;
   double* lds = reinterpret_cast<double*>(buf);
   const int items = (B / R) * M2;
   for (int it = threadIdx.x; it < items; it += kThreads) {
@@ -309,7 +318,7 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
 #pragma unroll
     for (int u = 0; u < R + 4 * KK; ++u) {
       if (u % 8 == 0) __builtin_amdgcn_sched_barrier(0);
-      const double v = ext_sample_s(x, L, (nb0 - 2 * KK + u) * M + k, sh);
+      const double v = ext_sample_t<SH>(x, L, (nb0 - 2 * KK + u) * M + k, sh);
 #pragma unroll
       for (int b = 0; b < R; ++b) {
         const int d = u - b;
@@ -480,7 +489,7 @@ __device__ __forceinline__ void stream_fft(double2* __restrict__ buf, const Stre
   }
 }
 
-template <int KK, int LG, int R, int THR>
+template <int KK, int LG, int R, int THR, bool SH>
 __global__ __launch_bounds__(THR) void k_t2w_stream(
     const double* __restrict__ X, int64_t L, int64_t ldx, int n_series, int64_t N,
     const double* __restrict__ filt, int m_H, const double2* __restrict__ tw, int C,
@@ -503,13 +512,13 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
   const int64_t t0 = (blockIdx.x % chunks) * C;
   const int nt = (int)((tiles - t0) < C ? (tiles - t0) : C);
   const double* x = X + (int64_t)s * ldx;
-  const int64_t sh = shifts ? shifts[s] : 0;   // shifted copies of one series (ldx 0)
+  const int64_t sh = SH ? shifts[s] : 0;   // shifted copies of one series (ldx 0)
   double* lds = reinterpret_cast<double*>(buf);
   double* win = lds + 2 * B * M;
   double2* tws = reinterpret_cast<double2*>(win + WR * M);   // kRegTw false only
   const int64_t N0 = t0 * B;
   const int64_t base = (N0 - 2 * KK) * M;   // sample of ring row 0, column 0 at the start
-  for (int i = threadIdx.x; i < WR * M; i += THR) win[i] = ext_sample_s(x, L, base + i, sh);
+  for (int i = threadIdx.x; i < WR * M; i += THR) win[i] = ext_sample_t<SH>(x, L, base + i, sh);
   if constexpr (!kRegTw) {
     for (int i = threadIdx.x; i < M; i += THR) tws[i] = tw[i];
   }
@@ -535,7 +544,8 @@ __global__ __launch_bounds__(THR) void k_t2w_stream(
       // ring rows (t + 1) B + 4 KK + 1 .. + B - 1, relative to the chunk's first row
       const int64_t nb = base + ((int64_t)(t + 1) * B + 4 * KK + 1) * M;
 #pragma unroll
-      for (int p = 0; p < P; ++p) pre[p] = ext_sample_s(x, L, nb + threadIdx.x + p * THR, sh);
+      for (int p = 0; p < P; ++p)
+        pre[p] = ext_sample_t<SH>(x, L, nb + threadIdx.x + p * THR, sh);
     }
     // fold: sample u of the thread's column (ring row t B + r R + u + krow,
     // wrapping at most once) adds h[(u - b) / 2] x to every bin b of the same
@@ -628,9 +638,14 @@ template <int LG, int R, int THR>
 void launch_stream(unsigned grid, hipStream_t st, const double* x, int64_t L, int64_t ldx,
                    int n_series, int64_t N, const double* filt, int m_H, const double2* tw,
                    int C, int quadrature, double2* out, const int64_t* shifts) {
-  hipLaunchKernelGGL((k_t2w_stream<6, LG, R, THR>), dim3(grid), dim3(THR),
-                     stream_lds_bytes(1 << LG, 6, THR, R), st, x, L, ldx, n_series, N, filt, m_H,
-                     tw, C, quadrature, out, shifts);
+  if (shifts)
+    hipLaunchKernelGGL((k_t2w_stream<6, LG, R, THR, true>), dim3(grid), dim3(THR),
+                       stream_lds_bytes(1 << LG, 6, THR, R), st, x, L, ldx, n_series, N, filt,
+                       m_H, tw, C, quadrature, out, shifts);
+  else
+    hipLaunchKernelGGL((k_t2w_stream<6, LG, R, THR, false>), dim3(grid), dim3(THR),
+                       stream_lds_bytes(1 << LG, 6, THR, R), st, x, L, ldx, n_series, N, filt,
+                       m_H, tw, C, quadrature, out, shifts);
 }
 
 // The 1024-thread shape needs more than 64 KiB of dynamic LDS: allowed once per
@@ -638,7 +653,10 @@ void launch_stream(unsigned grid, hipStream_t st, const double* x, int64_t L, in
 bool stream_wide_ready() {
   static const bool ok = [] {
     const size_t need = stream_lds_bytes(kStreamWideM, 6, 1024, 4);
-    const bool set = hipFuncSetAttribute((const void*)k_t2w_stream<6, 9, 4, 1024>,
+    const bool set = hipFuncSetAttribute((const void*)k_t2w_stream<6, 9, 4, 1024, false>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)need) == hipSuccess &&
+                     hipFuncSetAttribute((const void*)k_t2w_stream<6, 9, 4, 1024, true>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)need) == hipSuccess;
     (void)hipGetLastError();
@@ -653,9 +671,14 @@ void launch_poly(dim3 grid, hipStream_t st, const double* x, int64_t L, int64_t 
                  int log2_2M, int B, int quadrature, double2* out, const int64_t* shifts) {
   const size_t lds = (size_t)B * M * sizeof(double2);
   if ((m_H - 1) / (2 * M) == 6) {
-    hipLaunchKernelGGL((k_t2w_poly_h<R, 6>), grid, dim3(kThreads), lds + M * sizeof(double2),
-                       st, x, L, ldx,
-                       n_series, M, N, filt, m_H, tw, log2_2M, B, quadrature, out, shifts);
+    if (shifts)
+      hipLaunchKernelGGL((k_t2w_poly_h<R, 6, true>), grid, dim3(kThreads),
+                         lds + M * sizeof(double2), st, x, L, ldx, n_series, M, N, filt, m_H, tw,
+                         log2_2M, B, quadrature, out, shifts);
+    else
+      hipLaunchKernelGGL((k_t2w_poly_h<R, 6, false>), grid, dim3(kThreads),
+                         lds + M * sizeof(double2), st, x, L, ldx, n_series, M, N, filt, m_H, tw,
+                         log2_2M, B, quadrature, out, shifts);
   } else {
     hipLaunchKernelGGL((k_t2w_poly<R>), grid, dim3(kThreads), lds, st, x, L, ldx, n_series, M,
                        N, filt, m_H, tw, log2_2M, B, quadrature, out);
