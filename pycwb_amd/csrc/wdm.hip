@@ -652,13 +652,18 @@ void launch_stream(unsigned grid, hipStream_t st, const double* x, int64_t L, in
 // process; false if the device refuses (the level then takes k_t2w_poly_h).
 bool stream_wide_ready() {
   static const bool ok = [] {
-    const size_t need = stream_lds_bytes(kStreamWideM, 6, 1024, 4);
-    const bool set = hipFuncSetAttribute((const void*)k_t2w_stream<6, 9, 4, 1024, false>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)need) == hipSuccess &&
-                     hipFuncSetAttribute((const void*)k_t2w_stream<6, 9, 4, 1024, true>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)need) == hipSuccess;
+    const int need = (int)stream_lds_bytes(kStreamWideM, 6, 1024, 4);
+    const int need256 = (int)stream_lds_bytes(256, 6, 1024, 8);
+    const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
+    const bool set =
+        hipFuncSetAttribute((const void*)k_t2w_stream<6, 9, 4, 1024, false>, a, need) ==
+            hipSuccess &&
+        hipFuncSetAttribute((const void*)k_t2w_stream<6, 9, 4, 1024, true>, a, need) ==
+            hipSuccess &&
+        hipFuncSetAttribute((const void*)k_t2w_stream<6, 8, 8, 1024, false>, a, need256) ==
+            hipSuccess &&
+        hipFuncSetAttribute((const void*)k_t2w_stream<6, 8, 8, 1024, true>, a, need256) ==
+            hipSuccess;
     (void)hipGetLastError();
     return set;
   }();
@@ -959,11 +964,12 @@ int t2w_impl(const double* x, int64_t n_samples, int64_t ldx, int n_series, int 
   const int64_t N = padded / stride;
   int log2_2M = 0;
   while ((1 << log2_2M) < 2 * M) ++log2_2M;
-  const bool wide = stride == M && (m_H - 1) == 12 * M && M == kStreamWideM && stream_wide_ready();
+  const bool wide = stride == M && (m_H - 1) == 12 * M && (M == kStreamWideM || M == 256) &&
+                    stream_wide_ready();
   if ((stride == M && (m_H - 1) == 12 * M && M >= 2 && M <= kStreamMaxM) || wide) {
     // streaming polyphase path: tiles of B bins (1024 / M, or 4 at M = 512), C
     // consecutive tiles per block
-    const int B = wide ? 4 : 1024 / M;
+    const int B = wide ? (M == 256 ? 16 : 4) : 1024 / M;
     const int64_t tiles = (N + B - 1) / B;
     const int64_t target = wide ? PYCWB_T2W_STREAM_BLOCKS / 4 : PYCWB_T2W_STREAM_BLOCKS;
     int64_t C = (tiles * n_series + target - 1) / target;
@@ -984,6 +990,7 @@ int t2w_impl(const double* x, int64_t n_samples, int64_t ldx, int n_series, int 
       case 5: launch_stream<5, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
       case 6: launch_stream<6, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
       case 7: launch_stream<7, 8, 256>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
+      case 8: launch_stream<8, 8, 1024>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
       default: launch_stream<9, 4, 1024>(g, st, x, n_samples, ldx, n_series, N, filter, m_H, tw, (int)C, quadrature, o, shifts); break;
     }
     hipError_t e = hipGetLastError();

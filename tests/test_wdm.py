@@ -106,9 +106,10 @@ def test_oracle_power_map_matches_quadrature(oracle):
                                    (256, 65536, 3), (512, 150000, 2), (1024, 200000, 1),
                                    # long enough for blocks that stream several tiles
                                    # through one window ring
-                                   (16, 2000003, 1), (512, 1100000, 1),
+                                   (16, 2000003, 1), (256, 1100000, 1), (512, 1100000, 1),
                                    # barely longer than the filter: one partial tile
-                                   (8, 100, 3), (128, 1601, 2), (512, 6200, 1)])
+                                   (8, 100, 3), (128, 1601, 2), (256, 3200, 1),
+                                   (512, 6200, 1)])
 def test_gpu_t2w_matches_oracle(oracle, M, L, S):
     import torch
 
