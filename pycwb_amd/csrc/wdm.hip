@@ -388,11 +388,13 @@ __global__ __launch_bounds__(kThreads) void k_t2w_poly_h(
 //   M <= 128:  THR = 256, R = 8 (tiles of 1,024 samples); the FFT twiddles of a
 //              thread's butterflies are kept in registers too (stream_fft) and
 //              the FFT buffer is in the swizzled layout (swz);
-//   M = 512:   THR = 1024, R = 4 (tiles of 4 bins, 156 KiB of LDS, one block
-//              per CU); the FFT is lds_fft with its twiddles in LDS, the buffer
-//              plain.
-// Series s may be a shifted copy of one series (shifts, ldx 0), read through
-// ext_sample_s.
+//   M = 256, 512: THR = 1024, R = 8 at M = 256 (tiles of 16 bins, 152 KiB of
+//              LDS) and R = 4 at M = 512 (tiles of 4 bins, 156 KiB), one block
+//              per CU; the FFT is lds_fft with its twiddles in LDS, the buffer
+//              plain (at M = 128 this shape measured slower than the 256-thread
+//              one).
+// With SH, series s is a shifted copy of one series (shifts, ldx 0), read
+// through ext_sample_s; without, the plain mirror-padded read.
 // The fold forms the same terms in the same order as k_t2w_poly_h (each window
 // sample is added to every bin that uses it as soon as it is read, so each bin
 // sums in ascending q), the FFT runs the same stages as lds_fft and the split
@@ -648,8 +650,8 @@ void launch_stream(unsigned grid, hipStream_t st, const double* x, int64_t L, in
                        m_H, tw, C, quadrature, out, shifts);
 }
 
-// The 1024-thread shape needs more than 64 KiB of dynamic LDS: allowed once per
-// process; false if the device refuses (the level then takes k_t2w_poly_h).
+// The 1024-thread shapes need more than 64 KiB of dynamic LDS: allowed once per
+// process; false if the device refuses (M = 256 and 512 then take k_t2w_poly_h).
 bool stream_wide_ready() {
   static const bool ok = [] {
     const int need = (int)stream_lds_bytes(kStreamWideM, 6, 1024, 4);
@@ -967,8 +969,8 @@ int t2w_impl(const double* x, int64_t n_samples, int64_t ldx, int n_series, int 
   const bool wide = stride == M && (m_H - 1) == 12 * M && (M == kStreamWideM || M == 256) &&
                     stream_wide_ready();
   if ((stride == M && (m_H - 1) == 12 * M && M >= 2 && M <= kStreamMaxM) || wide) {
-    // streaming polyphase path: tiles of B bins (1024 / M, or 4 at M = 512), C
-    // consecutive tiles per block
+    // streaming polyphase path: tiles of B bins (1024 / M with 256 threads; with
+    // 1024 threads 16 at M = 256 and 4 at M = 512), C consecutive tiles per block
     const int B = wide ? (M == 256 ? 16 : 4) : 1024 / M;
     const int64_t tiles = (N + B - 1) / B;
     const int64_t target = wide ? PYCWB_T2W_STREAM_BLOCKS / 4 : PYCWB_T2W_STREAM_BLOCKS;
