@@ -13,6 +13,7 @@ coherence_native projection / time-delay modules.
   gamma_to_gauss_device          (types/time_frequency_map.py:143-246,
       TimeFrequencyMap.Gamma2Gauss): counts, order statistics, sums and the
       transform on the GPU (pycwb_g2g_phase), the Gamma fit on the host;
+      gamma_to_gauss_device_many does several maps with shared host reads;
       gamma_to_gauss is the same on host arrays (used when a histogram is
       requested)
   max_energy                     (projection.py:63-124)
@@ -191,72 +192,125 @@ def gamma_to_gauss_device(energy: torch.Tensor, edge: float, wavelet_rate: int, 
     the Gamma fit on those scalars is the reference's host arithmetic.  The
     device log differs from NumPy's in the last bit for some values, so ALP
     and the map agree with the host version to about 1e-15 relative."""
-    T, M1 = (int(v) for v in energy.shape)
-    flat = energy.to(torch.float64).contiguous().reshape(-1)
-    n = int(flat.numel())
-    if n < 4:
-        return energy, 0.0
-    nL = int(float(edge or 0.0) * wavelet_rate * M1)
-    nL = max(0, min(nL, n - 2))
-    nR = n - nL - 1
-    if nR <= nL:
-        return energy, 0.0
-    dev = flat.device
-    win = nR - nL + 1
-    n_chunks = max(1, (nR - nL + 8191) // 8192)
-    count = torch.zeros(2, dtype=torch.int64, device=dev)
-    srt = torch.empty(win, dtype=torch.float64, device=dev)
-    n_valid = torch.zeros(1, dtype=torch.int64, device=dev)
-    chunk = torch.empty(2 * n_chunks, dtype=torch.float64, device=dev)
-    out = torch.empty_like(flat)
-    d = _lib.G2gDesc()
-    d.flat, d.n, d.nL, d.nR = ptr(flat), n, nL, nR
-    d.count, d.sorted, d.n_valid, d.chunk_sums, d.out = ptr(count), ptr(srt), ptr(n_valid), \
-        ptr(chunk), ptr(out)
-    nb = C.c_size_t()
-    _lib.check(lib().pycwb_g2g_work_bytes(C.byref(d), C.byref(nb)), "pycwb_g2g_work_bytes")
-    work = torch.empty(max(int(nb.value), 1), dtype=torch.uint8, device=dev)
-    d.work, d.work_bytes = ptr(work), int(nb.value)
+    return gamma_to_gauss_device_many([energy], edge, wavelet_rate, stream)[0]
+
+
+def gamma_to_gauss_device_many(energies, edge: float, wavelet_rate: int, stream=None):
+    """gamma_to_gauss_device of several maps (the detectors of one level), their
+    phases interleaved: each phase runs for every map still going, then one
+    read to the host serves them all.  Every map goes through the same kernels
+    and the same host arithmetic as a call of its own, so each result equals
+    the single-map call's; only the host round trips are shared."""
     stream = stream or torch.cuda.current_stream()
     st = stream_ptr(stream)
-    phase = lambda k: _lib.check(lib().pycwb_g2g_phase(C.byref(d), k, st), "pycwb_g2g_phase")
-    phase(1)
-    stream.synchronize()
-    wavecount_1 = int(count[0].item())
-    fff = (nR - nL) * wavecount_1 / float(n)
-    split = max(nL, min(nR - int(0.5 * fff), nR))
-    med = float(srt[split - nL].item())
-    if med <= 0.0:
-        return energy, 0.0
-    d.lo, d.hi = 0.01, 20.0 * med
-    phase(2)
-    stream.synchronize()
-    m = int(n_valid.item())
-    if m == 0:
-        return energy, 0.0
-    ch = chunk[:2 * ((m + 8191) // 8192)].cpu().numpy()
-    aaa = bbb = 0.0
-    for k in range(ch.size // 2):
-        aaa += float(ch[2 * k])
-        bbb += float(ch[2 * k + 1])
-    cnt = float(m)
-    alp = np.log(aaa / cnt) - bbb / cnt
-    if alp <= 0.0:
-        return energy, 0.0
-    alp = (3.0 - alp + np.sqrt((alp - 3.0) * (alp - 3.0) + 24.0 * alp)) / (12.0 * alp)
-    avr = med * (3.0 * alp + 0.2) / (3.0 * alp - 0.8)
-    ALP = med * alp / avr
-    d.alp, d.avr, d.ALP = float(alp), float(avr), float(ALP)
-    phase(3)
-    stream.synchronize()
-    wavecount_2 = int(count[1].item())
-    split = max(nL, min(nR - int(0.3173 * wavecount_2), nR))
-    qv = float(srt[split - nL].item())
-    if qv <= 0.0:
-        return energy, 0.0
-    d.scale = 1.0 / qv
-    phase(4)
-    return out.view(T, M1), float(ALP)
+    res = [None] * len(energies)
+    jobs = []
+    for idx, energy in enumerate(energies):
+        T, M1 = (int(v) for v in energy.shape)
+        flat = energy.to(torch.float64).contiguous().reshape(-1)
+        n = int(flat.numel())
+        if n < 4:
+            res[idx] = (energy, 0.0)
+            continue
+        nL = int(float(edge or 0.0) * wavelet_rate * M1)
+        nL = max(0, min(nL, n - 2))
+        nR = n - nL - 1
+        if nR <= nL:
+            res[idx] = (energy, 0.0)
+            continue
+        dev = flat.device
+        win = nR - nL + 1
+        n_chunks = max(1, (nR - nL + 8191) // 8192)
+        j = dict(idx=idx, energy=energy, T=T, M1=M1, flat=flat, n=n, nL=nL, nR=nR,
+                 count=torch.zeros(2, dtype=torch.int64, device=dev),
+                 srt=torch.empty(win, dtype=torch.float64, device=dev),
+                 n_valid=torch.zeros(1, dtype=torch.int64, device=dev),
+                 chunk=torch.empty(2 * n_chunks, dtype=torch.float64, device=dev),
+                 out=torch.empty_like(flat))
+        d = _lib.G2gDesc()
+        d.flat, d.n, d.nL, d.nR = ptr(flat), n, nL, nR
+        d.count, d.sorted, d.n_valid, d.chunk_sums, d.out = ptr(j["count"]), ptr(j["srt"]), \
+            ptr(j["n_valid"]), ptr(j["chunk"]), ptr(j["out"])
+        nb = C.c_size_t()
+        _lib.check(lib().pycwb_g2g_work_bytes(C.byref(d), C.byref(nb)), "pycwb_g2g_work_bytes")
+        j["work"] = torch.empty(max(int(nb.value), 1), dtype=torch.uint8, device=dev)
+        d.work, d.work_bytes = ptr(j["work"]), int(nb.value)
+        j["d"] = d
+        jobs.append(j)
+
+    def phase(j, k):
+        _lib.check(lib().pycwb_g2g_phase(C.byref(j["d"]), k, st), "pycwb_g2g_phase")
+
+    def fail(j):
+        res[j["idx"]] = (j["energy"], 0.0)
+
+    def host(vals):  # one device-to-host read of one scalar per job
+        stream.synchronize()
+        return torch.stack(vals).cpu().tolist()
+
+    for j in jobs:
+        phase(j, 1)
+    if jobs:
+        for j, c in zip(jobs, host([j["count"][0] for j in jobs])):
+            fff = (j["nR"] - j["nL"]) * int(c) / float(j["n"])
+            j["split"] = max(j["nL"], min(j["nR"] - int(0.5 * fff), j["nR"]))
+        meds = host([j["srt"][j["split"] - j["nL"]] for j in jobs])
+        live = []
+        for j, med in zip(jobs, meds):
+            med = float(med)
+            if med <= 0.0:
+                fail(j)
+                continue
+            j["med"] = med
+            j["d"].lo, j["d"].hi = 0.01, 20.0 * med
+            phase(j, 2)
+            live.append(j)
+        jobs = live
+    if jobs:
+        nv = [int(m) for m in host([j["n_valid"][0] for j in jobs])]
+        live = [(j, m) for j, m in zip(jobs, nv) if m > 0]
+        for j, m in zip(jobs, nv):
+            if m == 0:
+                fail(j)
+        jobs = []
+        if live:
+            parts = [j["chunk"][:2 * ((m + 8191) // 8192)] for j, m in live]
+            stream.synchronize()
+            allch = torch.cat(parts).cpu().numpy()
+            off = 0
+            for (j, m), part in zip(live, parts):
+                ch = allch[off:off + part.numel()]
+                off += part.numel()
+                aaa = bbb = 0.0
+                for k in range(ch.size // 2):
+                    aaa += float(ch[2 * k])
+                    bbb += float(ch[2 * k + 1])
+                cnt = float(m)
+                alp = np.log(aaa / cnt) - bbb / cnt
+                if alp <= 0.0:
+                    fail(j)
+                    continue
+                med = j["med"]
+                alp = (3.0 - alp + np.sqrt((alp - 3.0) * (alp - 3.0) + 24.0 * alp)) / (12.0 * alp)
+                avr = med * (3.0 * alp + 0.2) / (3.0 * alp - 0.8)
+                ALP = med * alp / avr
+                j["ALP"] = ALP
+                j["d"].alp, j["d"].avr, j["d"].ALP = float(alp), float(avr), float(ALP)
+                phase(j, 3)
+                jobs.append(j)
+    if jobs:
+        for j, c in zip(jobs, host([j["count"][1] for j in jobs])):
+            j["split2"] = max(j["nL"], min(j["nR"] - int(0.3173 * int(c)), j["nR"]))
+        qvs = host([j["srt"][j["split2"] - j["nL"]] for j in jobs])
+        for j, qv in zip(jobs, qvs):
+            qv = float(qv)
+            if qv <= 0.0:
+                fail(j)
+                continue
+            j["d"].scale = 1.0 / qv
+            phase(j, 4)
+            res[j["idx"]] = (j["out"].view(j["T"], j["M1"]), float(j["ALP"]))
+    return res
 
 
 def _series_of(tf_map, wdm, device):

@@ -146,7 +146,7 @@ def run(xd, ifos, rate, stream, order=8, up_tdf=4, td_size=12, bpp=0.001, seg_ed
     synchronised at each stage's end) and the cluster times."""
     from .batch import BatchedSkyScan, ClusterBatch, net_tables
     from .coherence_native.clustering import cluster_pixels_device
-    from .coherence_native.projection import gamma_to_gauss_device, max_energy_device
+    from .coherence_native.projection import gamma_to_gauss_device_many, max_energy_device
     from .coherence_native.selection import select_network_pixels_device, selection_cache_device
     from .coherence_native.veto_threshold import compute_threshold_device
     from .tdamp import TDBatchInputs
@@ -180,8 +180,8 @@ def run(xd, ifos, rate, stream, order=8, up_tdf=4, td_size=12, bpp=0.001, seg_ed
         add("max_energy", time.perf_counter() - t)
         t = time.perf_counter()
         maps, alp = [], 0.0
-        for e in energy:
-            g, a = gamma_to_gauss_device(e, seg_edge, int(round(1 / dt_bin)), stream=stream)
+        for g, a in gamma_to_gauss_device_many(energy, seg_edge, int(round(1 / dt_bin)),
+                                               stream=stream):
             maps.append(g)
             alp += a
         stack = torch.stack([g.t() for g in maps]).contiguous()      # (n, M+1, N)

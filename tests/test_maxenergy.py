@@ -199,3 +199,26 @@ def test_gpu_gamma_to_gauss_matches_host(shape, edge, zeros):
     z = torch.zeros((40, 9), dtype=torch.float64, device="cuda:0")
     same, a0 = gamma_to_gauss_device(z, 0.0, 16)
     assert a0 == 0.0 and same is z
+
+
+@pytest.mark.gpu
+def test_gpu_gamma_to_gauss_many_equals_single_calls():
+    """Several maps with shared host reads give each map's single-call result
+    bit for bit, also when one of them gives up (nothing above the floor)."""
+    import torch
+
+    from pycwb_amd.coherence_native.projection import (gamma_to_gauss_device,
+                                                       gamma_to_gauss_device_many)
+
+    print("case gamma-to-gauss many", flush=True)
+    rng = np.random.default_rng(5)
+    maps = [torch.from_numpy(np.ascontiguousarray(rng.gamma(1.7, 1.3, size=(m, t)).T))
+            .to("cuda:0") for m, t in ((33, 4000), (65, 3000), (9, 20000))]
+    maps.insert(1, torch.zeros((40, 9), dtype=torch.float64, device="cuda:0"))
+    many = gamma_to_gauss_device_many(maps, 1.0, 16)
+    for e, (g, a) in zip(maps, many):
+        g1, a1 = gamma_to_gauss_device(e, 1.0, 16)
+        assert a == a1
+        assert torch.equal(g, g1)
+    assert many[1][1] == 0.0 and many[1][0] is maps[1]
+
